@@ -1,0 +1,179 @@
+/*
+ * hci_gpu.h -- C ABI of the MI355X (gfx950) HCI variational hot path.
+ *
+ * The library builds the sparse variational Hamiltonian (connection search over
+ * alpha/beta determinant bitstrings + Slater-Condon elements) and runs the
+ * Davidson eigensolver on it.  It replaces, on the host side of the reference
+ * (aaholmes/risq):
+ *   - gen_sparse_ham_fast_lookup   src/var/fast_ham.rs:527-575   -> hci_build_h
+ *   - SparseMatUpperTri + MatrixOperations::matrix_vector_prod
+ *                                  src/var/sparse.rs:24-88,
+ *                                  src/var/eigenvalues/matrix_operations.rs:14-27
+ *                                                                -> hci_sigma
+ *   - Davidson::new (DPR, lowest root) as driven by sparse_optimize
+ *                                  src/var/eigenvalues/algorithms/davidson.rs:106-277,
+ *                                  src/var/davidson.rs:43-113    -> hci_davidson
+ *   - Ham::ham_diag / ham_off_diag_no_excite (element consumers: selection, PT2)
+ *                                  src/ham/mod.rs:92-124,256-303 -> hci_ham_diag,
+ *                                                                   hci_elements
+ *
+ * Conventions
+ *   - Every function returns 0 (HCI_OK) on success and a negative code on
+ *     failure; hci_davidson returns HCI_NOT_CONVERGED (1) when the iteration
+ *     limit is hit (the reference's Err(DavidsonError)).  No exception or abort
+ *     crosses the ABI.  hci_last_error gives the message of the context's last
+ *     failure.
+ *   - Determinants are 4 uint64 words each: [up_lo, up_hi, dn_lo, dn_hi]
+ *     (the reference's Config{up: u128, dn: u128}, src/wf/det.rs:16-19), bit p
+ *     = spatial orbital p.  All determinants of one call must carry the same
+ *     numbers of up and down electrons, and must be distinct.
+ *   - Integral tables use the reference layout (src/utils/ints.rs:38-68):
+ *     one_body[combine_2(p,q)], two_body[combine_4(p,q,r,s)], 1-based.
+ *   - Arrays passed in are borrowed for the duration of the call.  `mem`
+ *     arguments say where they live: HCI_MEM_HOST or HCI_MEM_DEVICE (device
+ *     pointers on the context's GPU).  Handles and device buffers are owned by
+ *     the library.  Calls on one context are serialised by the caller; the
+ *     library uses one HIP stream per context.
+ */
+#ifndef HCI_GPU_H
+#define HCI_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HCI_OK 0
+#define HCI_NOT_CONVERGED 1
+#define HCI_ERR_ARG (-1)
+#define HCI_ERR_HIP (-2)
+#define HCI_ERR_OOM (-3)
+#define HCI_ERR_STATE (-4)
+
+#define HCI_MEM_HOST 0
+#define HCI_MEM_DEVICE 1
+
+/* value storage of the sparse H */
+#define HCI_PREC_F64 64 /* int32 col + float64 value (12 B / nonzero) */
+#define HCI_PREC_F32 32 /* int32 col + float32 hi + float32 lo (hi read alone in
+                           mixed sigmas, hi+lo in refinement sigmas) */
+
+/* which singles are stored (doubles are always stored iff H != 0,
+ * src/var/off_diag.rs:33) */
+#define HCI_SINGLES_ABS_GT_1E12 0 /* src/var/fast_ham.rs:505,510 (n > 100 path) */
+#define HCI_SINGLES_NONZERO 1     /* src/var/off_diag.rs:33 (n <= 100 path) */
+
+/* sigma precision */
+#define HCI_SIGMA_FULL 0 /* stored values at full precision */
+#define HCI_SIGMA_F32 1  /* FP32 values (hi word), FP64 x/y and accumulation */
+
+/* Davidson correction */
+#define HCI_PRECOND_DPR 0 /* t = r / (lambda - H_ii), davidson.rs:391-408 */
+#define HCI_PRECOND_MLP 1 /* learned MLP, DPR fallback when it does not reduce |r| */
+
+typedef struct hci_ctx hci_ctx;
+typedef struct hci_csr hci_csr;
+
+typedef struct hci_build_opts {
+    int value_prec;  /* HCI_PREC_F64 | HCI_PREC_F32 */
+    int single_rule; /* HCI_SINGLES_* */
+    int mem;         /* where dets/diag live */
+} hci_build_opts;
+
+typedef struct hci_build_stats {
+    int64_t nnz_u;          /* stored upper-triangle off-diagonals */
+    int64_t n_alpha;        /* unique alpha strings */
+    int64_t n_beta;         /* unique beta strings */
+    int64_t alpha_pairs;    /* single-connected alpha-string pairs (ordered) */
+    double seconds;         /* wall time of the call (device work included) */
+} hci_build_stats;
+
+typedef struct hci_dav_opts {
+    double tol;        /* residual norm (reference coeff_eps = 1e-4, src/var/mod.rs:89) */
+    double etol;       /* eigenvalue change (reference energy_eps = 1e-8, src/var/mod.rs:90) */
+    int max_iters;     /* 50 (davidson.rs:75) */
+    int max_sub;       /* subspace limit, reference nvalues + 10 = 11 (davidson.rs:64-68) */
+    int precond;       /* HCI_PRECOND_* */
+    int mixed;         /* 0: full-precision sigmas; 1: FP32 sigmas, then FP64 refinement */
+    double refine_tol; /* mixed: residual below which refinement starts */
+    int mem;           /* where init / coeffs live */
+} hci_dav_opts;
+
+typedef struct hci_dav_stats {
+    int iters;          /* outer iterations */
+    int n_sigma_f32;    /* FP32-value sigma products */
+    int n_sigma_full;   /* full-precision sigma products */
+    int converged;
+    int mlp_applied;    /* iterations whose correction came from the MLP */
+    int mlp_fallbacks;  /* iterations where the MLP was rejected for DPR */
+    int refine_iter;    /* iteration at which refinement started (-1: never) */
+    double residual;    /* final |r| */
+    double seconds;
+} hci_dav_stats;
+
+/* context ---------------------------------------------------------------- */
+int hci_create(int device, hci_ctx** out);
+void hci_destroy(hci_ctx* ctx);
+const char* hci_last_error(const hci_ctx* ctx);
+int hci_device_sync(hci_ctx* ctx);
+
+/* Integrals (replaces Ham.ints, src/ham/read_ints.rs:33-42).  Copied into
+ * library-owned device memory; n1/n2 are the table lengths, at least the
+ * reference's allocation for n_orb (read_ints.rs:78-80).  n_orb <= 128. */
+int hci_set_integrals(hci_ctx* ctx, int n_orb, double e_core, const double* one_body,
+                      size_t n1, const double* two_body, size_t n2);
+
+/* Ham::ham_diag for n determinants (src/ham/mod.rs:92-124). */
+int hci_ham_diag(hci_ctx* ctx, const uint64_t* dets, int64_t n, double* diag, int mem);
+
+/* Ham::ham_off_diag_no_excite(det1[m], det2[m]) (src/ham/mod.rs:256-303);
+ * degree[m] (nullable) = excitation degree, 3 meaning "more than double". */
+int hci_elements(hci_ctx* ctx, const uint64_t* det1, const uint64_t* det2, int64_t m,
+                 double* value, int8_t* degree, int mem);
+
+/* Sparse H build (gen_sparse_ham_fast_lookup, src/var/fast_ham.rs:527-575).
+ * Rows in the caller's determinant order, upper triangle only, each row sorted
+ * by column with no duplicates (sparse.rs:44-56); diagonal taken from `diag`
+ * (new_sparse_ham / expand_sparse_ham_rows, src/wf/mod.rs:1646-1683).
+ * Incremental form: with prev != NULL (built over dets[0..n_prev)) and
+ * n_stored == n_prev, prev's rows are kept and only pairs with at least one
+ * index >= n_stored are added (fast_ham.rs:397,429,444-447,480).  With
+ * prev == NULL, n_stored must be 0. */
+int hci_build_h(hci_ctx* ctx, const uint64_t* dets, const double* diag, int64_t n,
+                const hci_csr* prev, int64_t n_stored, const hci_build_opts* opts,
+                hci_csr** out, hci_build_stats* stats);
+
+int hci_csr_info(const hci_csr* h, int64_t* n, int64_t* nnz_u, int* value_prec);
+/* copies to host arrays: row_ptr[n+1], col[nnz_u], val[nnz_u] (stored values
+ * widened to double: hi+lo for HCI_PREC_F32), diag[n]; any may be NULL */
+int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* val,
+                     double* diag);
+void hci_csr_free(hci_csr* h);
+
+/* y[:,c] = H x[:,c] for ncol column-major vectors of length n
+ * (matrix_vector_prod / matrix_matrix_prod, src/var/sparse.rs:64-88). */
+int hci_sigma(const hci_csr* h, int ncol, const double* x, double* y, int sigma_mode, int mem);
+
+/* Davidson, lowest root (Davidson::new via sparse_optimize).  init (nullable)
+ * is a length-n start vector (the warm start of src/var/davidson.rs:71-83);
+ * coeffs receives the eigenvector.  Returns HCI_OK, HCI_NOT_CONVERGED, or an
+ * error.  On non-convergence e0/coeffs are left unchanged (the reference keeps
+ * the previous coefficients, src/var/davidson.rs:109-111). */
+void hci_dav_default_opts(hci_dav_opts* opts);
+int hci_davidson(hci_ctx* ctx, const hci_csr* h, const double* init, const hci_dav_opts* opts,
+                 double* e0, double* coeffs, hci_dav_stats* stats);
+
+/* Device-memory and timing plumbing (no reference counterpart: for callers
+ * that keep inputs resident in HBM, and for timing on the context's stream).
+ * Events are 16 slots per context recorded on the context stream. */
+int hci_dev_alloc(hci_ctx* ctx, size_t bytes, void** ptr);
+int hci_dev_free(hci_ctx* ctx, void* ptr);
+int hci_memcpy(hci_ctx* ctx, void* dst, const void* src, size_t bytes, int dst_mem, int src_mem);
+int hci_event_record(hci_ctx* ctx, int slot);
+int hci_event_elapsed_ms(hci_ctx* ctx, int slot_a, int slot_b, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

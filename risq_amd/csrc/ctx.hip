@@ -1,0 +1,309 @@
+// Context, integral tables, per-determinant and per-pair element entry points.
+#include <cstring>
+
+#include "hci_internal.h"
+#include "mlp.h"
+
+int hci_fail(hci_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+namespace {
+
+template <class S>
+__global__ void __launch_bounds__(256) k_ham_diag(DevInts T, const uint64_t* dets, int64_t n,
+                                                  double* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t* w = dets + 4 * i;
+    out[i] = elem_diag<S>(T, load_string<S>(w), load_string<S>(w + 2));
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_elements(DevInts T, const uint64_t* a, const uint64_t* b,
+                                                  int64_t m, double* val, int8_t* deg) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    int d = 0;
+    const uint64_t* x = a + 4 * k;
+    const uint64_t* y = b + 4 * k;
+    val[k] = elem_generic<S>(T, load_string<S>(x), load_string<S>(x + 2), load_string<S>(y),
+                             load_string<S>(y + 2), &d);
+    if (deg) deg[k] = (int8_t)d;
+}
+
+// combine_2 / combine_4 (src/utils/ints.rs:38-68), 1-based
+inline size_t hc2(size_t i, size_t j) { return i > j ? (i * (i - 1)) / 2 + j : (j * (j - 1)) / 2 + i; }
+inline size_t hc4(size_t p, size_t q, size_t r, size_t s) { return hc2(hc2(p, q), hc2(r, s)); }
+
+// Bit check of the word layout against n_orb: words above n_orb must be zero.
+bool words_fit(const uint64_t* w, int64_t n, int n_orb) {
+    uint64_t lo_mask = n_orb >= 64 ? ~0ull : ((1ull << n_orb) - 1);
+    uint64_t hi_mask = n_orb <= 64 ? 0ull : (n_orb >= 128 ? ~0ull : ((1ull << (n_orb - 64)) - 1));
+    for (int64_t i = 0; i < n; ++i)
+        for (int s = 0; s < 2; ++s)
+            if ((w[4 * i + 2 * s] & ~lo_mask) || (w[4 * i + 2 * s + 1] & ~hi_mask)) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hci_create(int device, hci_ctx** out) {
+    if (!out) return HCI_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HCI_ERR_HIP;
+    if (device < 0 || device >= ndev) return HCI_ERR_ARG;
+    hci_ctx* c = new hci_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return HCI_ERR_HIP;
+    }
+    *out = c;
+    return HCI_OK;
+}
+
+void hci_destroy(hci_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    mlp_free_state(c);
+    hipFree(c->d_two);
+    hipFree(c->d_h1);
+    hipFree(c->d_J3);
+    hipFree(c->d_D3);
+    for (auto& e : c->events)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int hci_dev_alloc(hci_ctx* c, size_t bytes, void** ptr) {
+    if (!c || !ptr) return HCI_ERR_ARG;
+    *ptr = nullptr;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMalloc(ptr, bytes ? bytes : 16));
+    return HCI_OK;
+}
+
+int hci_dev_free(hci_ctx* c, void* ptr) {
+    if (!c) return HCI_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipFree(ptr));
+    return HCI_OK;
+}
+
+int hci_memcpy(hci_ctx* c, void* dst, const void* src, size_t bytes, int dst_mem, int src_mem) {
+    if (!c || (bytes && (!dst || !src))) return HCI_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipMemcpyKind k = dst_mem == HCI_MEM_DEVICE
+                          ? (src_mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice)
+                          : (src_mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost);
+    HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HCI_OK;
+}
+
+int hci_event_record(hci_ctx* c, int slot) {
+    if (!c || slot < 0 || slot >= 16) return HCI_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->events[slot]) HIP_TRY(c, hipEventCreate(&c->events[slot]));
+    HIP_TRY(c, hipEventRecord(c->events[slot], c->stream));
+    return HCI_OK;
+}
+
+int hci_event_elapsed_ms(hci_ctx* c, int a, int b, float* ms) {
+    if (!c || !ms || a < 0 || a >= 16 || b < 0 || b >= 16 || !c->events[a] || !c->events[b])
+        return c ? hci_fail(c, HCI_ERR_ARG, "hci_event_elapsed_ms: unrecorded slot") : HCI_ERR_ARG;
+    HIP_TRY(c, hipEventSynchronize(c->events[b]));
+    HIP_TRY(c, hipEventElapsedTime(ms, c->events[a], c->events[b]));
+    return HCI_OK;
+}
+
+const char* hci_last_error(const hci_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int hci_device_sync(hci_ctx* c) {
+    if (!c) return HCI_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HCI_OK;
+}
+
+int hci_set_integrals(hci_ctx* c, int n_orb, double e_core, const double* one_body, size_t n1,
+                      const double* two_body, size_t n2) {
+    if (!c) return HCI_ERR_ARG;
+    if (n_orb < 1 || n_orb > 128 || !one_body || !two_body)
+        return hci_fail(c, HCI_ERR_ARG, "hci_set_integrals: need 1 <= n_orb <= 128 and tables");
+    const size_t N = (size_t)n_orb;
+    const size_t need1 = hc2(N + 1, N + 1), need2 = hc4(N + 1, N + 1, N + 1, N + 1);
+    if (n1 < need1 || n2 < need2)
+        return hci_fail(c, HCI_ERR_ARG, "hci_set_integrals: tables shorter than the combine_2/"
+                                        "combine_4 layout for n_orb");
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::vector<double> h1(N * N), J3(N * N * N), D3(N * N * N);
+    for (size_t p = 0; p < N; ++p)
+        for (size_t q = 0; q < N; ++q) h1[p * N + q] = one_body[hc2(p + 1, q + 1)];
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j)
+            for (size_t k = 0; k < N; ++k) {
+                // Ham::direct(i,k,j,k) = (ij|kk); direct_plus_exchange(i,k,j,k) = (ij|kk) - (ik|kj)
+                double ijkk = two_body[hc4(i + 1, j + 1, k + 1, k + 1)];
+                double ikkj = two_body[hc4(i + 1, k + 1, k + 1, j + 1)];
+                J3[(i * N + j) * N + k] = ijkk;
+                D3[(i * N + j) * N + k] = ijkk - ikkj;
+            }
+    hipFree(c->d_two);
+    hipFree(c->d_h1);
+    hipFree(c->d_J3);
+    hipFree(c->d_D3);
+    c->d_two = c->d_h1 = c->d_J3 = c->d_D3 = nullptr;
+    HIP_TRY(c, hipMalloc(&c->d_two, need2 * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->d_h1, N * N * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->d_J3, N * N * N * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->d_D3, N * N * N * sizeof(double)));
+    HIP_TRY(c, hipMemcpy(c->d_two, two_body, need2 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_h1, h1.data(), N * N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_J3, J3.data(), N * N * N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_D3, D3.data(), N * N * N * sizeof(double), hipMemcpyHostToDevice));
+    c->n_orb = n_orb;
+    c->e_core = e_core;
+    c->n_two = need2;
+    return HCI_OK;
+}
+
+int hci_ham_diag(hci_ctx* c, const uint64_t* dets, int64_t n, double* diag, int mem) {
+    if (!c) return HCI_ERR_ARG;
+    if (!c->d_two) return hci_fail(c, HCI_ERR_STATE, "hci_ham_diag: integrals not set");
+    if (n < 0 || (n > 0 && (!dets || !diag))) return hci_fail(c, HCI_ERR_ARG, "hci_ham_diag: bad arguments");
+    if (n == 0) return HCI_OK;
+    if (mem == HCI_MEM_HOST && !words_fit(dets, n, c->n_orb))
+        return hci_fail(c, HCI_ERR_ARG, "hci_ham_diag: determinant bits beyond n_orb");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint64_t* d_dets = dets;
+    double* d_out = diag;
+    uint64_t* tmp_d = nullptr;
+    double* tmp_o = nullptr;
+    if (mem == HCI_MEM_HOST) {
+        HIP_TRY(c, hipMalloc(&tmp_d, (size_t)n * 32));
+        HIP_TRY(c, hipMalloc(&tmp_o, (size_t)n * 8));
+        HIP_TRY(c, hipMemcpyAsync(tmp_d, dets, (size_t)n * 32, hipMemcpyHostToDevice, c->stream));
+        d_dets = tmp_d;
+        d_out = tmp_o;
+    }
+    if (c->n_orb <= 64)
+        hipLaunchKernelGGL(k_ham_diag<uint64_t>, grid_for(n, 256), 256, 0, c->stream, c->ints(), d_dets, n, d_out);
+    else
+        hipLaunchKernelGGL(k_ham_diag<u128>, grid_for(n, 256), 256, 0, c->stream, c->ints(), d_dets, n, d_out);
+    HIP_TRY(c, hipGetLastError());
+    if (mem == HCI_MEM_HOST) {
+        HIP_TRY(c, hipMemcpyAsync(diag, tmp_o, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        hipFree(tmp_d);
+        hipFree(tmp_o);
+    }
+    return HCI_OK;
+}
+
+int hci_elements(hci_ctx* c, const uint64_t* det1, const uint64_t* det2, int64_t m, double* value,
+                 int8_t* degree, int mem) {
+    if (!c) return HCI_ERR_ARG;
+    if (!c->d_two) return hci_fail(c, HCI_ERR_STATE, "hci_elements: integrals not set");
+    if (m < 0 || (m > 0 && (!det1 || !det2 || !value)))
+        return hci_fail(c, HCI_ERR_ARG, "hci_elements: bad arguments");
+    if (m == 0) return HCI_OK;
+    if (mem == HCI_MEM_HOST && (!words_fit(det1, m, c->n_orb) || !words_fit(det2, m, c->n_orb)))
+        return hci_fail(c, HCI_ERR_ARG, "hci_elements: determinant bits beyond n_orb");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint64_t *a = det1, *b = det2;
+    double* v = value;
+    int8_t* dg = degree;
+    uint64_t *ta = nullptr, *tb = nullptr;
+    double* tv = nullptr;
+    int8_t* td = nullptr;
+    if (mem == HCI_MEM_HOST) {
+        HIP_TRY(c, hipMalloc(&ta, (size_t)m * 32));
+        HIP_TRY(c, hipMalloc(&tb, (size_t)m * 32));
+        HIP_TRY(c, hipMalloc(&tv, (size_t)m * 8));
+        if (degree) HIP_TRY(c, hipMalloc(&td, (size_t)m));
+        HIP_TRY(c, hipMemcpyAsync(ta, det1, (size_t)m * 32, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(tb, det2, (size_t)m * 32, hipMemcpyHostToDevice, c->stream));
+        a = ta;
+        b = tb;
+        v = tv;
+        dg = td;
+    }
+    if (c->n_orb <= 64)
+        hipLaunchKernelGGL(k_elements<uint64_t>, grid_for(m, 256), 256, 0, c->stream, c->ints(), a, b, m, v, dg);
+    else
+        hipLaunchKernelGGL(k_elements<u128>, grid_for(m, 256), 256, 0, c->stream, c->ints(), a, b, m, v, dg);
+    HIP_TRY(c, hipGetLastError());
+    if (mem == HCI_MEM_HOST) {
+        HIP_TRY(c, hipMemcpyAsync(value, tv, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream));
+        if (degree) HIP_TRY(c, hipMemcpyAsync(degree, td, (size_t)m, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        hipFree(ta);
+        hipFree(tb);
+        hipFree(tv);
+        hipFree(td);
+    }
+    return HCI_OK;
+}
+
+int hci_csr_info(const hci_csr* h, int64_t* n, int64_t* nnz_u, int* value_prec) {
+    if (!h) return HCI_ERR_ARG;
+    if (n) *n = h->n;
+    if (nnz_u) *nnz_u = h->nnz;
+    if (value_prec) *value_prec = h->prec;
+    return HCI_OK;
+}
+
+namespace {
+__global__ void k_widen(const float* hi, const float* lo, double* out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)hi[i] + (double)lo[i];
+}
+}  // namespace
+
+int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* val, double* diag) {
+    if (!h) return HCI_ERR_ARG;
+    hci_ctx* c = h->ctx;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (row_ptr) HIP_TRY(c, hipMemcpy(row_ptr, h->d_row_ptr, (size_t)(h->n + 1) * 8, hipMemcpyDeviceToHost));
+    if (col && h->nnz) HIP_TRY(c, hipMemcpy(col, h->d_col, (size_t)h->nnz * 4, hipMemcpyDeviceToHost));
+    if (diag && h->n) HIP_TRY(c, hipMemcpy(diag, h->d_diag, (size_t)h->n * 8, hipMemcpyDeviceToHost));
+    if (val && h->nnz) {
+        if (h->prec == HCI_PREC_F64) {
+            HIP_TRY(c, hipMemcpy(val, h->d_val, (size_t)h->nnz * 8, hipMemcpyDeviceToHost));
+        } else {
+            double* tmp = nullptr;
+            HIP_TRY(c, hipMalloc(&tmp, (size_t)h->nnz * 8));
+            hipLaunchKernelGGL(k_widen, grid_for(h->nnz, 256), 256, 0, c->stream, h->d_hi, h->d_lo, tmp, h->nnz);
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            HIP_TRY(c, hipMemcpy(val, tmp, (size_t)h->nnz * 8, hipMemcpyDeviceToHost));
+            hipFree(tmp);
+        }
+    }
+    return HCI_OK;
+}
+
+void hci_csr_free(hci_csr* h) {
+    if (!h) return;
+    hipSetDevice(h->ctx->device);
+    hipStreamSynchronize(h->ctx->stream);
+    hipFree(h->d_row_ptr);
+    hipFree(h->d_col);
+    hipFree(h->d_val);
+    hipFree(h->d_hi);
+    hipFree(h->d_lo);
+    hipFree(h->d_diag);
+    hipFree(h->d_chunk_row);
+    hipFree(h->d_rownorm);
+    delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,334 @@
+"""Python host binding of the MI355X HCI library (``librisq_hci.so``, C ABI in
+``include/hci_gpu.h``).
+
+The names mirror the reference's host interface so callers read the same:
+``gen_sparse_ham_fast_lookup`` (src/var/fast_ham.rs:527-575),
+``SparseMatUpperTri.matrix_vector_prod`` (src/var/sparse.rs:64-78),
+``Davidson`` / ``sparse_optimize`` (src/var/eigenvalues/algorithms/davidson.rs:106-277,
+src/var/davidson.rs:43-113).  Everything runs in the HIP library; there is no
+CPU fallback: if the library is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import weakref
+from dataclasses import dataclass
+
+import numpy as np
+
+from .fcidump import Integrals
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librisq_hci.so")
+
+HCI_OK = 0
+HCI_NOT_CONVERGED = 1
+MEM_HOST = 0
+MEM_DEVICE = 1
+PREC_F64 = 64
+PREC_F32 = 32
+SINGLES_ABS_GT_1E12 = 0
+SINGLES_NONZERO = 1
+SIGMA_FULL = 0
+SIGMA_F32 = 1
+PRECOND_DPR = 0
+PRECOND_MLP = 1
+
+
+class HciError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hci error {code}: {msg}")
+        self.code = code
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [("value_prec", C.c_int), ("single_rule", C.c_int), ("mem", C.c_int)]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("nnz_u", C.c_int64), ("n_alpha", C.c_int64), ("n_beta", C.c_int64),
+                ("alpha_pairs", C.c_int64), ("seconds", C.c_double)]
+
+
+class DavOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("etol", C.c_double), ("max_iters", C.c_int),
+                ("max_sub", C.c_int), ("precond", C.c_int), ("mixed", C.c_int),
+                ("refine_tol", C.c_double), ("mem", C.c_int)]
+
+
+class DavStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("n_sigma_f32", C.c_int), ("n_sigma_full", C.c_int),
+                ("converged", C.c_int), ("mlp_applied", C.c_int), ("mlp_fallbacks", C.c_int),
+                ("refine_iter", C.c_int), ("residual", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol include/hci_gpu.h declares: name -> (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "hci_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "hci_destroy": (None, [_P]),
+    "hci_last_error": (C.c_char_p, [_P]),
+    "hci_device_sync": (C.c_int, [_P]),
+    "hci_set_integrals": (C.c_int, [_P, C.c_int, C.c_double, _P, C.c_size_t, _P, C.c_size_t]),
+    "hci_ham_diag": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int]),
+    "hci_elements": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, C.c_int]),
+    "hci_build_h": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.c_int64, C.POINTER(BuildOpts),
+                              C.POINTER(_P), C.POINTER(BuildStats)]),
+    "hci_csr_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "hci_csr_download": (C.c_int, [_P, _P, _P, _P, _P]),
+    "hci_csr_free": (None, [_P]),
+    "hci_sigma": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.c_int]),
+    "hci_dav_default_opts": (None, [C.POINTER(DavOpts)]),
+    "hci_davidson": (C.c_int, [_P, _P, _P, C.POINTER(DavOpts), C.POINTER(C.c_double), _P,
+                               C.POINTER(DavStats)]),
+    "hci_dev_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "hci_dev_free": (C.c_int, [_P, _P]),
+    "hci_memcpy": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
+    "hci_event_record": (C.c_int, [_P, C.c_int]),
+    "hci_event_elapsed_ms": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library.  Raises if it is missing: there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class DeviceBuffer:
+    """Library-owned device allocation (inputs kept resident in HBM)."""
+
+    def __init__(self, ctx: "HciContext", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx._check(ctx.lib.hci_dev_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+        ctx._owned.add(self)
+
+    @classmethod
+    def from_array(cls, ctx: "HciContext", a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        ctx._check(ctx.lib.hci_memcpy(ctx.handle, b.ptr, _ptr(a), a.nbytes, MEM_DEVICE, MEM_HOST))
+        return b
+
+    def to_array(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        self.ctx._check(self.ctx.lib.hci_memcpy(self.ctx.handle, _ptr(out), self.ptr, out.nbytes,
+                                                MEM_HOST, MEM_DEVICE))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            self.ctx.lib.hci_dev_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HciContext:
+    """One GPU context (hci_ctx): integral tables, a HIP stream, the last error."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.hci_create(device, C.byref(h))
+        if rc != HCI_OK:
+            raise HciError(rc, f"hci_create(device={device}) failed (no GPU?)")
+        self.handle = h.value
+        self.ints: Integrals | None = None
+        # objects holding library handles of this context: freed before it
+        self._owned: weakref.WeakSet = weakref.WeakSet()
+
+    def close(self):
+        if self.handle:
+            for obj in list(self._owned):
+                obj.free()
+            self.lib.hci_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise HciError(rc, self.lib.hci_last_error(self.handle).decode())
+        return rc
+
+    # -- integrals and elements ------------------------------------------------
+    def set_integrals(self, ints: Integrals) -> None:
+        one = np.ascontiguousarray(ints.one_body, dtype=np.float64)
+        two = np.ascontiguousarray(ints.two_body, dtype=np.float64)
+        self._check(self.lib.hci_set_integrals(self.handle, ints.n_orb, ints.e_core, _ptr(one), one.size,
+                                               _ptr(two), two.size))
+        self.ints = ints
+
+    def ham_diag(self, dets: np.ndarray) -> np.ndarray:
+        dets = np.ascontiguousarray(dets, dtype=np.uint64)
+        out = np.empty(dets.shape[0])
+        self._check(self.lib.hci_ham_diag(self.handle, _ptr(dets), dets.shape[0], _ptr(out), MEM_HOST))
+        return out
+
+    def elements(self, det1: np.ndarray, det2: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        det1 = np.ascontiguousarray(det1, dtype=np.uint64)
+        det2 = np.ascontiguousarray(det2, dtype=np.uint64)
+        m = det1.shape[0]
+        val = np.empty(m)
+        deg = np.empty(m, dtype=np.int8)
+        self._check(self.lib.hci_elements(self.handle, _ptr(det1), _ptr(det2), m, _ptr(val), _ptr(deg),
+                                          MEM_HOST))
+        return val, deg
+
+    # -- timing ------------------------------------------------------------------
+    def record(self, slot: int) -> None:
+        self._check(self.lib.hci_event_record(self.handle, slot))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        self._check(self.lib.hci_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
+        return float(ms.value)
+
+    def sync(self) -> None:
+        self._check(self.lib.hci_device_sync(self.handle))
+
+    # -- sparse H --------------------------------------------------------------------
+    def build_h(self, dets, diag, n: int | None = None, prev: "SparseMatUpperTri | None" = None,
+                n_stored: int = 0, value_prec: int = PREC_F64,
+                single_rule: int = SINGLES_ABS_GT_1E12) -> "SparseMatUpperTri":
+        """hci_build_h.  dets/diag are numpy arrays (host) or DeviceBuffers."""
+        if isinstance(dets, DeviceBuffer):
+            mem = MEM_DEVICE
+            pd, pg = dets.ptr, diag.ptr
+            assert n is not None
+        else:
+            mem = MEM_HOST
+            dets = np.ascontiguousarray(dets, dtype=np.uint64)
+            diag = np.ascontiguousarray(diag, dtype=np.float64)
+            n = dets.shape[0]
+            pd, pg = _ptr(dets), _ptr(diag)
+        opts = BuildOpts(value_prec, single_rule, mem)
+        out = C.c_void_p()
+        st = BuildStats()
+        self._check(self.lib.hci_build_h(self.handle, pd, pg, n, prev.handle if prev else None,
+                                         n_stored, C.byref(opts), C.byref(out), C.byref(st)))
+        return SparseMatUpperTri(self, out.value, st)
+
+    def davidson(self, h: "SparseMatUpperTri", init: np.ndarray | None = None, tol: float = 1e-4,
+                 etol: float = 1e-8, max_iters: int = 50, max_sub: int = 11,
+                 precond: int = PRECOND_DPR, mixed: bool = False,
+                 refine_tol: float = 1e-3) -> "DavidsonResult":
+        opts = DavOpts()
+        self.lib.hci_dav_default_opts(C.byref(opts))
+        opts.tol, opts.etol, opts.max_iters, opts.max_sub = tol, etol, max_iters, max_sub
+        opts.precond, opts.mixed, opts.refine_tol, opts.mem = precond, int(mixed), refine_tol, MEM_HOST
+        coeffs = np.zeros(h.n)
+        e0 = C.c_double(0.0)
+        st = DavStats()
+        pin = None
+        if init is not None:
+            init = np.ascontiguousarray(init, dtype=np.float64)
+            pin = _ptr(init)
+        rc = self._check(self.lib.hci_davidson(self.handle, h.handle, pin, C.byref(opts), C.byref(e0),
+                                               _ptr(coeffs), C.byref(st)))
+        return DavidsonResult(rc == HCI_OK, e0.value, coeffs, st.as_dict())
+
+
+@dataclass
+class DavidsonResult:
+    converged: bool
+    energy: float
+    coeffs: np.ndarray
+    stats: dict
+
+
+class SparseMatUpperTri:
+    """Device-resident upper-triangle CSR (reference SparseMatUpperTri,
+    src/var/sparse.rs:24-36): diagonal + rows of (col > row, H_ij)."""
+
+    def __init__(self, ctx: HciContext, handle: int, stats: BuildStats):
+        self.ctx = ctx
+        self.handle = handle
+        n = C.c_int64()
+        nnz = C.c_int64()
+        prec = C.c_int()
+        ctx._check(ctx.lib.hci_csr_info(handle, C.byref(n), C.byref(nnz), C.byref(prec)))
+        self.n, self.nnz, self.prec = n.value, nnz.value, prec.value
+        self.build_stats = {k: getattr(stats, k) for k, _ in stats._fields_}
+        ctx._owned.add(self)
+
+    def free(self):
+        if self.handle and self.ctx.handle:
+            self.ctx.lib.hci_csr_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def download(self) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+        rp = np.empty(self.n + 1, dtype=np.int64)
+        col = np.empty(max(self.nnz, 1), dtype=np.int32)
+        val = np.empty(max(self.nnz, 1))
+        diag = np.empty(self.n)
+        self.ctx._check(self.ctx.lib.hci_csr_download(self.handle, _ptr(rp), _ptr(col), _ptr(val), _ptr(diag)))
+        return rp, col[:self.nnz], val[:self.nnz], diag
+
+    def matrix_vector_prod(self, x: np.ndarray, mode: int = SIGMA_FULL) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        ncol = 1 if x.ndim == 1 else x.shape[1]
+        xc = np.asfortranarray(x.reshape(self.n, ncol))
+        y = np.zeros((self.n, ncol), order="F")
+        self.ctx._check(self.ctx.lib.hci_sigma(self.handle, ncol, xc.ctypes.data, y.ctypes.data, mode, MEM_HOST))
+        return y[:, 0] if x.ndim == 1 else y
+
+    def sigma_device(self, x: DeviceBuffer, y: DeviceBuffer, mode: int = SIGMA_FULL) -> None:
+        self.ctx._check(self.ctx.lib.hci_sigma(self.handle, 1, x.ptr, y.ptr, mode, MEM_DEVICE))
+
+
+def gen_sparse_ham_fast_lookup(ctx: HciContext, dets: np.ndarray, diag: np.ndarray,
+                               prev: SparseMatUpperTri | None = None, n_stored: int = 0,
+                               value_prec: int = PREC_F64) -> SparseMatUpperTri:
+    """The reference's H build (src/var/fast_ham.rs:527-575), on the GPU."""
+    return ctx.build_h(dets, diag, prev=prev, n_stored=n_stored, value_prec=value_prec)
+
+
+def sparse_optimize(ctx: HciContext, dets: np.ndarray, diag: np.ndarray,
+                    prev: SparseMatUpperTri | None = None, n_stored: int = 0,
+                    init: np.ndarray | None = None, coeff_eps: float = 1e-4,
+                    energy_eps: float = 1e-8, **dav) -> tuple[SparseMatUpperTri, DavidsonResult]:
+    """Build H then run Davidson (reference src/var/davidson.rs:43-113).  The
+    n <= 100 small-system path of the reference stores singles iff H != 0."""
+    rule = SINGLES_NONZERO if dets.shape[0] <= 100 else SINGLES_ABS_GT_1E12
+    h = ctx.build_h(dets, diag, prev=prev, n_stored=n_stored, single_rule=rule)
+    res = ctx.davidson(h, init=init, tol=coeff_eps, etol=energy_eps, **dav)
+    return h, res
