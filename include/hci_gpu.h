@@ -59,6 +59,13 @@ extern "C" {
 #define HCI_PREC_F32 32 /* int32 col + float32 hi + float32 lo (hi read alone in
                            mixed sigmas, hi+lo in refinement sigmas) */
 
+/* how the matrix is held on the device (both keep the reference's pair set;
+ * rows/columns are renumbered internally in alpha-major string order) */
+#define HCI_STORE_FULL 0  /* both triangles: sigma without atomics, bitwise
+                             reproducible, 2x the value memory */
+#define HCI_STORE_UPPER 1 /* upper triangle only: sigma scatters the transpose
+                             half with FP64 atomics */
+
 /* which singles are stored (doubles are always stored iff H != 0,
  * src/var/off_diag.rs:33) */
 #define HCI_SINGLES_ABS_GT_1E12 0 /* src/var/fast_ham.rs:505,510 (n > 100 path) */
@@ -79,6 +86,7 @@ typedef struct hci_build_opts {
     int value_prec;  /* HCI_PREC_F64 | HCI_PREC_F32 */
     int single_rule; /* HCI_SINGLES_* */
     int mem;         /* where dets/diag live */
+    int storage;     /* HCI_STORE_FULL | HCI_STORE_UPPER */
 } hci_build_opts;
 
 typedef struct hci_build_stats {
@@ -110,6 +118,7 @@ typedef struct hci_dav_stats {
     int refine_iter;    /* iteration at which refinement started (-1: never) */
     double residual;    /* final |r| */
     double seconds;
+    double sigma_ms;    /* summed device time of the sigma products (HIP events) */
 } hci_dav_stats;
 
 /* context ---------------------------------------------------------------- */
@@ -145,6 +154,11 @@ int hci_build_h(hci_ctx* ctx, const uint64_t* dets, const double* diag, int64_t 
                 hci_csr** out, hci_build_stats* stats);
 
 int hci_csr_info(const hci_csr* h, int64_t* n, int64_t* nnz_u, int* value_prec);
+/* one row of the upper triangle in the caller's order: the columns j > row
+ * (ascending) and values of H[row, j]; *count receives the row length, at most
+ * cap entries are written.  HCI_STORE_FULL only. */
+int hci_csr_row(const hci_csr* h, int64_t row, int32_t* cols, double* vals, int64_t cap,
+                int64_t* count);
 /* copies to host arrays: row_ptr[n+1], col[nnz_u], val[nnz_u] (stored values
  * widened to double: hi+lo for HCI_PREC_F32), diag[n]; any may be NULL */
 int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* val,

@@ -72,11 +72,13 @@ def _u128(x: int) -> np.ndarray:
 
 
 def permute(a: int, b: int) -> int:
-    return lib().or_permute(_p(_u128(a)), _p(_u128(b)))
+    x, y = _u128(a), _u128(b)  # keep the buffers alive across the call
+    return lib().or_permute(_p(x), _p(y))
 
 
 def permute_2(a: int, b: int, v) -> int:
-    return lib().or_permute_2(_p(_u128(a)), _p(_u128(b)), _p(np.asarray(v, dtype=np.int32)))
+    x, y, w = _u128(a), _u128(b), np.asarray(v, dtype=np.int32)
+    return lib().or_permute_2(_p(x), _p(y), _p(w))
 
 
 class OracleHam:
@@ -166,7 +168,8 @@ class OracleSparse:
 def spmv_csr(rp, col, val, diag, x):
     n = rp.size - 1
     y = np.empty(n)
-    lib().or_spmv_csr(n, _p(rp), _p(col), _p(val), _p(diag), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(y))
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().or_spmv_csr(n, _p(rp), _p(col), _p(val), _p(diag), _p(x), _p(y))
     return y
 
 

@@ -18,8 +18,14 @@
 //        single-connected alpha groups : beta popcount(XOR) 2 -> opposite-spin double
 //      The three ranges are disjoint, so no pair is seen twice by one row.
 //      Kept pairs are compacted with a wave ballot + popcount prefix.
-//   4. a count pass sizes every row, a scan gives row_ptr, a fill pass writes
-//      the same sequence, and a segmented radix sort orders each row by column.
+//   4. a count pass sizes every row, a scan gives row_ptr, and a fill pass
+//      writes the same sequence straight into the final arrays.
+// Rows and columns are stored in the internal alpha-major order (the order of
+// step 1), so a row's columns come in a few contiguous runs (its own alpha
+// group, its neighbour groups) and sigma products gather x from few cache
+// lines.  FULL storage keeps both triangles (every pair in both rows), UPPER
+// keeps the pairs with p < q.  Elements are always evaluated with the
+// lower caller index first, as the reference does.
 //
 // Storage rules are the reference's: singles iff |H| > 1e-12 (fast_ham.rs:505,510)
 // or H != 0 for the n <= 100 path, doubles iff H != 0 (off_diag.rs:33).
@@ -43,30 +49,59 @@ struct BuildView {
     const int32_t* gb_of;   // beta group of each determinant
     const S* ua;            // unique alpha strings, ascending
     const int32_t* ga_off;  // alpha group g = alpha-sorted positions [ga_off[g], ga_off[g+1])
-    const S* sa_dn;         // beta string at each alpha-sorted position
-    const int32_t* sa_idx;  // determinant at each alpha-sorted position
+    const S* sa_dn;         // beta string at each alpha-sorted (= internal) position
+    const int32_t* sa_idx;  // caller index at each internal position
     const int32_t* gb_off;  // beta group ranges in beta-sorted order
     const S* sb_up;         // alpha string at each beta-sorted position
-    const int32_t* sb_idx;
+    const int32_t* sb_idx;  // caller index at each beta-sorted position
+    const int32_t* sb_pos;  // internal position at each beta-sorted position
     const int32_t* nbr_off;  // single-connected alpha groups of group g: nbr[nbr_off[g]..nbr_off[g+1])
-    const int32_t* nbr;
+    const int32_t* nbr;      // ascending within each group
 };
 
 __device__ __forceinline__ bool keep_single(double v, int rule) {
     return rule == HCI_SINGLES_NONZERO ? (v != 0.0) : (fabs(v) > 1e-12);
 }
 
-// One wavefront per alpha-sorted position p; the row is determinant i = sa_idx[p].
-// FILL == false: row_cnt[i] = number of kept pairs (j > i, j >= n_stored).
-// FILL == true : writes them at row_ptr[i] + prev_len(i) onwards.
-template <class S, bool FILL>
+template <class S>
+__device__ __forceinline__ void swap_if(bool c, S& a, S& b) {
+    const S t = a;
+    a = c ? b : a;
+    b = c ? t : b;
+}
+
+// Output of the fill pass: FP64 values, or FP32 hi + lo words.
+struct FillOut {
+    int32_t* col;
+    double* val;
+    float* hi;
+    float* lo;
+};
+
+__device__ __forceinline__ void put(const FillOut& o, int64_t e, int32_t q, double v) {
+    o.col[e] = q;
+    if (o.val) {
+        o.val[e] = v;
+    } else {
+        const float h = (float)v;
+        o.hi[e] = h;
+        o.lo[e] = (float)(v - (double)h);
+    }
+}
+
+// One wavefront per internal row p (caller determinant i = sa_idx[p]).
+// A candidate q (caller j) is kept iff its pair is connected, its element passes
+// the storage rule, max(i, j) >= n_stored, and (UPPER) p < q.
+// FILL == false: row_cnt[p] = kept + prev_cnt[p];  FILL == true: writes them at
+// row_ptr[p] + prev_cnt[p] onwards.
+template <class S, bool FILL, bool FULL>
 __global__ void __launch_bounds__(kBlock)
     k_connect(BuildView<S> B, DevInts T, int32_t n_stored, int single_rule, int64_t* row_cnt,
-              const int64_t* row_ptr, const int64_t* prev_row_ptr, int32_t n_prev,
-              int32_t* col_out, double* val_out) {
+              const int64_t* row_ptr, const int64_t* prev_cnt, FillOut out) {
     const int lane = threadIdx.x & 63;
-    const int64_t p = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (p >= B.N) return;  // wave-uniform
+    const int64_t p64 = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (p64 >= B.N) return;  // wave-uniform
+    const int32_t p = (int32_t)p64;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
     const int32_t i = B.sa_idx[p];
@@ -74,22 +109,15 @@ __global__ void __launch_bounds__(kBlock)
     const int32_t h = B.gb_of[i];
     const S up_i = B.ua[g];
     const S dn_i = B.sa_dn[p];
-    int64_t base = 0;
-    if (FILL) {
-        base = row_ptr[i];
-        if (prev_row_ptr && i < n_prev) base += prev_row_ptr[i + 1] - prev_row_ptr[i];
-    }
+    const int64_t pc = prev_cnt ? prev_cnt[p] : 0;
+    const int64_t base = FILL ? row_ptr[p] + pc : 0;
     int64_t cnt = 0;
 
-#define HCI_EMIT(keep, j, v)                                            \
-    do {                                                                \
-        const uint64_t _m = __ballot(keep);                             \
-        if (FILL && (keep)) {                                           \
-            const int64_t _o = base + cnt + __popcll(_m & lt_mask);    \
-            col_out[_o] = (j);                                          \
-            val_out[_o] = (v);                                          \
-        }                                                               \
-        cnt += __popcll(_m);                                            \
+#define HCI_EMIT(keep, q, v)                                         \
+    do {                                                             \
+        const uint64_t _m = __ballot(keep);                          \
+        if (FILL && (keep)) put(out, base + cnt + __popcll(_m & lt_mask), (q), (v)); \
+        cnt += __popcll(_m);                                         \
     } while (0)
 
     // (1) same alpha string: beta singles and beta doubles
@@ -99,47 +127,49 @@ __global__ void __launch_bounds__(kBlock)
             const int32_t q = qb + lane;
             bool keep = false;
             double v = 0.0;
-            int32_t j = 0;
-            if (q < q1) {
-                j = B.sa_idx[q];
-                if (j > i && j >= n_stored) {
-                    const S db = B.sa_dn[q];
-                    const int c = popc(dn_i ^ db);
+            if (q < q1 && (FULL || q > p)) {
+                const int32_t j = B.sa_idx[q];
+                if (max(i, j) >= n_stored) {
+                    S a = dn_i, b = B.sa_dn[q];
+                    const int c = popc(a ^ b);
+                    swap_if(j < i, a, b);
                     if (c == 2) {
-                        v = elem_single<S>(T, dn_i, db, up_i);
+                        v = elem_single<S>(T, a, b, up_i);
                         keep = keep_single(v, single_rule);
                     } else if (c == 4) {
-                        v = elem_same_double<S>(T, dn_i, db);
+                        v = elem_same_double<S>(T, a, b);
                         keep = (v != 0.0);
                     }
                 }
             }
-            HCI_EMIT(keep, j, v);
+            HCI_EMIT(keep, q, v);
         }
     }
     // (2) same beta string: alpha singles and alpha doubles
     {
         const int32_t q0 = B.gb_off[h], q1 = B.gb_off[h + 1];
         for (int32_t qb = q0; qb < q1; qb += 64) {
-            const int32_t q = qb + lane;
+            const int32_t qq = qb + lane;
             bool keep = false;
             double v = 0.0;
-            int32_t j = 0;
-            if (q < q1) {
-                j = B.sb_idx[q];
-                if (j > i && j >= n_stored) {
-                    const S ub = B.sb_up[q];
-                    const int c = popc(up_i ^ ub);
+            int32_t q = 0;
+            if (qq < q1) {
+                q = B.sb_pos[qq];
+                const int32_t j = B.sb_idx[qq];
+                if ((FULL || q > p) && max(i, j) >= n_stored) {
+                    S a = up_i, b = B.sb_up[qq];
+                    const int c = popc(a ^ b);
+                    swap_if(j < i, a, b);
                     if (c == 2) {
-                        v = elem_single<S>(T, up_i, ub, dn_i);
+                        v = elem_single<S>(T, a, b, dn_i);
                         keep = keep_single(v, single_rule);
                     } else if (c == 4) {
-                        v = elem_same_double<S>(T, up_i, ub);
+                        v = elem_same_double<S>(T, a, b);
                         keep = (v != 0.0);
                     }
                 }
             }
-            HCI_EMIT(keep, j, v);
+            HCI_EMIT(keep, q, v);
         }
     }
     // (3) alpha strings one electron away: opposite-spin doubles.  The ranges of
@@ -154,6 +184,7 @@ __global__ void __launch_bounds__(kBlock)
                 gid = B.nbr[nb + lane];
                 st = B.ga_off[gid];
                 sz = B.ga_off[gid + 1] - st;
+                if (!FULL && st + sz <= p + 1) sz = 0;  // UPPER: whole group below the row
             }
             int32_t inc = sz;
 #pragma unroll
@@ -175,33 +206,34 @@ __global__ void __launch_bounds__(kBlock)
                     else
                         hi = mid - 1;
                 }
+                // the largest lane whose prefix is <= f owns f (empty lanes
+                // share the prefix of the next lane and are never the largest)
                 const int32_t gL = __shfl(gid, lo);
                 const int32_t sL = __shfl(st, lo);
                 const int32_t eL = __shfl(exc, lo);
                 bool keep = false;
                 double v = 0.0;
-                int32_t j = 0;
+                int32_t q = 0;
                 if (f < total) {
-                    const int32_t q = sL + (f - eL);
-                    j = B.sa_idx[q];
-                    if (j > i && j >= n_stored) {
-                        const S db = B.sa_dn[q];
-                        if (popc(dn_i ^ db) == 2) {
-                            v = elem_opp_double<S>(T, up_i, B.ua[gL], dn_i, db);
+                    q = sL + (f - eL);
+                    const int32_t j = B.sa_idx[q];
+                    if ((FULL || q > p) && max(i, j) >= n_stored) {
+                        const S dj = B.sa_dn[q];
+                        if (popc(dn_i ^ dj) == 2) {
+                            S u1 = up_i, u2 = B.ua[gL], d1 = dn_i, d2 = dj;
+                            swap_if(j < i, u1, u2);
+                            swap_if(j < i, d1, d2);
+                            v = elem_opp_double<S>(T, u1, u2, d1, d2);
                             keep = (v != 0.0);
                         }
                     }
                 }
-                HCI_EMIT(keep, j, v);
+                HCI_EMIT(keep, q, v);
             }
         }
     }
 #undef HCI_EMIT
-    if (!FILL && lane == 0) {
-        int64_t c = cnt;
-        if (prev_row_ptr && i < n_prev) c += prev_row_ptr[i + 1] - prev_row_ptr[i];
-        row_cnt[i] = c;
-    }
+    if (!FILL && lane == 0) row_cnt[p] = cnt + pc;
 }
 
 // ---- sorting / bucketing helpers ------------------------------------------------
@@ -311,17 +343,48 @@ __global__ void k_nbr_fill(const int32_t* val, const int32_t* scan, const int32_
     }
 }
 
-// copy rows of a previous build in front of the new entries of each row
-__global__ void k_copy_prev(const int64_t* prp, const int32_t* pcol, const double* pval,
-                            const float* phi, const float* plo, int32_t n_prev,
-                            const int64_t* row_ptr, int32_t* col, double* val) {
+// Incremental form: entries of the previous build are renumbered from its
+// internal order to the new one through the caller indices.
+// FULL: previous row r becomes new row pos[pperm[r]] as a whole, in order.
+__global__ void k_prev_count_full(const int64_t* prp, const int32_t* pperm, int32_t n_prev,
+                                  const int32_t* pos, int64_t* prev_cnt) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_prev) return;
+    prev_cnt[pos[pperm[r]]] = prp[r + 1] - prp[r];
+}
+
+// UPPER: an entry lands in row min(p, q) of the new order.
+__global__ void k_prev_count_upper(const int64_t* prp, const int32_t* pcol, const int32_t* pperm,
+                                   int32_t n_prev, const int32_t* pos, int64_t* prev_cnt) {
     const int lane = threadIdx.x & 63;
     const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= n_prev) return;
-    const int64_t a = prp[r], b = prp[r + 1], dst = row_ptr[r];
-    for (int64_t e = a + lane; e < b; e += 64) {
-        col[dst + (e - a)] = pcol[e];
-        val[dst + (e - a)] = pval ? pval[e] : ((double)phi[e] + (double)plo[e]);
+    const int32_t p = pos[pperm[r]];
+    for (int64_t e = prp[r] + lane; e < prp[r + 1]; e += 64) {
+        const int32_t q = pos[pperm[pcol[e]]];
+        atomicAdd((unsigned long long*)&prev_cnt[min(p, q)], 1ull);
+    }
+}
+
+__global__ void k_prev_fill(const int64_t* prp, const int32_t* pcol, const double* pval,
+                            const float* phi, const float* plo, const int32_t* pperm, int32_t n_prev,
+                            const int32_t* pos, const int64_t* row_ptr, int64_t* cursor, bool full,
+                            FillOut out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (r >= n_prev) return;
+    const int32_t p = pos[pperm[r]];
+    const int64_t a = prp[r];
+    for (int64_t e = a + lane; e < prp[r + 1]; e += 64) {
+        const int32_t q = pos[pperm[pcol[e]]];
+        const double v = pval ? pval[e] : ((double)phi[e] + (double)plo[e]);
+        if (full) {
+            put(out, row_ptr[p] + (e - a), q, v);
+        } else {
+            const int32_t row = min(p, q);
+            const int64_t k = (int64_t)atomicAdd((unsigned long long*)&cursor[row], 1ull);
+            put(out, row_ptr[row] + k, max(p, q), v);
+        }
     }
 }
 
@@ -332,6 +395,21 @@ __global__ void k_split(const double* v, int64_t n, float* hi, float* lo) {
     const float h = (float)x;
     hi[e] = h;
     lo[e] = (float)(x - (double)h);
+}
+
+__global__ void k_inverse_perm(const int32_t* perm, int64_t n, int32_t* pos) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) pos[perm[p]] = (int32_t)p;
+}
+
+__global__ void k_gather_i32(const int32_t* idx, const int32_t* src, int64_t n, int32_t* dst) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[idx[p]];
+}
+
+__global__ void k_gather_f64(const int32_t* idx, const double* src, int64_t n, double* dst) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[idx[p]];
 }
 
 // ---- host helpers -----------------------------------------------------------------
@@ -445,6 +523,7 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     Scratch tmp;
     DevBufs db;
     const int n_orb = c->n_orb;
+    const bool full = (o.storage == HCI_STORE_FULL);
 
     S *up = nullptr, *dn = nullptr;
     int32_t* iota = nullptr;
@@ -473,35 +552,58 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     if (herr[0] & 1) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: determinants differ in electron counts");
     if (herr[0] & 2) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: determinant bits beyond n_orb");
 
-    // alpha-major and beta-major orders
+    // alpha-major (internal) and beta-major orders
     uint64_t *key = nullptr, *skey = nullptr;
-    int32_t *sa_idx = nullptr, *sb_idx = nullptr, *ga_off = nullptr, *gb_off = nullptr;
+    int32_t *sa_idx = nullptr, *sb_idx = nullptr, *sb_pos = nullptr, *pos = nullptr, *ga_off = nullptr,
+            *gb_off = nullptr;
     S *sa_dn = nullptr, *sb_up = nullptr;
     BT(db.alloc(&key, N));
     BT(db.alloc(&skey, N));
-    BT(db.alloc(&sa_idx, N));
     BT(db.alloc(&sb_idx, N));
+    BT(db.alloc(&sb_pos, N));
     BT(db.alloc(&sa_dn, N));
     BT(db.alloc(&sb_up, N));
     BT(db.alloc(&ga_off, (size_t)Ua + 1));
     BT(db.alloc(&gb_off, (size_t)Ub + 1));
+    hci_csr* H = new hci_csr();
+    H->ctx = c;
+    H->n = N;
+    H->prec = o.value_prec;
+    H->storage = o.storage;
+    auto fail_free = [&](int code) {
+        hci_csr_free(H);
+        return code;
+    };
+#define BTH(expr)                                                                                   \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess)                                                                       \
+            return fail_free(hci_fail(c, _e == hipErrorOutOfMemory ? HCI_ERR_OOM : HCI_ERR_HIP,     \
+                                      std::string("hci_build_h: ") + #expr + ": " + hipGetErrorString(_e))); \
+    } while (0)
+    BTH(hipMalloc(&H->d_perm, (size_t)N * 4 + 16));
+    BTH(hipMalloc(&H->d_pos, (size_t)N * 4 + 16));
+    sa_idx = H->d_perm;
+    pos = H->d_pos;
     const int32_t N32 = (int32_t)N;
-    BT(hipMemcpyAsync(ga_off + Ua, &N32, 4, hipMemcpyHostToDevice, s));
-    BT(hipMemcpyAsync(gb_off + Ub, &N32, 4, hipMemcpyHostToDevice, s));
+    BTH(hipMemcpy(ga_off + Ua, &N32, 4, hipMemcpyHostToDevice));
+    BTH(hipMemcpy(gb_off + Ub, &N32, 4, hipMemcpyHostToDevice));
     const int key_bits_a = 32 + bits_for(Ua);
     const int key_bits_b = 32 + bits_for(Ub);
     hipLaunchKernelGGL(k_pair_keys, grid_for(N, 256), 256, 0, s, ra, rb, N, key, iota);
-    BT(sort_pairs(tmp, key, skey, iota, sa_idx, (size_t)N, key_bits_a, s));
+    BTH(sort_pairs(tmp, key, skey, iota, sa_idx, (size_t)N, key_bits_a, s));
     hipLaunchKernelGGL(k_sorted_arrays<S>, grid_for(N, 256), 256, 0, s, skey, sa_idx, N, dn, sa_dn,
                        ga_off, d_err + 1);
     hipLaunchKernelGGL(k_pair_keys, grid_for(N, 256), 256, 0, s, rb, ra, N, key, iota);
-    BT(sort_pairs(tmp, key, skey, iota, sb_idx, (size_t)N, key_bits_b, s));
+    BTH(sort_pairs(tmp, key, skey, iota, sb_idx, (size_t)N, key_bits_b, s));
     hipLaunchKernelGGL(k_sorted_arrays<S>, grid_for(N, 256), 256, 0, s, skey, sb_idx, N, up, sb_up,
                        gb_off, d_err + 1);
-    BT(hipGetLastError());
-    BT(hipMemcpyAsync(herr, d_err, 8, hipMemcpyDeviceToHost, s));
-    BT(hipStreamSynchronize(s));
-    if (herr[1]) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: duplicate determinants");
+    hipLaunchKernelGGL(k_inverse_perm, grid_for(N, 256), 256, 0, s, sa_idx, N, pos);
+    hipLaunchKernelGGL(k_gather_i32, grid_for(N, 256), 256, 0, s, sb_idx, pos, N, sb_pos);
+    BTH(hipGetLastError());
+    BTH(hipMemcpyAsync(herr, d_err, 8, hipMemcpyDeviceToHost, s));
+    BTH(hipStreamSynchronize(s));
+    if (herr[1]) return fail_free(hci_fail(c, HCI_ERR_ARG, "hci_build_h: duplicate determinants"));
     db.release(key);
     db.release(skey);
 
@@ -509,58 +611,64 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     int n_alpha = 0;
     {
         uint64_t w[4];
-        BT(hipMemcpy(w, dets_dev, 32, hipMemcpyDeviceToHost));
+        BTH(hipMemcpy(w, dets_dev, 32, hipMemcpyDeviceToHost));
         n_alpha = __builtin_popcountll(w[0]) + __builtin_popcountll(w[1]);
     }
     const int64_t E = (int64_t)Ua * n_alpha;
     int32_t *nbr_off = nullptr, *nbr = nullptr;
     int32_t n_pairs = 0;
-    BT(db.alloc(&nbr_off, (size_t)Ua + 1));
-    BT(hipMemsetAsync(nbr_off, 0, ((size_t)Ua + 1) * 4, s));
+    BTH(db.alloc(&nbr_off, (size_t)Ua + 1));
+    BTH(hipMemsetAsync(nbr_off, 0, ((size_t)Ua + 1) * 4, s));
     if (E > 0) {
         S *k1 = nullptr, *k1s = nullptr;
         int32_t *v1 = nullptr, *v1s = nullptr, *flag = nullptr, *scan = nullptr, *run_start = nullptr,
-                *cnt = nullptr, *cursor = nullptr;
-        BT(db.alloc(&k1, E));
-        BT(db.alloc(&k1s, E));
-        BT(db.alloc(&v1, E));
-        BT(db.alloc(&v1s, E));
-        BT(db.alloc(&flag, E));
-        BT(db.alloc(&scan, E));
-        BT(db.alloc(&run_start, E + 1));
-        BT(db.alloc(&cnt, (size_t)Ua + 1));
-        BT(db.alloc(&cursor, (size_t)Ua));
+                *cnt = nullptr, *cursor = nullptr, *nbr_u = nullptr;
+        BTH(db.alloc(&k1, E));
+        BTH(db.alloc(&k1s, E));
+        BTH(db.alloc(&v1, E));
+        BTH(db.alloc(&v1s, E));
+        BTH(db.alloc(&flag, E));
+        BTH(db.alloc(&scan, E));
+        BTH(db.alloc(&run_start, E + 1));
+        BTH(db.alloc(&cnt, (size_t)Ua + 1));
+        BTH(db.alloc(&cursor, (size_t)Ua));
         hipLaunchKernelGGL(k_r1_keys<S>, grid_for(Ua, 256), 256, 0, s, ua, Ua, n_alpha, k1, v1);
-        BT(sort_pairs(tmp, k1, k1s, v1, v1s, (size_t)E, n_orb, s));
+        BTH(sort_pairs(tmp, k1, k1s, v1, v1s, (size_t)E, n_orb, s));
         hipLaunchKernelGGL(k_heads<S>, grid_for(E, 256), 256, 0, s, k1s, E, flag);
-        BT(inclusive_scan(tmp, flag, scan, (size_t)E, s));
+        BTH(inclusive_scan(tmp, flag, scan, (size_t)E, s));
         int32_t R = 0;
-        BT(hipMemcpyAsync(&R, scan + E - 1, 4, hipMemcpyDeviceToHost, s));
-        BT(hipStreamSynchronize(s));
+        BTH(hipMemcpyAsync(&R, scan + E - 1, 4, hipMemcpyDeviceToHost, s));
+        BTH(hipStreamSynchronize(s));
         const int32_t E32 = (int32_t)E;
-        BT(hipMemcpyAsync(run_start + R, &E32, 4, hipMemcpyHostToDevice, s));
+        BTH(hipMemcpy(run_start + R, &E32, 4, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_run_starts, grid_for(E, 256), 256, 0, s, flag, scan, E, run_start);
-        BT(hipMemsetAsync(cnt, 0, ((size_t)Ua + 1) * 4, s));
-        BT(hipMemsetAsync(cursor, 0, (size_t)Ua * 4, s));
+        BTH(hipMemsetAsync(cnt, 0, ((size_t)Ua + 1) * 4, s));
+        BTH(hipMemsetAsync(cursor, 0, (size_t)Ua * 4, s));
         hipLaunchKernelGGL(k_nbr_count, grid_for(E, 256), 256, 0, s, v1s, scan, run_start, E, cnt);
-        BT(exclusive_scan(tmp, cnt, nbr_off, (size_t)Ua + 1, s));
-        BT(hipMemcpyAsync(&n_pairs, nbr_off + Ua, 4, hipMemcpyDeviceToHost, s));
-        BT(hipStreamSynchronize(s));
-        BT(db.alloc(&nbr, (size_t)n_pairs + 1));
+        BTH(exclusive_scan(tmp, cnt, nbr_off, (size_t)Ua + 1, s));
+        BTH(hipMemcpyAsync(&n_pairs, nbr_off + Ua, 4, hipMemcpyDeviceToHost, s));
+        BTH(hipStreamSynchronize(s));
+        BTH(db.alloc(&nbr_u, (size_t)n_pairs + 1));
+        BTH(db.alloc(&nbr, (size_t)n_pairs + 1));
         hipLaunchKernelGGL(k_nbr_fill, grid_for(E, 256), 256, 0, s, v1s, scan, run_start, E, nbr_off,
-                           cursor, nbr);
-        BT(hipGetLastError());
-        db.release(k1);
-        db.release(k1s);
-        db.release(v1);
-        db.release(v1s);
-        db.release(flag);
-        db.release(scan);
-        db.release(run_start);
-        db.release(cnt);
-        db.release(cursor);
+                           cursor, nbr_u);
+        BTH(hipGetLastError());
+        // ascending neighbour lists: a deterministic enumeration order
+        if (n_pairs > 0) {
+            size_t need = 0;
+            const int gbits = bits_for(Ua);
+            BTH(rocprim::segmented_radix_sort_keys(nullptr, need, nbr_u, nbr, (size_t)n_pairs, (unsigned)Ua,
+                                                   nbr_off, nbr_off + 1, 0, gbits, s));
+            BTH(tmp.ensure(need));
+            BTH(rocprim::segmented_radix_sort_keys(tmp.p, need, nbr_u, nbr, (size_t)n_pairs, (unsigned)Ua,
+                                                   nbr_off, nbr_off + 1, 0, gbits, s));
+        }
+        BTH(hipStreamSynchronize(s));
+        for (void* q : {(void*)k1, (void*)k1s, (void*)v1, (void*)v1s, (void*)flag, (void*)scan,
+                        (void*)run_start, (void*)cnt, (void*)cursor, (void*)nbr_u})
+            db.release(q);
     } else {
-        BT(db.alloc(&nbr, 1));
+        BTH(db.alloc(&nbr, 1));
     }
 
     BuildView<S> B;
@@ -574,105 +682,122 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     B.gb_off = gb_off;
     B.sb_up = sb_up;
     B.sb_idx = sb_idx;
+    B.sb_pos = sb_pos;
     B.nbr_off = nbr_off;
     B.nbr = nbr;
     const DevInts T = c->ints();
-    const int64_t* prp = prev ? prev->d_row_ptr : nullptr;
+
+    // previous build (incremental form): per-row counts of its renumbered entries
+    int64_t* prev_cnt = nullptr;
     const int32_t n_prev = prev ? (int32_t)prev->n : 0;
+    if (prev) {
+        BTH(db.alloc(&prev_cnt, (size_t)N));
+        BTH(hipMemsetAsync(prev_cnt, 0, (size_t)N * 8, s));
+        if (n_prev > 0) {
+            if (full)
+                hipLaunchKernelGGL(k_prev_count_full, grid_for(n_prev, 256), 256, 0, s, prev->d_row_ptr,
+                                   prev->d_perm, n_prev, pos, prev_cnt);
+            else
+                hipLaunchKernelGGL(k_prev_count_upper, grid_for(n_prev, 4), 256, 0, s, prev->d_row_ptr,
+                                   prev->d_col, prev->d_perm, n_prev, pos, prev_cnt);
+        }
+    }
 
     // count pass
-    int64_t *row_cnt = nullptr, *row_ptr = nullptr;
-    BT(db.alloc(&row_cnt, (size_t)N + 1));
-    BT(hipMemsetAsync(row_cnt, 0, ((size_t)N + 1) * 8, s));
+    int64_t* row_cnt = nullptr;
+    BTH(db.alloc(&row_cnt, (size_t)N + 1));
+    BTH(hipMemsetAsync(row_cnt, 0, ((size_t)N + 1) * 8, s));
     const unsigned grid = grid_for(N, kWaves);
-    hipLaunchKernelGGL((k_connect<S, false>), grid, kBlock, 0, s, B, T, (int32_t)n_stored,
-                       o.single_rule, row_cnt, nullptr, prp, n_prev, nullptr, nullptr);
-    BT(hipGetLastError());
-    hci_csr* H = new hci_csr();
-    H->ctx = c;
-    H->n = N;
-    H->prec = o.value_prec;
-    auto fail_free = [&](int code) {
-        hci_csr_free(H);
-        return code;
-    };
-    if (hipMalloc(&H->d_row_ptr, ((size_t)N + 1) * 8) != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_OOM, "hci_build_h: row_ptr allocation"));
-    row_ptr = H->d_row_ptr;
-    if (exclusive_scan(tmp, row_cnt, row_ptr, (size_t)N + 1, s) != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_HIP, "hci_build_h: scan"));
+    FillOut none{nullptr, nullptr, nullptr, nullptr};
+    if (full)
+        hipLaunchKernelGGL((k_connect<S, false, true>), grid, kBlock, 0, s, B, T, (int32_t)n_stored,
+                           o.single_rule, row_cnt, nullptr, prev_cnt, none);
+    else
+        hipLaunchKernelGGL((k_connect<S, false, false>), grid, kBlock, 0, s, B, T, (int32_t)n_stored,
+                           o.single_rule, row_cnt, nullptr, prev_cnt, none);
+    BTH(hipGetLastError());
+    BTH(hipMalloc(&H->d_row_ptr, ((size_t)N + 1) * 8));
+    int64_t* row_ptr = H->d_row_ptr;
+    BTH(exclusive_scan(tmp, row_cnt, row_ptr, (size_t)N + 1, s));
     int64_t nnz = 0;
-    hipMemcpyAsync(&nnz, row_ptr + N, 8, hipMemcpyDeviceToHost, s);
-    if (hipStreamSynchronize(s) != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_HIP, "hci_build_h: count pass failed"));
-    H->nnz = nnz;
+    BTH(hipMemcpyAsync(&nnz, row_ptr + N, 8, hipMemcpyDeviceToHost, s));
+    BTH(hipStreamSynchronize(s));
     db.release(row_cnt);
+    H->nnz_s = nnz;
+    H->nnz_u = full ? nnz / 2 : nnz;
 
-    // fill pass into unsorted rows
-    int32_t* col_u = nullptr;
-    double* val_u = nullptr;
-    if (db.alloc(&col_u, (size_t)nnz + 1) != hipSuccess || db.alloc(&val_u, (size_t)nnz + 1) != hipSuccess ||
-        hipMalloc(&H->d_col, (size_t)nnz * 4 + 16) != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_OOM, "hci_build_h: CSR allocation"));
-    if (prev && n_prev > 0) {
-        hipLaunchKernelGGL(k_copy_prev, grid_for(n_prev, 4), 256, 0, s, prp, prev->d_col, prev->d_val,
-                           prev->d_hi, prev->d_lo, n_prev, row_ptr, col_u, val_u);
+    // fill: straight into the final arrays, except an incremental UPPER build,
+    // whose renumbered rows need a sort to become deterministic
+    const bool sort_rows = prev && !full;
+    BTH(hipMalloc(&H->d_col, (size_t)nnz * 4 + 16));
+    FillOut fo{H->d_col, nullptr, nullptr, nullptr};
+    int32_t* col_tmp = nullptr;
+    double* val_tmp = nullptr;
+    if (sort_rows) {
+        BTH(db.alloc(&col_tmp, (size_t)nnz + 1));
+        BTH(db.alloc(&val_tmp, (size_t)nnz + 1));
+        fo = FillOut{col_tmp, val_tmp, nullptr, nullptr};
+    } else if (o.value_prec == HCI_PREC_F64) {
+        BTH(hipMalloc(&H->d_val, (size_t)nnz * 8 + 16));
+        fo.val = H->d_val;
+    } else {
+        BTH(hipMalloc(&H->d_hi, (size_t)nnz * 4 + 16));
+        BTH(hipMalloc(&H->d_lo, (size_t)nnz * 4 + 16));
+        fo.hi = H->d_hi;
+        fo.lo = H->d_lo;
     }
-    hipLaunchKernelGGL((k_connect<S, true>), grid, kBlock, 0, s, B, T, (int32_t)n_stored,
-                       o.single_rule, nullptr, row_ptr, prp, n_prev, col_u, val_u);
-    if (hipGetLastError() != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_HIP, "hci_build_h: fill launch"));
-    // release search structures before the sort
-    db.release(up);
-    db.release(dn);
-    db.release(sa_dn);
-    db.release(sb_up);
-    db.release(sb_idx);
-    db.release(nbr);
-
-    // per-row sort by column
-    double* val_s = nullptr;
-    if (hipMalloc(&val_s, (size_t)nnz * 8 + 16) != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_OOM, "hci_build_h: sort buffer"));
-    if (nnz > 0) {
+    if (prev && n_prev > 0) {
+        int64_t* cursor = nullptr;
+        if (!full) {
+            BTH(db.alloc(&cursor, (size_t)N));
+            BTH(hipMemsetAsync(cursor, 0, (size_t)N * 8, s));
+        }
+        hipLaunchKernelGGL(k_prev_fill, grid_for(n_prev, 4), 256, 0, s, prev->d_row_ptr, prev->d_col,
+                           prev->d_val, prev->d_hi, prev->d_lo, prev->d_perm, n_prev, pos, row_ptr, cursor,
+                           full, fo);
+    }
+    if (full)
+        hipLaunchKernelGGL((k_connect<S, true, true>), grid, kBlock, 0, s, B, T, (int32_t)n_stored,
+                           o.single_rule, nullptr, row_ptr, prev_cnt, fo);
+    else
+        hipLaunchKernelGGL((k_connect<S, true, false>), grid, kBlock, 0, s, B, T, (int32_t)n_stored,
+                           o.single_rule, nullptr, row_ptr, prev_cnt, fo);
+    BTH(hipGetLastError());
+    if (sort_rows && nnz > 0) {
+        double* val_s = nullptr;
+        BTH(db.alloc(&val_s, (size_t)nnz + 1));
         size_t need = 0;
         const int cbits = bits_for(N);
-        rocprim::segmented_radix_sort_pairs(nullptr, need, col_u, H->d_col, val_u, val_s, (size_t)nnz,
-                                            (unsigned)N, row_ptr, row_ptr + 1, 0, cbits, s);
-        if (tmp.ensure(need) != hipSuccess) {
-            hipFree(val_s);
-            return fail_free(hci_fail(c, HCI_ERR_OOM, "hci_build_h: sort scratch"));
+        BTH(rocprim::segmented_radix_sort_pairs(nullptr, need, col_tmp, H->d_col, val_tmp, val_s, (size_t)nnz,
+                                                (unsigned)N, row_ptr, row_ptr + 1, 0, cbits, s));
+        BTH(tmp.ensure(need));
+        BTH(rocprim::segmented_radix_sort_pairs(tmp.p, need, col_tmp, H->d_col, val_tmp, val_s, (size_t)nnz,
+                                                (unsigned)N, row_ptr, row_ptr + 1, 0, cbits, s));
+        if (o.value_prec == HCI_PREC_F64) {
+            BTH(hipMalloc(&H->d_val, (size_t)nnz * 8 + 16));
+            BTH(hipMemcpyAsync(H->d_val, val_s, (size_t)nnz * 8, hipMemcpyDeviceToDevice, s));
+        } else {
+            BTH(hipMalloc(&H->d_hi, (size_t)nnz * 4 + 16));
+            BTH(hipMalloc(&H->d_lo, (size_t)nnz * 4 + 16));
+            hipLaunchKernelGGL(k_split, grid_for(nnz, 256), 256, 0, s, val_s, nnz, H->d_hi, H->d_lo);
         }
-        if (rocprim::segmented_radix_sort_pairs(tmp.p, need, col_u, H->d_col, val_u, val_s, (size_t)nnz,
-                                                (unsigned)N, row_ptr, row_ptr + 1, 0, cbits, s) !=
-            hipSuccess) {
-            hipFree(val_s);
-            return fail_free(hci_fail(c, HCI_ERR_HIP, "hci_build_h: segmented sort"));
+        BTH(hipStreamSynchronize(s));
+    } else if (sort_rows) {
+        if (o.value_prec == HCI_PREC_F64)
+            BTH(hipMalloc(&H->d_val, 16));
+        else {
+            BTH(hipMalloc(&H->d_hi, 16));
+            BTH(hipMalloc(&H->d_lo, 16));
         }
     }
-    db.release(col_u);
-    db.release(val_u);
-    if (o.value_prec == HCI_PREC_F32) {
-        if (hipMalloc(&H->d_hi, (size_t)nnz * 4 + 16) != hipSuccess ||
-            hipMalloc(&H->d_lo, (size_t)nnz * 4 + 16) != hipSuccess) {
-            hipFree(val_s);
-            return fail_free(hci_fail(c, HCI_ERR_OOM, "hci_build_h: FP32 value allocation"));
-        }
-        if (nnz > 0) hipLaunchKernelGGL(k_split, grid_for(nnz, 256), 256, 0, s, val_s, nnz, H->d_hi, H->d_lo);
-        hipStreamSynchronize(s);
-        hipFree(val_s);
-    } else {
-        H->d_val = val_s;
-    }
-    if (hipMalloc(&H->d_diag, (size_t)N * 8 + 16) != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_OOM, "hci_build_h: diag allocation"));
-    hipMemcpyAsync(H->d_diag, diag_dev, (size_t)N * 8, hipMemcpyDeviceToDevice, s);
+    BTH(hipMalloc(&H->d_diag, (size_t)N * 8 + 16));
+    hipLaunchKernelGGL(k_gather_f64, grid_for(N, 256), 256, 0, s, sa_idx, diag_dev, N, H->d_diag);
     int prc = spmv_prepare(H);
     if (prc) return fail_free(prc);
-    if (hipStreamSynchronize(s) != hipSuccess)
-        return fail_free(hci_fail(c, HCI_ERR_HIP, "hci_build_h: device error"));
+    BTH(hipStreamSynchronize(s));
+#undef BTH
     if (stats) {
-        stats->nnz_u = nnz;
+        stats->nnz_u = H->nnz_u;
         stats->n_alpha = Ua;
         stats->n_beta = Ub;
         stats->alpha_pairs = n_pairs;
@@ -698,10 +823,14 @@ extern "C" int hci_build_h(hci_ctx* c, const uint64_t* dets, const double* diag,
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: value_prec must be 64 or 32");
     if (opts->single_rule != HCI_SINGLES_ABS_GT_1E12 && opts->single_rule != HCI_SINGLES_NONZERO)
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: bad single_rule");
+    if (opts->storage != HCI_STORE_FULL && opts->storage != HCI_STORE_UPPER)
+        return hci_fail(c, HCI_ERR_ARG, "hci_build_h: bad storage");
     if (prev) {
         if (prev->ctx != c) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: prev from another context");
         if (n_stored != prev->n || n < prev->n)
             return hci_fail(c, HCI_ERR_ARG, "hci_build_h: incremental build needs n_stored == prev n <= n");
+        if (prev->storage != opts->storage)
+            return hci_fail(c, HCI_ERR_ARG, "hci_build_h: prev has another storage kind");
     } else if (n_stored != 0) {
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: n_stored must be 0 without prev");
     }

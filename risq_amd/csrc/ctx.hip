@@ -1,4 +1,5 @@
 // Context, integral tables, per-determinant and per-pair element entry points.
+#include <algorithm>
 #include <cstring>
 
 #include "hci_internal.h"
@@ -256,37 +257,146 @@ int hci_elements(hci_ctx* c, const uint64_t* det1, const uint64_t* det2, int64_t
 int hci_csr_info(const hci_csr* h, int64_t* n, int64_t* nnz_u, int* value_prec) {
     if (!h) return HCI_ERR_ARG;
     if (n) *n = h->n;
-    if (nnz_u) *nnz_u = h->nnz;
+    if (nnz_u) *nnz_u = h->nnz_u;
     if (value_prec) *value_prec = h->prec;
     return HCI_OK;
 }
 
+}  // extern "C"
+
 namespace {
+__global__ void k_gather_perm(const int32_t* perm, const int32_t* idx, int64_t n, int32_t* out) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = perm[idx[k]];
+}
+
 __global__ void k_widen(const float* hi, const float* lo, double* out, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = (double)hi[i] + (double)lo[i];
 }
+
+// internal arrays -> host (values widened to double)
+int fetch_internal(const hci_csr* h, std::vector<int64_t>& rp, std::vector<int32_t>& col,
+                   std::vector<double>& val, std::vector<int32_t>& perm) {
+    hci_ctx* c = h->ctx;
+    rp.resize((size_t)h->n + 1);
+    col.resize((size_t)h->nnz_s);
+    val.resize((size_t)h->nnz_s);
+    perm.resize((size_t)h->n);
+    HIP_TRY(c, hipMemcpy(rp.data(), h->d_row_ptr, rp.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(perm.data(), h->d_perm, perm.size() * 4, hipMemcpyDeviceToHost));
+    if (h->nnz_s == 0) return HCI_OK;
+    HIP_TRY(c, hipMemcpy(col.data(), h->d_col, col.size() * 4, hipMemcpyDeviceToHost));
+    if (h->prec == HCI_PREC_F64) {
+        HIP_TRY(c, hipMemcpy(val.data(), h->d_val, val.size() * 8, hipMemcpyDeviceToHost));
+    } else {
+        double* tmp = nullptr;
+        HIP_TRY(c, hipMalloc(&tmp, val.size() * 8));
+        hipLaunchKernelGGL(k_widen, grid_for(h->nnz_s, 256), 256, 0, c->stream, h->d_hi, h->d_lo, tmp, h->nnz_s);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(val.data(), tmp, val.size() * 8, hipMemcpyDeviceToHost);
+        hipFree(tmp);
+        HIP_TRY(c, e);
+    }
+    return HCI_OK;
+}
 }  // namespace
 
+extern "C" {
+
+// Caller-order upper triangle: row i holds (j, H_ij) for j > i, ascending j
+// (the reference's SparseMatUpperTri after sort_remove_duplicates).
 int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* val, double* diag) {
     if (!h) return HCI_ERR_ARG;
     hci_ctx* c = h->ctx;
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (row_ptr) HIP_TRY(c, hipMemcpy(row_ptr, h->d_row_ptr, (size_t)(h->n + 1) * 8, hipMemcpyDeviceToHost));
-    if (col && h->nnz) HIP_TRY(c, hipMemcpy(col, h->d_col, (size_t)h->nnz * 4, hipMemcpyDeviceToHost));
-    if (diag && h->n) HIP_TRY(c, hipMemcpy(diag, h->d_diag, (size_t)h->n * 8, hipMemcpyDeviceToHost));
-    if (val && h->nnz) {
+    std::vector<int64_t> rp;
+    std::vector<int32_t> ic, perm;
+    std::vector<double> iv;
+    int rc = fetch_internal(h, rp, ic, iv, perm);
+    if (rc) return rc;
+    const int64_t n = h->n;
+    std::vector<int64_t> cnt((size_t)n + 1, 0);
+    auto each = [&](auto f) {
+        for (int64_t p = 0; p < n; ++p)
+            for (int64_t e = rp[p]; e < rp[p + 1]; ++e) {
+                int32_t i = perm[p], j = perm[ic[e]];
+                if (h->storage == HCI_STORE_FULL) {
+                    if (i < j) f(i, j, iv[e]);
+                } else {
+                    f(std::min(i, j), std::max(i, j), iv[e]);
+                }
+            }
+    };
+    each([&](int32_t i, int32_t, double) { ++cnt[(size_t)i + 1]; });
+    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<std::pair<int32_t, double>> ent((size_t)cnt[n]);
+    std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+    each([&](int32_t i, int32_t j, double v) { ent[(size_t)cur[i]++] = {j, v}; });
+    for (int64_t i = 0; i < n; ++i)
+        std::sort(ent.begin() + cnt[i], ent.begin() + cnt[i + 1],
+                  [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+                      return a.first < b.first;
+                  });
+    if (row_ptr) std::copy(cnt.begin(), cnt.end(), row_ptr);
+    for (size_t e = 0; e < ent.size(); ++e) {
+        if (col) col[e] = ent[e].first;
+        if (val) val[e] = ent[e].second;
+    }
+    if (diag && n) {
+        std::vector<double> d((size_t)n);
+        HIP_TRY(c, hipMemcpy(d.data(), h->d_diag, (size_t)n * 8, hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < n; ++p) diag[perm[p]] = d[p];
+    }
+    return HCI_OK;
+}
+
+int hci_csr_row(const hci_csr* h, int64_t row, int32_t* cols, double* vals, int64_t cap, int64_t* count) {
+    if (!h || !count) return HCI_ERR_ARG;
+    hci_ctx* c = h->ctx;
+    if (h->storage != HCI_STORE_FULL) return hci_fail(c, HCI_ERR_ARG, "hci_csr_row: needs HCI_STORE_FULL");
+    if (row < 0 || row >= h->n) return hci_fail(c, HCI_ERR_ARG, "hci_csr_row: row out of range");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int32_t p = 0;
+    HIP_TRY(c, hipMemcpy(&p, h->d_pos + row, 4, hipMemcpyDeviceToHost));
+    int64_t rr[2];
+    HIP_TRY(c, hipMemcpy(rr, h->d_row_ptr + p, 16, hipMemcpyDeviceToHost));
+    const int64_t len = rr[1] - rr[0];
+    std::vector<int32_t> q((size_t)len);
+    std::vector<double> v((size_t)len);
+    if (len) {
+        HIP_TRY(c, hipMemcpy(q.data(), h->d_col + rr[0], (size_t)len * 4, hipMemcpyDeviceToHost));
         if (h->prec == HCI_PREC_F64) {
-            HIP_TRY(c, hipMemcpy(val, h->d_val, (size_t)h->nnz * 8, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(v.data(), h->d_val + rr[0], (size_t)len * 8, hipMemcpyDeviceToHost));
         } else {
-            double* tmp = nullptr;
-            HIP_TRY(c, hipMalloc(&tmp, (size_t)h->nnz * 8));
-            hipLaunchKernelGGL(k_widen, grid_for(h->nnz, 256), 256, 0, c->stream, h->d_hi, h->d_lo, tmp, h->nnz);
-            HIP_TRY(c, hipStreamSynchronize(c->stream));
-            HIP_TRY(c, hipMemcpy(val, tmp, (size_t)h->nnz * 8, hipMemcpyDeviceToHost));
-            hipFree(tmp);
+            std::vector<float> a((size_t)len), b((size_t)len);
+            HIP_TRY(c, hipMemcpy(a.data(), h->d_hi + rr[0], (size_t)len * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(c, hipMemcpy(b.data(), h->d_lo + rr[0], (size_t)len * 4, hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < len; ++k) v[k] = (double)a[k] + (double)b[k];
         }
+    }
+    std::vector<int32_t> jj((size_t)len);
+    if (len) {
+        int32_t* tmp = nullptr;
+        HIP_TRY(c, hipMalloc(&tmp, (size_t)len * 4));
+        hipLaunchKernelGGL(k_gather_perm, grid_for(len, 256), 256, 0, c->stream, h->d_perm, h->d_col + rr[0], len, tmp);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(jj.data(), tmp, (size_t)len * 4, hipMemcpyDeviceToHost);
+        hipFree(tmp);
+        HIP_TRY(c, e);
+    }
+    std::vector<std::pair<int32_t, double>> ent;
+    for (int64_t k = 0; k < len; ++k)
+        if (jj[k] > row) ent.push_back({jj[k], v[k]});
+    std::sort(ent.begin(), ent.end(), [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+        return a.first < b.first;
+    });
+    *count = (int64_t)ent.size();
+    for (int64_t k = 0; k < (int64_t)ent.size() && k < cap; ++k) {
+        if (cols) cols[k] = ent[k].first;
+        if (vals) vals[k] = ent[k].second;
     }
     return HCI_OK;
 }
@@ -302,7 +412,10 @@ void hci_csr_free(hci_csr* h) {
     hipFree(h->d_lo);
     hipFree(h->d_diag);
     hipFree(h->d_chunk_row);
+    hipFree(h->d_item_row);
     hipFree(h->d_rownorm);
+    hipFree(h->d_perm);
+    hipFree(h->d_pos);
     delete h;
 }
 

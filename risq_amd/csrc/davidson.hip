@@ -295,8 +295,23 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
     hci_dav_stats st{};
     st.refine_iter = -1;
     int sigma_mode = use_f32 ? HCI_SIGMA_F32 : HCI_SIGMA_FULL;
+    // every sigma product is bracketed by events on the context stream
+    std::vector<hipEvent_t> ev;
+    struct EvFree {
+        std::vector<hipEvent_t>& e;
+        ~EvFree() {
+            for (auto x : e) hipEventDestroy(x);
+        }
+    } ev_free{ev};
     auto sigma = [&](const double* v, double* out) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+            return hci_fail(c, HCI_ERR_HIP, "hci_davidson: event creation");
+        ev.push_back(a);
+        ev.push_back(b);
+        hipEventRecord(a, s);
         int rc = spmv_device(h, v, out, sigma_mode);
+        hipEventRecord(b, s);
         if (sigma_mode == HCI_SIGMA_F32)
             ++st.n_sigma_f32;
         else
@@ -311,9 +326,15 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
 
     int dim_sub = std::min<int64_t>(init_dim, N);
     if (init) {
-        HIP_TRY(c, hipMemcpyAsync(w.V, init, vb, opts->mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        // caller order -> internal order
+        HIP_TRY(c, hipMemcpyAsync(w.t, init, vb, opts->mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        permute_to_internal(h, w.t, w.V);
     } else {
-        for (int k = 0; k < dim_sub; ++k) hipLaunchKernelGGL(k_unit, grid_for(N, 256), 256, 0, s, w.col(w.V, k), N, (int64_t)k);
+        // identity columns of the caller's first determinants (davidson.rs:325-327)
+        int32_t hp[2] = {0, 0};
+        HIP_TRY(c, hipMemcpy(hp, h->d_pos, (size_t)dim_sub * 4, hipMemcpyDeviceToHost));
+        for (int k = 0; k < dim_sub; ++k)
+            hipLaunchKernelGGL(k_unit, grid_for(N, 256), 256, 0, s, w.col(w.V, k), N, (int64_t)hp[k]);
     }
     std::vector<double> P;  // row-major dim_sub x dim_sub
     auto rebuild_projection = [&]() -> int {
@@ -437,10 +458,15 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
     st.residual = err;
     if (converged) {
         *e0 = lam;
-        HIP_TRY(c, hipMemcpyAsync(coeffs, w.x, vb, opts->mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        permute_to_caller(h, w.x, w.r);
+        HIP_TRY(c, hipMemcpyAsync(coeffs, w.r, vb, opts->mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
     HIP_TRY(c, hipStreamSynchronize(s));
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t k = 0; k + 1 < ev.size(); k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) st.sigma_ms += ms;
+    }
     if (stats) *stats = st;
     return converged ? HCI_OK : HCI_NOT_CONVERGED;
 }

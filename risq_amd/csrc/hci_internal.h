@@ -38,21 +38,35 @@ struct hci_ctx {
     }
 };
 
+// Device-resident Hamiltonian.  Rows/columns are numbered in the internal
+// alpha-major order p (determinants sorted by alpha string, then beta string);
+// perm[p] is the caller's index of internal row p and pos is its inverse.
+// storage FULL holds both triangles (every connected pair twice), UPPER holds
+// p < q only.  nnz_u counts the reference's stored upper-triangle pairs.
 struct hci_csr {
     hci_ctx* ctx = nullptr;
     int64_t n = 0;
-    int64_t nnz = 0;
+    int64_t nnz_u = 0;             // reference count (upper pairs)
+    int64_t nnz_s = 0;             // stored entries (2*nnz_u for FULL)
     int prec = HCI_PREC_F64;
+    int storage = HCI_STORE_FULL;
+    int32_t* d_perm = nullptr;     // internal -> caller index
+    int32_t* d_pos = nullptr;      // caller index -> internal
     int64_t* d_row_ptr = nullptr;  // n+1
-    int32_t* d_col = nullptr;      // nnz
-    double* d_val = nullptr;       // nnz (HCI_PREC_F64)
-    float* d_hi = nullptr;         // nnz (HCI_PREC_F32)
-    float* d_lo = nullptr;         // nnz (HCI_PREC_F32)
-    double* d_diag = nullptr;      // n
-    // nnz-balanced SpMV work split: first row of every chunk of kSpmvChunk nonzeros
+    int32_t* d_col = nullptr;      // nnz_s, internal column index
+    double* d_val = nullptr;       // nnz_s (HCI_PREC_F64)
+    float* d_hi = nullptr;         // nnz_s (HCI_PREC_F32)
+    float* d_lo = nullptr;         // nnz_s (HCI_PREC_F32)
+    double* d_diag = nullptr;      // n, internal order
+    // nnz-balanced SpMV work split: row containing the first entry of every chunk
+    // (UPPER storage)
     int32_t* d_chunk_row = nullptr;
     int64_t n_chunks = 0;
-    // per-row absolute sums (feature for the learned preconditioner), lazily built
+    // row-aligned work items (FULL storage): item k covers rows
+    // [item_row[k], item_row[k+1]); a long row is an item of its own
+    int32_t* d_item_row = nullptr;
+    int64_t n_items = 0;
+    // per-row absolute sums (learned-preconditioner feature), lazily built
     double* d_rownorm = nullptr;
 };
 
@@ -69,10 +83,15 @@ int hci_fail(hci_ctx* ctx, int code, const std::string& msg);
 
 // SpMV (spmv.hip) ---------------------------------------------------------------
 constexpr int kSpmvChunk = 1024;  // nonzeros per wave in the nnz-balanced SpMV
+constexpr int kShortRows = 1024;  // FULL SpMV: rows are packed into items of at most this many entries
 int spmv_prepare(hci_csr* h);      // builds d_chunk_row
 // y = H x on device vectors, on the context stream; mode = HCI_SIGMA_*
+// y = H x on device vectors in internal order, on the context stream
 int spmv_device(const hci_csr* h, const double* x, double* y, int mode);
-int csr_rownorm(hci_csr* h);       // fills d_rownorm[i] = sum_j |H_ij| (both triangles)
+int csr_rownorm(hci_csr* h);       // fills d_rownorm[p] = sum_q |H_pq| (off-diagonal)
+// caller order <-> internal order
+void permute_to_internal(const hci_csr* h, const double* x_caller, double* x_int);
+void permute_to_caller(const hci_csr* h, const double* x_int, double* x_caller);
 
 // launch helpers
 inline unsigned grid_for(int64_t n, int block) {

@@ -1,15 +1,18 @@
-// Symmetric sigma product y = H x from the upper-triangle CSR.
+// Symmetric sigma product y = H x.
 //
 // Replaces SparseMatUpperTri::matrix_vector_prod (reference src/var/sparse.rs:64-78):
 //   y_i = d_i x_i + sum_{j>i} H_ij x_j + sum_{j<i} H_ji x_j.
-// Work split: one wavefront per chunk of kSpmvChunk stored nonzeros (balanced
-// regardless of the skewed row lengths of an HCI Hamiltonian).  For every
-// nonzero the transpose term H_ij x_i is added to y_j with a hardware FP64
-// atomic; the row terms H_ij x_j are summed across the lanes that share a row
-// (segmented shuffle reduction) and added to y_i with one atomic per row
-// segment.  y starts as d o x.  HBM traffic is one pass over (col, value) of
-// the upper triangle plus the vectors; reordered FP64 sums make the result
-// agree with the reference loop to rounding, not bitwise.
+// Vectors are in the matrix's internal (alpha-major) order.  Work split: one
+// wavefront per chunk of kSpmvChunk stored entries, balanced regardless of the
+// skewed row lengths of an HCI Hamiltonian; 64 consecutive entries per step,
+// each lane finds its row by binary search inside the chunk's row range, and
+// lanes of one row are summed by a segmented shuffle reduction.
+//   FULL storage: rows are complete, so no atomics: row-aligned work items
+//     (k_spmv_items) give every row one owner.  Results are bitwise
+//     reproducible.
+//   UPPER storage: the transpose term H_pq x_p goes to y_q with a hardware FP64
+//     atomic (order-dependent rounding).
+// y starts as d o x in both cases.
 #include "hci_internal.h"
 
 namespace {
@@ -38,20 +41,122 @@ __global__ void k_diag_mul(const double* d, const double* x, double* y, int64_t 
 }
 
 // MODE 0: FP64 values; 1: FP32 hi word; 2: FP32 hi + lo
+// The matrix is streamed once per product: non-temporal loads keep it from
+// evicting the x vector out of L2.
 template <int MODE>
 __device__ __forceinline__ double load_val(const double* val, const float* hi, const float* lo,
                                            int64_t e) {
-    if (MODE == 0) return val[e];
-    if (MODE == 1) return (double)hi[e];
-    return (double)hi[e] + (double)lo[e];
+    if (MODE == 0) return __builtin_nontemporal_load(val + e);
+    if (MODE == 1) return (double)__builtin_nontemporal_load(hi + e);
+    return (double)__builtin_nontemporal_load(hi + e) + (double)__builtin_nontemporal_load(lo + e);
 }
 
+__device__ __forceinline__ int32_t load_col(const int32_t* col, int64_t e) {
+    return __builtin_nontemporal_load(col + e);
+}
+
+// FULL storage: one wavefront per row-aligned work item.  A long row streams
+// through the wave with per-lane partial sums (four entries in flight per
+// lane) and one reduction at the end; a pack of short rows is processed 64
+// entries per step with a segmented reduction, a row that continues into the
+// next step being carried in registers.  Every row has exactly one owner:
+// y_p = d_p x_p + sum_q H_pq x_q is written once, with no atomics.
 template <int MODE, bool ABS>
+__global__ void __launch_bounds__(256)
+    k_spmv_items(int64_t n_items, const int32_t* __restrict__ item_row, const int64_t* __restrict__ rp,
+                 const int32_t* __restrict__ col, const double* __restrict__ val,
+                 const float* __restrict__ hi, const float* __restrict__ lo,
+                 const double* __restrict__ diag, const double* __restrict__ x, double* __restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (k >= n_items) return;
+    const int32_t ra = item_row[k], rb = item_row[k + 1];
+    const int64_t e0 = rp[ra], e1 = rp[rb];
+    if (rb - ra == 1 && e1 - e0 > 64) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int64_t e = e0 + lane;
+        for (; e + 192 < e1; e += 256) {
+            const double v0 = load_val<MODE>(val, hi, lo, e);
+            const double v1 = load_val<MODE>(val, hi, lo, e + 64);
+            const double v2 = load_val<MODE>(val, hi, lo, e + 128);
+            const double v3 = load_val<MODE>(val, hi, lo, e + 192);
+            if (ABS) {
+                s0 += fabs(v0);
+                s1 += fabs(v1);
+                s2 += fabs(v2);
+                s3 += fabs(v3);
+            } else {
+                const int32_t c0 = load_col(col, e), c1 = load_col(col, e + 64), c2 = load_col(col, e + 128),
+                              c3 = load_col(col, e + 192);
+                s0 += v0 * x[c0];
+                s1 += v1 * x[c1];
+                s2 += v2 * x[c2];
+                s3 += v3 * x[c3];
+            }
+        }
+        for (; e < e1; e += 64) {
+            const double v0 = load_val<MODE>(val, hi, lo, e);
+            s0 += ABS ? fabs(v0) : v0 * x[load_col(col, e)];
+        }
+        double s = (s0 + s1) + (s2 + s3);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+        if (lane == 0) y[ra] = ABS ? s : diag[ra] * x[ra] + s;
+        return;
+    }
+    // rows of the pack with no entries
+    for (int32_t r = ra + lane; r < rb; r += 64)
+        if (rp[r] == rp[r + 1]) y[r] = ABS ? 0.0 : diag[r] * x[r];
+    int32_t r_lo = ra;
+    int32_t open_row = -1;  // row continuing from the previous step (wave-uniform)
+    double open_sum = 0.0;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+        const int64_t e = eb + lane;
+        const bool valid = e < e1;
+        int32_t r = -1;
+        double s = 0.0;
+        if (valid) {
+            int32_t a = r_lo, b = rb - 1;
+            while (a < b) {
+                const int32_t mid = (a + b + 1) >> 1;
+                if (rp[mid] <= e)
+                    a = mid;
+                else
+                    b = mid - 1;
+            }
+            r = a;
+            const double v = load_val<MODE>(val, hi, lo, e);
+            s = ABS ? fabs(v) : v * x[load_col(col, e)];
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double t = __shfl_down(s, o);
+            const int32_t rr = __shfl_down(r, o);
+            if (lane + o < 64 && rr == r) s += t;
+        }
+        const int32_t rprev = __shfl_up(r, 1);
+        const bool head = valid && (lane == 0 || rprev != r);
+        if (lane == 0 && r == open_row) s += open_sum;
+        const int last_valid = (int)min((int64_t)63, e1 - 1 - eb);
+        const int32_t r_last = __shfl(r, last_valid);
+        const bool last_open = rp[r_last + 1] > min(eb + 64, e1);
+        const uint64_t heads = __ballot(head);
+        const uint64_t upto = last_valid == 63 ? ~0ull : ((2ull << last_valid) - 1);
+        const int head_last = 63 - __clzll(heads & upto);
+        if (head && !(r == r_last && last_open)) y[r] = ABS ? s : diag[r] * x[r] + s;
+        const double s_last = __shfl(s, head_last);
+        open_row = last_open ? r_last : -1;
+        open_sum = last_open ? s_last : 0.0;
+        r_lo = r_last;
+    }
+}
+
+template <int MODE>
 __global__ void __launch_bounds__(256)
     k_spmv_upper(int64_t nnz, int64_t n_chunks, const int64_t* __restrict__ rp,
                  const int32_t* __restrict__ chunk_row, const int32_t* __restrict__ col,
                  const double* __restrict__ val, const float* __restrict__ hi,
-                 const float* __restrict__ lo, const double* __restrict__ x, double* y) {
+                 const float* __restrict__ lo, const double* __restrict__ x, double* y, bool absval) {
     const int lane = threadIdx.x & 63;
     const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (c >= n_chunks) return;
@@ -75,12 +180,15 @@ __global__ void __launch_bounds__(256)
             }
             r = a;
             double v = load_val<MODE>(val, hi, lo, e);
-            if (ABS) v = fabs(v);
             const int32_t j = col[e];
-            const double xr = ABS ? 1.0 : x[r];
-            const double xj = ABS ? 1.0 : x[j];
-            unsafeAtomicAdd(&y[j], v * xr);
-            s = v * xj;
+            if (absval) {
+                v = fabs(v);
+                unsafeAtomicAdd(&y[j], v);
+                s = v;
+            } else {
+                unsafeAtomicAdd(&y[j], v * x[r]);
+                s = v * x[j];
+            }
         }
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -95,37 +203,97 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+__global__ void k_gather(const int32_t* perm, const double* src, int64_t n, double* dst) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[perm[p]];
+}
+
+__global__ void k_scatter(const int32_t* perm, const double* src, int64_t n, double* dst) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[perm[p]] = src[p];
+}
+
 }  // namespace
+
+void permute_to_internal(const hci_csr* h, const double* x_caller, double* x_int) {
+    hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, h->ctx->stream, h->d_perm, x_caller, h->n, x_int);
+}
+
+void permute_to_caller(const hci_csr* h, const double* x_int, double* x_caller) {
+    hipLaunchKernelGGL(k_scatter, grid_for(h->n, 256), 256, 0, h->ctx->stream, h->d_perm, x_int, h->n, x_caller);
+}
 
 int spmv_prepare(hci_csr* h) {
     hci_ctx* c = h->ctx;
     hipFree(h->d_chunk_row);
+    hipFree(h->d_item_row);
     h->d_chunk_row = nullptr;
-    h->n_chunks = (h->nnz + kSpmvChunk - 1) / kSpmvChunk;
-    HIP_TRY(c, hipMalloc(&h->d_chunk_row, (size_t)(h->n_chunks + 1) * 4 + 16));
-    if (h->nnz > 0)
-        hipLaunchKernelGGL(k_chunk_rows, grid_for(h->n_chunks + 1, 256), 256, 0, c->stream, h->d_row_ptr,
-                           h->n, h->nnz, h->n_chunks, h->d_chunk_row);
-    HIP_TRY(c, hipGetLastError());
+    h->d_item_row = nullptr;
+    if (h->storage == HCI_STORE_UPPER) {
+        h->n_chunks = (h->nnz_s + kSpmvChunk - 1) / kSpmvChunk;
+        HIP_TRY(c, hipMalloc(&h->d_chunk_row, (size_t)(h->n_chunks + 1) * 4 + 16));
+        if (h->nnz_s > 0)
+            hipLaunchKernelGGL(k_chunk_rows, grid_for(h->n_chunks + 1, 256), 256, 0, c->stream, h->d_row_ptr,
+                               h->n, h->nnz_s, h->n_chunks, h->d_chunk_row);
+        HIP_TRY(c, hipGetLastError());
+        return HCI_OK;
+    }
+    // FULL: pack consecutive rows into items of <= kShortRows entries; a row
+    // longer than that is an item of its own
+    std::vector<int64_t> rp((size_t)h->n + 1);
+    HIP_TRY(c, hipMemcpyAsync(rp.data(), h->d_row_ptr, rp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> items;
+    items.reserve((size_t)(h->nnz_s / kShortRows + h->n / 64 + 2));
+    int64_t r = 0;
+    while (r < h->n) {
+        items.push_back((int32_t)r);
+        const int64_t start = rp[r];
+        if (rp[r + 1] - start > kShortRows) {
+            ++r;
+            continue;
+        }
+        int64_t q = r + 1;
+        while (q < h->n && rp[q + 1] - start <= kShortRows && q - r < 4096) ++q;
+        r = q;
+    }
+    items.push_back((int32_t)h->n);
+    h->n_items = (int64_t)items.size() - 1;
+    HIP_TRY(c, hipMalloc(&h->d_item_row, items.size() * 4 + 16));
+    HIP_TRY(c, hipMemcpy(h->d_item_row, items.data(), items.size() * 4, hipMemcpyHostToDevice));
     return HCI_OK;
 }
 
+namespace {
+template <int MODE>
+void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
+    hipStream_t s = h->ctx->stream;
+    if (h->storage == HCI_STORE_FULL) {
+        const unsigned grid = grid_for(h->n_items, 4);
+        if (absval)
+            hipLaunchKernelGGL((k_spmv_items<MODE, true>), grid, 256, 0, s, h->n_items, h->d_item_row, h->d_row_ptr,
+                               h->d_col, h->d_val, h->d_hi, h->d_lo, h->d_diag, x, y);
+        else
+            hipLaunchKernelGGL((k_spmv_items<MODE, false>), grid, 256, 0, s, h->n_items, h->d_item_row,
+                               h->d_row_ptr, h->d_col, h->d_val, h->d_hi, h->d_lo, h->d_diag, x, y);
+    } else {
+        if (!absval)
+            hipLaunchKernelGGL(k_diag_mul, grid_for(h->n, 256), 256, 0, s, h->d_diag, x, y, h->n);
+        if (h->nnz_s > 0)
+            hipLaunchKernelGGL((k_spmv_upper<MODE>), grid_for(h->n_chunks, 4), 256, 0, s, h->nnz_s, h->n_chunks,
+                               h->d_row_ptr, h->d_chunk_row, h->d_col, h->d_val, h->d_hi, h->d_lo, x, y, absval);
+    }
+}
+}  // namespace
+
 int spmv_device(const hci_csr* h, const double* x, double* y, int mode) {
     hci_ctx* c = h->ctx;
-    hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_diag_mul, grid_for(h->n, 256), 256, 0, s, h->d_diag, x, y, h->n);
-    if (h->nnz > 0) {
-        const unsigned grid = grid_for(h->n_chunks, 4);
-        if (h->prec == HCI_PREC_F64)
-            hipLaunchKernelGGL((k_spmv_upper<0, false>), grid, 256, 0, s, h->nnz, h->n_chunks, h->d_row_ptr,
-                               h->d_chunk_row, h->d_col, h->d_val, nullptr, nullptr, x, y);
-        else if (mode == HCI_SIGMA_F32)
-            hipLaunchKernelGGL((k_spmv_upper<1, false>), grid, 256, 0, s, h->nnz, h->n_chunks, h->d_row_ptr,
-                               h->d_chunk_row, h->d_col, nullptr, h->d_hi, nullptr, x, y);
-        else
-            hipLaunchKernelGGL((k_spmv_upper<2, false>), grid, 256, 0, s, h->nnz, h->n_chunks, h->d_row_ptr,
-                               h->d_chunk_row, h->d_col, nullptr, h->d_hi, h->d_lo, x, y);
-    }
+    if (h->prec == HCI_PREC_F64)
+        launch_spmv<0>(h, x, y, false);
+    else if (mode == HCI_SIGMA_F32)
+        launch_spmv<1>(h, x, y, false);
+    else
+        launch_spmv<2>(h, x, y, false);
     HIP_TRY(c, hipGetLastError());
     return HCI_OK;
 }
@@ -133,18 +301,12 @@ int spmv_device(const hci_csr* h, const double* x, double* y, int mode) {
 int csr_rownorm(hci_csr* h) {
     if (h->d_rownorm) return HCI_OK;
     hci_ctx* c = h->ctx;
-    hipStream_t s = c->stream;
     HIP_TRY(c, hipMalloc(&h->d_rownorm, (size_t)h->n * 8 + 16));
-    HIP_TRY(c, hipMemsetAsync(h->d_rownorm, 0, (size_t)h->n * 8, s));
-    if (h->nnz > 0) {
-        const unsigned grid = grid_for(h->n_chunks, 4);
-        if (h->prec == HCI_PREC_F64)
-            hipLaunchKernelGGL((k_spmv_upper<0, true>), grid, 256, 0, s, h->nnz, h->n_chunks, h->d_row_ptr,
-                               h->d_chunk_row, h->d_col, h->d_val, nullptr, nullptr, nullptr, h->d_rownorm);
-        else
-            hipLaunchKernelGGL((k_spmv_upper<2, true>), grid, 256, 0, s, h->nnz, h->n_chunks, h->d_row_ptr,
-                               h->d_chunk_row, h->d_col, nullptr, h->d_hi, h->d_lo, nullptr, h->d_rownorm);
-    }
+    HIP_TRY(c, hipMemsetAsync(h->d_rownorm, 0, (size_t)h->n * 8, c->stream));
+    if (h->prec == HCI_PREC_F64)
+        launch_spmv<0>(h, nullptr, h->d_rownorm, true);
+    else
+        launch_spmv<2>(h, nullptr, h->d_rownorm, true);
     HIP_TRY(c, hipGetLastError());
     return HCI_OK;
 }
@@ -158,29 +320,38 @@ extern "C" int hci_sigma(const hci_csr* h, int ncol, const double* x, double* y,
         return hci_fail(c, HCI_ERR_ARG, "hci_sigma: bad sigma_mode");
     HIP_TRY(c, hipSetDevice(c->device));
     const size_t bytes = (size_t)h->n * 8;
-    if (mem == HCI_MEM_DEVICE) {
-        for (int k = 0; k < ncol; ++k) {
-            int rc = spmv_device(h, x + (size_t)k * h->n, y + (size_t)k * h->n, sigma_mode);
-            if (rc) return rc;
-        }
-        return HCI_OK;
-    }
-    double *dx = nullptr, *dy = nullptr;
-    HIP_TRY(c, hipMalloc(&dx, bytes + 16));
-    HIP_TRY(c, hipMalloc(&dy, bytes + 16));
+    double *dx = nullptr, *dy = nullptr, *xi = nullptr, *yi = nullptr;
     int rc = HCI_OK;
+    auto cleanup = [&]() {
+        hipStreamSynchronize(c->stream);
+        hipFree(dx);
+        hipFree(dy);
+        hipFree(xi);
+        hipFree(yi);
+    };
+    if (hipMalloc(&xi, bytes + 16) != hipSuccess || hipMalloc(&yi, bytes + 16) != hipSuccess ||
+        (mem == HCI_MEM_HOST && (hipMalloc(&dx, bytes + 16) != hipSuccess || hipMalloc(&dy, bytes + 16) != hipSuccess))) {
+        cleanup();
+        return hci_fail(c, HCI_ERR_OOM, "hci_sigma: vector allocation");
+    }
     for (int k = 0; k < ncol && rc == HCI_OK; ++k) {
-        if (hipMemcpyAsync(dx, x + (size_t)k * h->n, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
-            rc = hci_fail(c, HCI_ERR_HIP, "hci_sigma: copy in");
-            break;
+        const double* xk = x + (size_t)k * h->n;
+        double* yk = y + (size_t)k * h->n;
+        if (mem == HCI_MEM_HOST) {
+            if (hipMemcpyAsync(dx, xk, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+                rc = hci_fail(c, HCI_ERR_HIP, "hci_sigma: copy in");
+                break;
+            }
+            xk = dx;
         }
-        rc = spmv_device(h, dx, dy, sigma_mode);
-        if (rc == HCI_OK &&
-            hipMemcpyAsync(y + (size_t)k * h->n, dy, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        permute_to_internal(h, xk, xi);
+        rc = spmv_device(h, xi, yi, sigma_mode);
+        if (rc) break;
+        permute_to_caller(h, yi, mem == HCI_MEM_HOST ? dy : yk);
+        if (mem == HCI_MEM_HOST &&
+            hipMemcpyAsync(yk, dy, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
             rc = hci_fail(c, HCI_ERR_HIP, "hci_sigma: copy out");
     }
-    hipStreamSynchronize(c->stream);
-    hipFree(dx);
-    hipFree(dy);
+    cleanup();
     return rc;
 }

@@ -34,6 +34,8 @@ SIGMA_FULL = 0
 SIGMA_F32 = 1
 PRECOND_DPR = 0
 PRECOND_MLP = 1
+STORE_FULL = 0
+STORE_UPPER = 1
 
 
 class HciError(RuntimeError):
@@ -43,7 +45,7 @@ class HciError(RuntimeError):
 
 
 class BuildOpts(C.Structure):
-    _fields_ = [("value_prec", C.c_int), ("single_rule", C.c_int), ("mem", C.c_int)]
+    _fields_ = [("value_prec", C.c_int), ("single_rule", C.c_int), ("mem", C.c_int), ("storage", C.c_int)]
 
 
 class BuildStats(C.Structure):
@@ -60,7 +62,8 @@ class DavOpts(C.Structure):
 class DavStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("n_sigma_f32", C.c_int), ("n_sigma_full", C.c_int),
                 ("converged", C.c_int), ("mlp_applied", C.c_int), ("mlp_fallbacks", C.c_int),
-                ("refine_iter", C.c_int), ("residual", C.c_double), ("seconds", C.c_double)]
+                ("refine_iter", C.c_int), ("residual", C.c_double), ("seconds", C.c_double),
+                ("sigma_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,6 +83,7 @@ SIGNATURES = {
                               C.POINTER(_P), C.POINTER(BuildStats)]),
     "hci_csr_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "hci_csr_download": (C.c_int, [_P, _P, _P, _P, _P]),
+    "hci_csr_row": (C.c_int, [_P, C.c_int64, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "hci_csr_free": (None, [_P]),
     "hci_sigma": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.c_int]),
     "hci_dav_default_opts": (None, [C.POINTER(DavOpts)]),
@@ -222,7 +226,7 @@ class HciContext:
     # -- sparse H --------------------------------------------------------------------
     def build_h(self, dets, diag, n: int | None = None, prev: "SparseMatUpperTri | None" = None,
                 n_stored: int = 0, value_prec: int = PREC_F64,
-                single_rule: int = SINGLES_ABS_GT_1E12) -> "SparseMatUpperTri":
+                single_rule: int = SINGLES_ABS_GT_1E12, storage: int = STORE_FULL) -> "SparseMatUpperTri":
         """hci_build_h.  dets/diag are numpy arrays (host) or DeviceBuffers."""
         if isinstance(dets, DeviceBuffer):
             mem = MEM_DEVICE
@@ -234,7 +238,7 @@ class HciContext:
             diag = np.ascontiguousarray(diag, dtype=np.float64)
             n = dets.shape[0]
             pd, pg = _ptr(dets), _ptr(diag)
-        opts = BuildOpts(value_prec, single_rule, mem)
+        opts = BuildOpts(value_prec, single_rule, mem, storage)
         out = C.c_void_p()
         st = BuildStats()
         self._check(self.lib.hci_build_h(self.handle, pd, pg, n, prev.handle if prev else None,
@@ -302,6 +306,15 @@ class SparseMatUpperTri:
         diag = np.empty(self.n)
         self.ctx._check(self.ctx.lib.hci_csr_download(self.handle, _ptr(rp), _ptr(col), _ptr(val), _ptr(diag)))
         return rp, col[:self.nnz], val[:self.nnz], diag
+
+    def row(self, i: int) -> tuple[np.ndarray, np.ndarray]:
+        """Upper-triangle row i in caller order: (columns j > i ascending, H_ij)."""
+        cnt = C.c_int64()
+        self.ctx._check(self.ctx.lib.hci_csr_row(self.handle, i, None, None, 0, C.byref(cnt)))
+        cols = np.empty(max(cnt.value, 1), dtype=np.int32)
+        vals = np.empty(max(cnt.value, 1))
+        self.ctx._check(self.ctx.lib.hci_csr_row(self.handle, i, _ptr(cols), _ptr(vals), cnt.value, C.byref(cnt)))
+        return cols[:cnt.value], vals[:cnt.value]
 
     def matrix_vector_prod(self, x: np.ndarray, mode: int = SIGMA_FULL) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64)

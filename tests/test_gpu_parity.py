@@ -9,7 +9,7 @@ import pytest
 
 from helpers import csr_to_dense, hubbard_space, mol_space
 from risq_amd.dets import from_orbitals, full_space
-from risq_amd.hci import PREC_F32, SIGMA_F32, SINGLES_NONZERO, HciError
+from risq_amd.hci import PREC_F32, SIGMA_F32, SINGLES_NONZERO, STORE_FULL, STORE_UPPER, HciError
 
 pytestmark = pytest.mark.gpu
 
@@ -57,12 +57,13 @@ def test_elements_bit_exact(gpu_ctx, oracle_lib):
     assert np.array_equal(gv[nz].view(np.uint64), gv2[nz].view(np.uint64))
 
 
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER])
 @pytest.mark.parametrize("space", [MOL0, (16, 32, 3, 3, 5, 2500)])
-def test_build_matches_oracle(gpu_ctx, oracle_lib, space):
+def test_build_matches_oracle(gpu_ctx, oracle_lib, space, storage):
     ints, dets = mol_space(*space)
     gpu_ctx.set_integrals(ints)
     diag = oracle_lib.OracleHam(ints).ham_diag(dets)
-    h = gpu_ctx.build_h(dets, diag)
+    h = gpu_ctx.build_h(dets, diag, storage=storage)
     rp, col, val, dg = h.download()
     S = _oracle_build(oracle_lib, ints, dets, diag)
     orp, ocol, oval, odg = S.export()
@@ -109,14 +110,15 @@ def test_build_hubbard_doubles_vanish(gpu_ctx, oracle_lib):
     assert np.all(np.isin(np.abs(val), [1.0]))
 
 
-def test_incremental_build(gpu_ctx, oracle_lib):
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER])
+def test_incremental_build(gpu_ctx, oracle_lib, storage):
     ints, dets = mol_space(*MOL0)
     gpu_ctx.set_integrals(ints)
     diag = oracle_lib.OracleHam(ints).ham_diag(dets)
     n1 = 1200
-    h1 = gpu_ctx.build_h(dets[:n1], diag[:n1])
-    h2 = gpu_ctx.build_h(dets, diag, prev=h1, n_stored=n1)
-    full = gpu_ctx.build_h(dets, diag)
+    h1 = gpu_ctx.build_h(dets[:n1], diag[:n1], storage=storage)
+    h2 = gpu_ctx.build_h(dets, diag, prev=h1, n_stored=n1, storage=storage)
+    full = gpu_ctx.build_h(dets, diag, storage=storage)
     a, b = h2.download(), full.download()
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
@@ -154,11 +156,12 @@ def test_small_n_single_rule(gpu_ctx, oracle_lib):
     assert np.array_equal(a[1], b[1])
 
 
-def test_sigma_matches_oracle(gpu_ctx, oracle_lib):
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER])
+def test_sigma_matches_oracle(gpu_ctx, oracle_lib, storage):
     ints, dets = mol_space(*MOL0)
     gpu_ctx.set_integrals(ints)
     diag = oracle_lib.OracleHam(ints).ham_diag(dets)
-    h = gpu_ctx.build_h(dets, diag)
+    h = gpu_ctx.build_h(dets, diag, storage=storage)
     S = _oracle_build(oracle_lib, ints, dets, diag)
     rng = np.random.default_rng(3)
     X = rng.standard_normal((dets.shape[0], 3))
@@ -259,3 +262,27 @@ def test_two_dets(gpu_ctx):
     h = gpu_ctx.build_h(dets, gpu_ctx.ham_diag(dets), single_rule=SINGLES_NONZERO)
     res = gpu_ctx.davidson(h)
     assert res.converged
+
+
+def test_full_storage_is_deterministic(gpu_ctx, oracle_lib):
+    """FULL storage: builds and sigma products are bitwise reproducible."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 20000)
+    gpu_ctx.set_integrals(ints)
+    diag = gpu_ctx.ham_diag(dets)
+    a = gpu_ctx.build_h(dets, diag)
+    b = gpu_ctx.build_h(dets, diag)
+    x = np.random.default_rng(5).standard_normal(dets.shape[0])
+    ya, ya2, yb = a.matrix_vector_prod(x), a.matrix_vector_prod(x), b.matrix_vector_prod(x)
+    assert np.array_equal(ya.view(np.uint64), ya2.view(np.uint64))
+    assert np.array_equal(ya.view(np.uint64), yb.view(np.uint64))
+
+
+def test_row_accessor_matches_download(gpu_ctx, oracle_lib):
+    ints, dets = mol_space(*MOL0)
+    gpu_ctx.set_integrals(ints)
+    h = gpu_ctx.build_h(dets, gpu_ctx.ham_diag(dets))
+    rp, col, val, _ = h.download()
+    for i in [0, 1, 7, 100, dets.shape[0] - 2, dets.shape[0] - 1]:
+        c, v = h.row(i)
+        assert np.array_equal(c, col[rp[i]:rp[i + 1]])
+        assert np.array_equal(v, val[rp[i]:rp[i + 1]])
