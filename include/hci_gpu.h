@@ -104,7 +104,7 @@ typedef struct hci_dav_opts {
     int max_sub;       /* subspace limit, reference nvalues + 10 = 11 (davidson.rs:64-68) */
     int precond;       /* HCI_PRECOND_* */
     int mixed;         /* 0: full-precision sigmas; 1: FP32 sigmas, then FP64 refinement */
-    double refine_tol; /* mixed: residual below which refinement starts */
+    double refine_tol; /* mixed: residual below which refinement starts (default 1e-4) */
     int mem;           /* where init / coeffs live */
 } hci_dav_opts;
 

@@ -257,7 +257,7 @@ extern "C" void hci_dav_default_opts(hci_dav_opts* o) {
     o->max_sub = 11;
     o->precond = HCI_PRECOND_DPR;
     o->mixed = 0;
-    o->refine_tol = 1e-3;
+    o->refine_tol = 1e-4;
     o->mem = HCI_MEM_HOST;
 }
 
@@ -387,7 +387,9 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
         } else {
             ++since_best;
         }
-        if (sigma_mode == HCI_SIGMA_F32 && (err < opts->refine_tol || since_best >= 3)) {
+        // FP32-value sigmas converge on the rounded matrix; refine once the
+        // residual reaches refine_tol, or if it has stopped improving
+        if (sigma_mode == HCI_SIGMA_F32 && (err < opts->refine_tol || since_best >= 25)) {
             // switch to full-precision sigmas: restart from the Ritz vector
             sigma_mode = HCI_SIGMA_FULL;
             st.refine_iter = it;
