@@ -106,6 +106,8 @@ typedef struct hci_dav_opts {
     int mixed;         /* 0: full-precision sigmas; 1: FP32 sigmas, then FP64 refinement */
     double refine_tol; /* mixed: residual below which refinement starts (default 1e-4) */
     int mem;           /* where init / coeffs live */
+    int restart_keep;  /* vectors kept at a restart: 1 = the Ritz vector (reference,
+                          davidson.rs:258-269); 2 = also the previous Ritz vector */
 } hci_dav_opts;
 
 typedef struct hci_dav_stats {
@@ -177,6 +179,52 @@ int hci_sigma(const hci_csr* h, int ncol, const double* x, double* y, int sigma_
 void hci_dav_default_opts(hci_dav_opts* opts);
 int hci_davidson(hci_ctx* ctx, const hci_csr* h, const double* init, const hci_dav_opts* opts,
                  double* e0, double* coeffs, hci_dav_stats* stats);
+
+/* Learned preconditioner (no reference counterpart: it replaces the diagonal
+ * correction of davidson.rs:391-408 when opts.precond == HCI_PRECOND_MLP).
+ * Network 24 -> 128 -> 128 -> 1 (SiLU), output o gives the bounded factor
+ * g = 1 + 2 tanh(o) that scales the diagonal correction:
+ * t_p = g_p r_p / (lambda - H_pp).  Parameters are FP32, row-major
+ * W1[24][128], b1[128], W2[128][128], b2[128], W3[128], b3 (HCI_MLP_NPARAM
+ * floats); features are standardised with feat_mean / feat_std.  Products run
+ * on BF16 matrix cores with FP32 accumulation.  Training rows: X[n][24] raw
+ * features, m[n] the scaled diagonal correction, y[n] the scaled exact
+ * correction; loss = mean((g m - y)^2). */
+#define HCI_MLP_NFEAT 24
+#define HCI_MLP_HIDDEN 128
+#define HCI_MLP_NPARAM 19841
+
+typedef struct hci_mlp_train_opts {
+    int epochs;
+    int batch;
+    float lr;          /* Adam learning rate */
+    uint64_t seed;     /* initialisation / shuffling */
+    int standardize;   /* 1: set feat_mean/feat_std from the training rows */
+    int mem;           /* where X/m/y live */
+} hci_mlp_train_opts;
+
+typedef struct hci_mlp_train_stats {
+    double first_epoch_loss;
+    double last_epoch_loss;
+    double seconds;
+} hci_mlp_train_stats;
+
+int hci_mlp_init(hci_ctx* ctx, uint64_t seed);
+int hci_mlp_set_params(hci_ctx* ctx, const float* params, const float* feat_mean, const float* feat_std);
+int hci_mlp_get_params(hci_ctx* ctx, float* params, float* feat_mean, float* feat_std);
+int hci_mlp_forward(hci_ctx* ctx, const float* X, int64_t n, float* out, int mem);
+int hci_mlp_loss_grad(hci_ctx* ctx, const float* X, const float* m, const float* y, int64_t n, float* grad,
+                      double* loss, int mem);
+void hci_mlp_default_train_opts(hci_mlp_train_opts* opts);
+int hci_mlp_train(hci_ctx* ctx, const float* X, const float* m, const float* y, int64_t n,
+                  const hci_mlp_train_opts* opts, hci_mlp_train_stats* stats);
+/* Runs Davidson (diagonal correction) on h and records training rows at every
+ * iteration: features of up to per_iter determinants (an evenly strided
+ * sample), m = scaled diagonal correction, y = scaled exact correction, the
+ * part of x_exact (caller order, the converged eigenvector) outside the
+ * current search space.  Writes at most cap rows; *count = rows written. */
+int hci_mlp_collect(hci_ctx* ctx, const hci_csr* h, const double* x_exact, const hci_dav_opts* opts,
+                    int64_t per_iter, float* X, float* m, float* y, int64_t cap, int64_t* count);
 
 /* Device-memory and timing plumbing (no reference counterpart: for callers
  * that keep inputs resident in HBM, and for timing on the context's stream).

@@ -397,6 +397,12 @@ __global__ void k_split(const double* v, int64_t n, float* hi, float* lo) {
     lo[e] = (float)(x - (double)h);
 }
 
+__global__ void k_gather_words(const int32_t* perm, const uint64_t* w, int64_t n, uint64_t* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * n) return;
+    out[i] = w[4 * (int64_t)perm[i >> 2] + (i & 3)];
+}
+
 __global__ void k_inverse_perm(const int32_t* perm, int64_t n, int32_t* pos) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) pos[perm[p]] = (int32_t)p;
@@ -792,6 +798,8 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     }
     BTH(hipMalloc(&H->d_diag, (size_t)N * 8 + 16));
     hipLaunchKernelGGL(k_gather_f64, grid_for(N, 256), 256, 0, s, sa_idx, diag_dev, N, H->d_diag);
+    BTH(hipMalloc(&H->d_words, (size_t)N * 32 + 16));
+    hipLaunchKernelGGL(k_gather_words, grid_for(4 * N, 256), 256, 0, s, sa_idx, dets_dev, N, H->d_words);
     int prc = spmv_prepare(H);
     if (prc) return fail_free(prc);
     BTH(hipStreamSynchronize(s));

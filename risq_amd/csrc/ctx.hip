@@ -416,6 +416,7 @@ void hci_csr_free(hci_csr* h) {
     hipFree(h->d_rownorm);
     hipFree(h->d_perm);
     hipFree(h->d_pos);
+    hipFree(h->d_words);
     delete h;
 }
 

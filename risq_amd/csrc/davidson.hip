@@ -95,10 +95,11 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(const double* partial, i
     }
 }
 
-// x = V y, r = W y - lam x, t = r / (lam - d); partial sums of r.r and t.t
+// x = V y, wx = W y (= H x), r = wx - lam x, t = r / (lam - d); partial sums of
+// r.r and t.t
 __global__ void __launch_bounds__(kRedThreads)
     k_ritz(const double* __restrict__ V, const double* __restrict__ W, int64_t ld, int k, Coef y,
-           double lam, const double* __restrict__ d, int64_t n, double* x, double* r, double* t,
+           double lam, const double* __restrict__ d, int64_t n, double* x, double* wx, double* r, double* t,
            double* partial) {
     double acc[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -112,6 +113,7 @@ __global__ void __launch_bounds__(kRedThreads)
         const double rv = wv - xv * lam;
         const double tv = rv / (lam - d[i]);
         x[i] = xv;
+        wx[i] = wv;
         r[i] = rv;
         t[i] = tv;
         acc[0] += rv * rv;
@@ -214,6 +216,8 @@ struct DavWork {
     double* V = nullptr;
     double* W = nullptr;
     double* x = nullptr;
+    double* wx = nullptr;
+    double* xp = nullptr;  // previous Ritz vector
     double* r = nullptr;
     double* t = nullptr;
     double* partial = nullptr;
@@ -223,6 +227,8 @@ struct DavWork {
         hipFree(V);
         hipFree(W);
         hipFree(x);
+        hipFree(wx);
+        hipFree(xp);
         hipFree(r);
         hipFree(t);
         hipFree(partial);
@@ -247,6 +253,58 @@ struct DavWork {
     }
 };
 
+// Training-row recorder for the learned preconditioner (hci_mlp_collect).
+struct Recorder {
+    const double* e_int = nullptr;  // converged eigenvector, internal order
+    double* e_perp = nullptr;       // its part outside the search space
+    float* feat = nullptr;          // n x HCI_MLP_NFEAT
+    float *X = nullptr, *m = nullptr, *y = nullptr;  // staged rows (device)
+    int64_t per_iter = 0, cap = 0, count = 0;
+    uint64_t iter = 0;
+};
+
+__global__ void k_collect(const float* feat, const double* t, const double* e, double ms, double ys, int64_t n,
+                          int64_t per, int64_t offset, int64_t stride, int64_t base, float* X, float* m, float* y) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= per) return;
+    const int64_t p = (offset + j * stride) % n;
+    for (int f = 0; f < HCI_MLP_NFEAT; ++f) X[(base + j) * HCI_MLP_NFEAT + f] = feat[p * HCI_MLP_NFEAT + f];
+    m[base + j] = (float)(t[p] * ms);
+    y[base + j] = (float)(e[p] * ys);
+}
+
+int record_rows(hci_ctx* c, const hci_csr* h, DavWork& w, Recorder& rec, double lam, double err, double tnorm,
+                int dim_sub) {
+    const int64_t N = w.n;
+    hipStream_t s = c->stream;
+    if (rec.count >= rec.cap) return HCI_OK;
+    // exact correction: x_exact aligned with the Ritz vector, minus its part in span(V)
+    int rc = w.multidot(w.x, 1, rec.e_int);
+    if (rc) return rc;
+    const double sgn = w.h_red[0] < 0 ? -1.0 : 1.0;
+    hipLaunchKernelGGL(k_scale, grid_for(N, 256), 256, 0, s, rec.e_int, sgn, rec.e_perp, N);
+    if ((rc = w.multidot(w.V, dim_sub, rec.e_perp))) return rc;
+    Coef hc{};
+    for (int k = 0; k < dim_sub; ++k) hc.c[k] = w.h_red[k];
+    hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, dim_sub, hc, rec.e_perp, N, w.partial);
+    if ((rc = w.reduce(dim_sub + 1))) return rc;
+    const double enorm = std::sqrt(w.h_red[dim_sub]);
+    if (!(enorm > 0.0) || !(tnorm > 0.0)) return HCI_OK;
+    if ((rc = w.multidot(w.t, 1, rec.e_perp))) return rc;
+    const double sgn2 = w.h_red[0] < 0 ? -1.0 : 1.0;
+    if ((rc = mlp_features(c, h, lam, err, tnorm, w.r, w.x, w.t, rec.feat))) return rc;
+    const int64_t per = std::min<int64_t>(std::min<int64_t>(rec.per_iter, N), rec.cap - rec.count);
+    const int64_t stride = std::max<int64_t>(1, N / std::max<int64_t>(per, 1));
+    const int64_t offset = (int64_t)((rec.iter * 0x9E3779B97F4A7C15ull) % (uint64_t)N);
+    const double sn = std::sqrt((double)N);
+    hipLaunchKernelGGL(k_collect, grid_for(per, 256), 256, 0, s, rec.feat, w.t, rec.e_perp, sn / tnorm,
+                       sgn2 * sn / enorm, N, per, offset, stride, rec.count, rec.X, rec.m, rec.y);
+    HIP_TRY(c, hipGetLastError());
+    rec.count += per;
+    ++rec.iter;
+    return HCI_OK;
+}
+
 }  // namespace
 
 extern "C" void hci_dav_default_opts(hci_dav_opts* o) {
@@ -259,10 +317,11 @@ extern "C" void hci_dav_default_opts(hci_dav_opts* o) {
     o->mixed = 0;
     o->refine_tol = 1e-4;
     o->mem = HCI_MEM_HOST;
+    o->restart_keep = 1;
 }
 
-extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, const hci_dav_opts* opts,
-                            double* e0, double* coeffs, hci_dav_stats* stats) {
+static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const hci_dav_opts* opts,
+                        double* e0, double* coeffs, hci_dav_stats* stats, Recorder* rec) {
     if (!c || !h || !opts || !e0 || !coeffs) return c ? hci_fail(c, HCI_ERR_ARG, "hci_davidson: null argument") : HCI_ERR_ARG;
     if (h->ctx != c) return hci_fail(c, HCI_ERR_ARG, "hci_davidson: matrix from another context");
     const int64_t N = h->n;
@@ -286,6 +345,8 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
     HIP_TRY(c, hipMalloc(&w.V, vb * (size_t)std::max(max_sub, 2) + 16));
     HIP_TRY(c, hipMalloc(&w.W, vb * (size_t)std::max(max_sub, 2) + 16));
     HIP_TRY(c, hipMalloc(&w.x, vb + 16));
+    HIP_TRY(c, hipMalloc(&w.wx, vb + 16));
+    HIP_TRY(c, hipMalloc(&w.xp, vb + 16));
     HIP_TRY(c, hipMalloc(&w.r, vb + 16));
     HIP_TRY(c, hipMalloc(&w.t, vb + 16));
     HIP_TRY(c, hipMalloc(&w.partial, (size_t)kRedBlocks * (kMaxSub + 1) * 8));
@@ -360,6 +421,7 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
     double err_prev = 1e300, best_err = 1e300;
     int since_best = 0;
     MlpState* net = (opts->precond == HCI_PRECOND_MLP) ? c->mlp : nullptr;
+    bool have_prev = false;
     if (net) DT(csr_rownorm(const_cast<hci_csr*>(h)));
     int it;
     for (it = 0; it < opts->max_iters; ++it) {
@@ -369,17 +431,19 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
         Coef y{};
         for (int k = 0; k < dim_sub; ++k) y.c[k] = vecs[(size_t)k * dim_sub];
         hipLaunchKernelGGL(k_ritz, w.red_grid(), kRedThreads, 0, s, w.V, w.W, N, dim_sub, y, lam, h->d_diag, N,
-                           w.x, w.r, w.t, w.partial);
+                           w.x, w.wx, w.r, w.t, w.partial);
         DT(w.reduce(2));
         err = std::sqrt(w.h_red[0]);
         double tnorm = std::sqrt(w.h_red[1]);
-        if (N <= 2 || (err < opts->tol && std::fabs(lam - last_eig) < opts->etol)) {
+        const bool conv = N <= 2 || (err < opts->tol && std::fabs(lam - last_eig) < opts->etol);
+        if (conv && sigma_mode == HCI_SIGMA_FULL) {
             converged = true;
             break;
         }
-        // MLP correction is judged by what it did to the residual
+        // an MLP correction is judged by what it did to the residual: after two
+        // steps that did not reduce it, the diagonal correction takes over
         if (net && mlp_prev && !(err < err_prev)) ++st.mlp_fallbacks;
-        const bool mlp_now = net && !(mlp_prev && !(err < err_prev));
+        const bool mlp_now = net && st.mlp_fallbacks < 2;
         last_eig = lam;
         if (err < best_err * 0.7) {
             best_err = err;
@@ -389,7 +453,7 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
         }
         // FP32-value sigmas converge on the rounded matrix; refine once the
         // residual reaches refine_tol, or if it has stopped improving
-        if (sigma_mode == HCI_SIGMA_F32 && (err < opts->refine_tol || since_best >= 25)) {
+        if (sigma_mode == HCI_SIGMA_F32 && (conv || err < opts->refine_tol || since_best >= 25)) {
             // switch to full-precision sigmas: restart from the Ritz vector
             sigma_mode = HCI_SIGMA_FULL;
             st.refine_iter = it;
@@ -403,10 +467,11 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
             continue;
         }
         err_prev = err;
+        if (rec && dim_sub + 1 <= max_sub) DT(record_rows(c, h, w, *rec, lam, err, tnorm, dim_sub));
         if (dim_sub + 1 <= max_sub) {
             double* t = w.col(w.V, dim_sub);
             if (mlp_now) {
-                DT(mlp_correction(c, h, lam, w.r, w.x, w.t /*dpr*/, t));
+                DT(mlp_correction(c, h, lam, err, tnorm, w.r, w.x, w.t /*dpr*/, t));
                 ++st.mlp_applied;
                 mlp_prev = true;
                 DT(w.multidot(t, 1, t));
@@ -446,6 +511,36 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
             }
             P.swap(np);
             dim_sub = nd;
+        } else if (opts->restart_keep >= 2 && have_prev) {
+            // thick restart: the Ritz vector (W column from W y, no product) and
+            // the previous Ritz vector orthonormalised against it
+            HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(c, hipMemcpyAsync(w.W, w.wx, vb, hipMemcpyDeviceToDevice, s));
+            double* v1 = w.col(w.V, 1);
+            HIP_TRY(c, hipMemcpyAsync(v1, w.xp, vb, hipMemcpyDeviceToDevice, s));
+            DT(w.multidot(w.V, 1, v1));
+            Coef hc{};
+            hc.c[0] = w.h_red[0];
+            hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, 1, hc, v1, N, w.partial);
+            DT(w.reduce(2));
+            hc.c[0] = w.h_red[0];
+            hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, 1, hc, v1, N, w.partial);
+            DT(w.reduce(2));
+            const double nrm = std::sqrt(w.h_red[1]);
+            dim_sub = 1;
+            if (nrm > 1e-9) {
+                hipLaunchKernelGGL(k_scale, grid_for(N, 256), 256, 0, s, v1, 1.0 / nrm, v1, N);
+                DT(sigma(v1, w.col(w.W, 1)));
+                dim_sub = 2;
+            }
+            P.assign((size_t)dim_sub * dim_sub, 0.0);
+            for (int k = 0; k < dim_sub; ++k) {
+                DT(w.multidot(w.V, dim_sub, w.col(w.W, k)));
+                for (int q = 0; q < dim_sub; ++q) P[(size_t)q * dim_sub + k] = w.h_red[q];
+            }
+            for (int q = 0; q < dim_sub; ++q)
+                for (int k = q + 1; k < dim_sub; ++k) P[(size_t)k * dim_sub + q] = P[(size_t)q * dim_sub + k];
+            mlp_prev = false;
         } else {
             // restart from the Ritz vector (davidson.rs:258-269)
             dim_sub = 1;
@@ -453,6 +548,8 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
             DT(rebuild_projection());
             mlp_prev = false;
         }
+        HIP_TRY(c, hipMemcpyAsync(w.xp, w.x, vb, hipMemcpyDeviceToDevice, s));
+        have_prev = true;
     }
 #undef DT
     st.iters = it;
@@ -471,4 +568,60 @@ extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, co
     }
     if (stats) *stats = st;
     return converged ? HCI_OK : HCI_NOT_CONVERGED;
+}
+
+extern "C" int hci_davidson(hci_ctx* c, const hci_csr* h, const double* init, const hci_dav_opts* opts,
+                            double* e0, double* coeffs, hci_dav_stats* stats) {
+    return davidson_run(c, h, init, opts, e0, coeffs, stats, nullptr);
+}
+
+extern "C" int hci_mlp_collect(hci_ctx* c, const hci_csr* h, const double* x_exact, const hci_dav_opts* opts,
+                               int64_t per_iter, float* X, float* m, float* y, int64_t cap, int64_t* count) {
+    if (!c || !h || !x_exact || !opts || !X || !m || !y || !count || cap < 0 || per_iter < 1)
+        return c ? hci_fail(c, HCI_ERR_ARG, "hci_mlp_collect: bad arguments") : HCI_ERR_ARG;
+    if (h->ctx != c) return hci_fail(c, HCI_ERR_ARG, "hci_mlp_collect: matrix from another context");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t N = h->n;
+    const size_t F = HCI_MLP_NFEAT;
+    Recorder rec;
+    rec.per_iter = per_iter;
+    rec.cap = cap;
+    double* e_dev = nullptr;
+    double* e_int = nullptr;
+    struct Free {
+        std::vector<void*> p;
+        ~Free() {
+            for (void* q : p) hipFree(q);
+        }
+    } fr;
+    auto alloc = [&](void** q, size_t bytes) {
+        hipError_t e = hipMalloc(q, bytes + 16);
+        if (e == hipSuccess) fr.p.push_back(*q);
+        return e;
+    };
+    HIP_TRY(c, alloc((void**)&e_dev, (size_t)N * 8));
+    HIP_TRY(c, alloc((void**)&e_int, (size_t)N * 8));
+    HIP_TRY(c, alloc((void**)&rec.e_perp, (size_t)N * 8));
+    HIP_TRY(c, alloc((void**)&rec.feat, (size_t)N * F * 4));
+    HIP_TRY(c, alloc((void**)&rec.X, (size_t)cap * F * 4));
+    HIP_TRY(c, alloc((void**)&rec.m, (size_t)cap * 4));
+    HIP_TRY(c, alloc((void**)&rec.y, (size_t)cap * 4));
+    HIP_TRY(c, hipMemcpyAsync(e_dev, x_exact, (size_t)N * 8,
+                              opts->mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    permute_to_internal(h, e_dev, e_int);
+    rec.e_int = e_int;
+    hci_dav_opts o = *opts;
+    o.precond = HCI_PRECOND_DPR;
+    o.mem = HCI_MEM_HOST;
+    std::vector<double> coeffs((size_t)N);
+    double e0 = 0.0;
+    int rc = davidson_run(c, h, nullptr, &o, &e0, coeffs.data(), nullptr, &rec);
+    if (rc < 0) return rc;
+    *count = rec.count;
+    if (rec.count > 0) {
+        HIP_TRY(c, hipMemcpy(X, rec.X, (size_t)rec.count * F * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(m, rec.m, (size_t)rec.count * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(y, rec.y, (size_t)rec.count * 4, hipMemcpyDeviceToHost));
+    }
+    return HCI_OK;
 }

@@ -58,6 +58,7 @@ struct hci_csr {
     float* d_hi = nullptr;         // nnz_s (HCI_PREC_F32)
     float* d_lo = nullptr;         // nnz_s (HCI_PREC_F32)
     double* d_diag = nullptr;      // n, internal order
+    uint64_t* d_words = nullptr;   // 4n: determinant words in internal order
     // nnz-balanced SpMV work split: row containing the first entry of every chunk
     // (UPPER storage)
     int32_t* d_chunk_row = nullptr;

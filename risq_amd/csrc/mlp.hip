@@ -1,9 +1,597 @@
-// Learned Davidson preconditioner (MLP).  Placeholder until the network lands.
+// Learned Davidson preconditioner: a 24 -> 128 -> 128 -> 1 SiLU network whose
+// output o_p gives a bounded factor g_p = 1 + 2 tanh(o_p) that rescales the
+// diagonal correction, t_p = g_p * r_p / (lambda - H_pp),
+// for every determinant p.  Training pairs are recorded from the library's own
+// Davidson runs on small systems: per-determinant features and the exact
+// correction, i.e. the part of the converged eigenvector not yet in the
+// search space.  The Davidson loop falls back to the diagonal correction
+// whenever a network step does not reduce the residual norm, and the converged
+// FP64 eigenvalue decides convergence, so the network can change the iteration
+// count but not the answer.
+//
+// All matrix products (forward and backward) run on the matrix cores through
+// v_mfma_f32_16x16x16_bf16: operands are rounded to bfloat16 on load, products
+// accumulate in FP32.  Master weights, Adam moments and activations are FP32.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
 #include "mlp.h"
 
-bool mlp_ready(const hci_ctx* c) { return c && c->mlp; }
-int mlp_correction(hci_ctx* c, const hci_csr*, double, const double*, const double*, const double*,
-                   double*) {
-    return hci_fail(c, HCI_ERR_STATE, "MLP preconditioner not loaded");
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int F = HCI_MLP_NFEAT;
+constexpr int Hd = HCI_MLP_HIDDEN;
+// parameter offsets (row-major weights: W[in][out])
+constexpr int oW1 = 0, ob1 = oW1 + F * Hd, oW2 = ob1 + Hd, ob2 = oW2 + Hd * Hd, oW3 = ob2 + Hd, ob3 = oW3 + Hd;
+static_assert(ob3 + 1 == HCI_MLP_NPARAM, "parameter layout");
+
+// round-to-nearest-even FP32 -> bfloat16 (finite inputs)
+__device__ __forceinline__ short to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
 }
-void mlp_free_state(hci_ctx*) {}
+
+// C(m, n) = sum_k A(m, k) B(k, n) (+ bias(n)) (+ C if accumulate), with
+// A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn], C(m, n) = C[m*scm + n*scn].
+// One wavefront per 16x16 tile of C, one MFMA per 16-deep slice of K.
+// Operand layout of v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)..+3]
+// and B[4(l/16)..+3][l%16]; the result lane l holds C[4(l/16)+i][l%16], i<4.
+__global__ void __launch_bounds__(256)
+    k_gemm_bf16(int M, int N, int K, const float* __restrict__ A, int64_t sam, int64_t sak,
+                const float* __restrict__ B, int64_t sbk, int64_t sbn, float* __restrict__ C, int64_t scm,
+                int64_t scn, const float* __restrict__ bias, int accumulate) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tiles_n = (N + 15) >> 4;
+    const int64_t tm = w / tiles_n;
+    const int tn = (int)(w % tiles_n);
+    if (tm * 16 >= M) return;
+    const int64_t m0 = tm * 16;
+    const int n0 = tn * 16;
+    const int row = lane & 15, kq = (lane >> 4) * 4;
+    const int64_t am = m0 + row;
+    const int bn = n0 + row;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = k0 + kq + i;
+            a[i] = (am < M && k < K) ? to_bf16(A[am * sam + (int64_t)k * sak]) : (short)0;
+            b[i] = (bn < N && k < K) ? to_bf16(B[(int64_t)k * sbk + (int64_t)bn * sbn]) : (short)0;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t mm = m0 + kq + i;
+        const int nn = n0 + row;
+        if (mm < M && nn < N) {
+            float v = acc[i] + (bias ? bias[nn] : 0.f);
+            float* dst = C + mm * scm + (int64_t)nn * scn;
+            *dst = accumulate ? *dst + v : v;
+        }
+    }
+}
+
+__device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
+
+__global__ void k_silu_fwd(const float* Z, float* H, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) H[i] = Z[i] * sigmoid(Z[i]);
+}
+
+__global__ void k_silu_bwd(const float* dH, const float* Z, float* dZ, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float s = sigmoid(Z[i]);
+    dZ[i] = dH[i] * s * (1.f + Z[i] * (1.f - s));
+}
+
+// The network's scale factor is bounded: g = 1 + 2 tanh(out), in (-1, 3).
+// loss = (1/n) sum_i (g_i m_i - y_i)^2 ; dout_i = 2 (g_i m_i - y_i) m_i g'_i / n
+__global__ void k_loss(const float* out, const float* m, const float* y, int64_t n, float* dout, double* partial) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float th = tanhf(out[i]);
+        const float r = (1.f + 2.f * th) * m[i] - y[i];
+        acc += (double)r * r;
+        dout[i] = 2.f * r * m[i] * 2.f * (1.f - th * th) / (float)n;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// out[c] = sum_r X[r*cols + c]; one block per column, fixed order
+__global__ void k_colsum(const float* X, int64_t rows, int cols, float* out) {
+    __shared__ float sh[256];
+    const int c = blockIdx.x;
+    float acc = 0.f;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) acc += X[r * cols + c];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
+}
+
+__global__ void k_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    m1[i] = b1 * m1[i] + (1.f - b1) * gi;
+    m2[i] = b2 * m2[i] + (1.f - b2) * gi * gi;
+    p[i] -= lr * (m1[i] / c1) / (sqrtf(m2[i] / c2) + eps);
+}
+
+__global__ void k_normalize(const float* X, const float* mean, const float* stdv, int64_t n, float* Xn) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * F) return;
+    const int f = (int)(i % F);
+    Xn[i] = (X[i] - mean[f]) / stdv[f];
+}
+
+__global__ void k_gather_rows(const float* X, const float* m, const float* y, const int64_t* idx, int64_t b,
+                              float* Xb, float* mb, float* yb) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b * F) return;
+    const int64_t r = i / F;
+    const int f = (int)(i % F);
+    const int64_t src = idx[r];
+    Xb[i] = X[src * F + f];
+    if (f == 0) {
+        mb[r] = m[src];
+        yb[r] = y[src];
+    }
+}
+
+// Per-determinant features (internal order), see feature_doc below.
+template <class S>
+__global__ void k_features(const uint64_t* words, const double* diag, const double* rownorm, const double* r,
+                           const double* x, const double* t, double lam, double rs, double ts, double xs,
+                           int64_t n, int n_up, int n_dn, int f0, float* X) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const S up = load_string<S>(words + 4 * p), dn = load_string<S>(words + 4 * p + 2);
+    const double d = diag[p] - lam;
+    float* o = X + p * F;
+    o[0] = (float)d;
+    o[1] = (float)log(fabs(d) + 1e-6);
+    o[2] = (float)(r[p] * rs);
+    o[3] = (float)(t[p] * ts);
+    o[4] = (float)(x[p] * xs);
+    o[5] = (float)rownorm[p];
+    o[6] = (float)(rownorm[p] / (fabs(d) + 1e-3));
+    for (int k = 0; k < 16; ++k) {
+        const int orb = f0 + k;
+        const int bu = (int)((up >> orb) & (S)1), bd = (int)((dn >> orb) & (S)1);
+        o[7 + k] = 0.5f * (float)(bu + bd);
+    }
+    const S hfu = lowmask<S>(n_up), hfd = lowmask<S>(n_dn);
+    o[23] = 0.125f * (float)(popc(up ^ hfu) + popc(dn ^ hfd));
+}
+
+__global__ void k_apply(const float* out, const double* dpr, int64_t n, double* t) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float g = 1.f + 2.f * tanhf(out[p]);
+    t[p] = (double)g * dpr[p];
+}
+
+unsigned gemm_grid(int64_t M, int N) { return grid_for(((M + 15) / 16) * ((N + 15) / 16), 4); }
+
+}  // namespace
+
+struct MlpState {
+    float* P = nullptr;      // parameters
+    float* G = nullptr;      // gradients
+    float* A1 = nullptr;     // Adam moments
+    float* A2 = nullptr;
+    float* mean = nullptr;   // feature standardisation
+    float* stdv = nullptr;
+    int64_t cap = 0;         // rows of the work buffers
+    float *Xn = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *out = nullptr;
+    float *dout = nullptr, *dH = nullptr, *dZ = nullptr, *Xb = nullptr, *mb = nullptr, *yb = nullptr;
+    int64_t* idx = nullptr;
+    double* partial = nullptr;
+    float* feat = nullptr;   // Davidson-time features (n x F) and outputs
+    float* gout = nullptr;
+    int64_t feat_cap = 0;
+    bool ready = false;      // parameters and standardisation loaded
+    int64_t adam_t = 0;
+
+    void free_work() {
+        for (float* q : {Xn, Z1, H1, Z2, H2, out, dout, dH, dZ, Xb, mb, yb}) hipFree(q);
+        hipFree(idx);
+        Xn = Z1 = H1 = Z2 = H2 = out = dout = dH = dZ = Xb = mb = yb = nullptr;
+        idx = nullptr;
+        cap = 0;
+    }
+    ~MlpState() {
+        free_work();
+        for (float* q : {P, G, A1, A2, mean, stdv, feat, gout}) hipFree(q);
+        hipFree(partial);
+    }
+};
+
+namespace {
+
+int ensure_state(hci_ctx* c) {
+    if (c->mlp) return HCI_OK;
+    MlpState* s = new MlpState();
+    c->mlp = s;
+    HIP_TRY(c, hipMalloc(&s->P, HCI_MLP_NPARAM * 4));
+    HIP_TRY(c, hipMalloc(&s->G, HCI_MLP_NPARAM * 4));
+    HIP_TRY(c, hipMalloc(&s->A1, HCI_MLP_NPARAM * 4));
+    HIP_TRY(c, hipMalloc(&s->A2, HCI_MLP_NPARAM * 4));
+    HIP_TRY(c, hipMalloc(&s->mean, F * 4));
+    HIP_TRY(c, hipMalloc(&s->stdv, F * 4));
+    HIP_TRY(c, hipMalloc(&s->partial, 1024 * 8));
+    HIP_TRY(c, hipMemset(s->A1, 0, HCI_MLP_NPARAM * 4));
+    HIP_TRY(c, hipMemset(s->A2, 0, HCI_MLP_NPARAM * 4));
+    std::vector<float> one(F, 1.f), zero(F, 0.f);
+    HIP_TRY(c, hipMemcpy(s->mean, zero.data(), F * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(s->stdv, one.data(), F * 4, hipMemcpyHostToDevice));
+    return HCI_OK;
+}
+
+int ensure_work(hci_ctx* c, int64_t rows) {
+    MlpState* s = c->mlp;
+    if (rows <= s->cap) return HCI_OK;
+    s->free_work();
+    const size_t r = (size_t)rows;
+    HIP_TRY(c, hipMalloc(&s->Xn, r * F * 4));
+    HIP_TRY(c, hipMalloc(&s->Xb, r * F * 4));
+    for (float** q : {&s->Z1, &s->H1, &s->Z2, &s->H2, &s->dH, &s->dZ}) HIP_TRY(c, hipMalloc(q, r * Hd * 4));
+    for (float** q : {&s->out, &s->dout, &s->mb, &s->yb}) HIP_TRY(c, hipMalloc(q, r * 4));
+    HIP_TRY(c, hipMalloc(&s->idx, r * 8));
+    s->cap = rows;
+    return HCI_OK;
+}
+
+void gemm(hipStream_t st, int64_t M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B,
+          int64_t sbk, int64_t sbn, float* C, int64_t scm, int64_t scn, const float* bias, int acc) {
+    hipLaunchKernelGGL(k_gemm_bf16, gemm_grid(M, N), 256, 0, st, (int)M, N, K, A, sam, sak, B, sbk, sbn, C, scm,
+                       scn, bias, acc);
+}
+
+// forward on rows already standardised in s->Xn[0..n)
+void forward(hci_ctx* c, int64_t n) {
+    MlpState* s = c->mlp;
+    hipStream_t st = c->stream;
+    const float* P = s->P;
+    gemm(st, n, Hd, F, s->Xn, F, 1, P + oW1, Hd, 1, s->Z1, Hd, 1, P + ob1, 0);
+    hipLaunchKernelGGL(k_silu_fwd, grid_for(n * Hd, 256), 256, 0, st, s->Z1, s->H1, n * Hd);
+    gemm(st, n, Hd, Hd, s->H1, Hd, 1, P + oW2, Hd, 1, s->Z2, Hd, 1, P + ob2, 0);
+    hipLaunchKernelGGL(k_silu_fwd, grid_for(n * Hd, 256), 256, 0, st, s->Z2, s->H2, n * Hd);
+    gemm(st, n, 1, Hd, s->H2, Hd, 1, P + oW3, 1, 1, s->out, 1, 1, P + ob3, 0);
+}
+
+// loss and gradients for the batch in Xn / mb / yb (n rows); returns the loss
+int backward(hci_ctx* c, int64_t n, double* loss) {
+    MlpState* s = c->mlp;
+    hipStream_t st = c->stream;
+    const float* P = s->P;
+    float* G = s->G;
+    const unsigned lg = (unsigned)std::min<int64_t>(grid_for(n, 256), 1024);
+    hipLaunchKernelGGL(k_loss, lg, 256, 0, st, s->out, s->mb, s->yb, n, s->dout, s->partial);
+    // layer 3
+    gemm(st, Hd, 1, (int)n, s->H2, 1, Hd, s->dout, 1, 0, G + oW3, 1, 1, nullptr, 0);
+    hipLaunchKernelGGL(k_colsum, 1, 256, 0, st, s->dout, n, 1, G + ob3);
+    gemm(st, n, Hd, 1, s->dout, 1, 0, P + oW3, 0, 1, s->dH, Hd, 1, nullptr, 0);
+    hipLaunchKernelGGL(k_silu_bwd, grid_for(n * Hd, 256), 256, 0, st, s->dH, s->Z2, s->dZ, n * Hd);
+    // layer 2
+    gemm(st, Hd, Hd, (int)n, s->H1, 1, Hd, s->dZ, Hd, 1, G + oW2, Hd, 1, nullptr, 0);
+    hipLaunchKernelGGL(k_colsum, Hd, 256, 0, st, s->dZ, n, Hd, G + ob2);
+    gemm(st, n, Hd, Hd, s->dZ, Hd, 1, P + oW2, 1, Hd, s->dH, Hd, 1, nullptr, 0);
+    hipLaunchKernelGGL(k_silu_bwd, grid_for(n * Hd, 256), 256, 0, st, s->dH, s->Z1, s->dZ, n * Hd);
+    // layer 1
+    gemm(st, F, Hd, (int)n, s->Xn, 1, F, s->dZ, Hd, 1, G + oW1, Hd, 1, nullptr, 0);
+    hipLaunchKernelGGL(k_colsum, Hd, 256, 0, st, s->dZ, n, Hd, G + ob1);
+    HIP_TRY(c, hipGetLastError());
+    if (loss) {
+        std::vector<double> part(lg);
+        HIP_TRY(c, hipMemcpyAsync(part.data(), s->partial, lg * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipStreamSynchronize(st));
+        double t = 0.0;
+        for (double v : part) t += v;
+        *loss = t / (double)n;
+    }
+    return HCI_OK;
+}
+
+// host or device input -> device pointer (copy when host)
+struct DevIn {
+    const float* p = nullptr;
+    float* own = nullptr;
+    ~DevIn() { hipFree(own); }
+};
+
+int dev_in(hci_ctx* c, const float* src, size_t count, int mem, DevIn& d) {
+    if (mem == HCI_MEM_DEVICE) {
+        d.p = src;
+        return HCI_OK;
+    }
+    HIP_TRY(c, hipMalloc(&d.own, count * 4 + 16));
+    HIP_TRY(c, hipMemcpyAsync(d.own, src, count * 4, hipMemcpyHostToDevice, c->stream));
+    d.p = d.own;
+    return HCI_OK;
+}
+
+// splitmix64 -> uniform in [0, 1)
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+}  // namespace
+
+bool mlp_ready(const hci_ctx* c) { return c && c->mlp && c->mlp->ready; }
+
+void mlp_free_state(hci_ctx* c) {
+    if (c && c->mlp) {
+        delete c->mlp;
+        c->mlp = nullptr;
+    }
+}
+
+int mlp_correction(hci_ctx* c, const hci_csr* h, double lam, double rnorm, double tnorm, const double* r,
+                   const double* x, const double* dpr, double* t) {
+    MlpState* s = c->mlp;
+    hipStream_t st = c->stream;
+    const int64_t n = h->n;
+    if (n > s->feat_cap) {
+        hipFree(s->feat);
+        hipFree(s->gout);
+        s->feat = s->gout = nullptr;
+        s->feat_cap = 0;
+        HIP_TRY(c, hipMalloc(&s->feat, (size_t)n * F * 4));
+        HIP_TRY(c, hipMalloc(&s->gout, (size_t)n * 4));
+        s->feat_cap = n;
+    }
+    int rc = mlp_features(c, h, lam, rnorm, tnorm, r, x, dpr, s->feat);
+    if (rc) return rc;
+    const int64_t chunk = 1 << 18;
+    rc = ensure_work(c, std::min<int64_t>(n, chunk));
+    if (rc) return rc;
+    for (int64_t a = 0; a < n; a += chunk) {
+        const int64_t b = std::min<int64_t>(n, a + chunk) - a;
+        hipLaunchKernelGGL(k_normalize, grid_for(b * F, 256), 256, 0, st, s->feat + a * F, s->mean, s->stdv, b, s->Xn);
+        forward(c, b);
+        HIP_TRY(c, hipMemcpyAsync(s->gout + a, s->out, (size_t)b * 4, hipMemcpyDeviceToDevice, st));
+    }
+    hipLaunchKernelGGL(k_apply, grid_for(n, 256), 256, 0, st, s->gout, dpr, n, t);
+    HIP_TRY(c, hipGetLastError());
+    return HCI_OK;
+}
+
+int mlp_features(hci_ctx* c, const hci_csr* h, double lam, double rnorm, double tnorm, const double* r,
+                 const double* x, const double* dpr, float* X) {
+    int rc = csr_rownorm(const_cast<hci_csr*>(h));
+    if (rc) return rc;
+    const int64_t n = h->n;
+    uint64_t w0[4];
+    HIP_TRY(c, hipMemcpy(w0, h->d_words, 32, hipMemcpyDeviceToHost));
+    const int n_up = __builtin_popcountll(w0[0]) + __builtin_popcountll(w0[1]);
+    const int n_dn = __builtin_popcountll(w0[2]) + __builtin_popcountll(w0[3]);
+    int f0 = (n_up + n_dn) / 2 - 8;
+    f0 = std::max(0, std::min(f0, c->n_orb - 16));
+    const double sn = std::sqrt((double)n);
+    const double rs = rnorm > 0 ? sn / rnorm : 0.0, ts = tnorm > 0 ? sn / tnorm : 0.0;
+    if (c->n_orb <= 64)
+        hipLaunchKernelGGL(k_features<uint64_t>, grid_for(n, 256), 256, 0, c->stream, h->d_words, h->d_diag,
+                           h->d_rownorm, r, x, dpr, lam, rs, ts, sn, n, n_up, n_dn, f0, X);
+    else
+        hipLaunchKernelGGL(k_features<u128>, grid_for(n, 256), 256, 0, c->stream, h->d_words, h->d_diag,
+                           h->d_rownorm, r, x, dpr, lam, rs, ts, sn, n, n_up, n_dn, f0, X);
+    HIP_TRY(c, hipGetLastError());
+    return HCI_OK;
+}
+
+extern "C" {
+
+int hci_mlp_init(hci_ctx* c, uint64_t seed) {
+    if (!c) return HCI_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = ensure_state(c);
+    if (rc) return rc;
+    // Kaiming-uniform weights, zero biases, last layer small so g starts near 1
+    std::vector<float> p(HCI_MLP_NPARAM, 0.f);
+    Rng g{seed};
+    auto fill = [&](int off, int fan_in, int count, double gain) {
+        const double a = gain * std::sqrt(6.0 / fan_in);
+        for (int i = 0; i < count; ++i) p[off + i] = (float)((2.0 * g.uni() - 1.0) * a);
+    };
+    fill(oW1, F, F * Hd, 1.0);
+    fill(oW2, Hd, Hd * Hd, 1.0);
+    fill(oW3, Hd, Hd, 0.05);
+    HIP_TRY(c, hipMemcpy(c->mlp->P, p.data(), HCI_MLP_NPARAM * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->mlp->A1, 0, HCI_MLP_NPARAM * 4));
+    HIP_TRY(c, hipMemset(c->mlp->A2, 0, HCI_MLP_NPARAM * 4));
+    c->mlp->adam_t = 0;
+    c->mlp->ready = true;
+    return HCI_OK;
+}
+
+int hci_mlp_set_params(hci_ctx* c, const float* params, const float* feat_mean, const float* feat_std) {
+    if (!c || !params || !feat_mean || !feat_std) return c ? hci_fail(c, HCI_ERR_ARG, "hci_mlp_set_params: null") : HCI_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = ensure_state(c);
+    if (rc) return rc;
+    for (int f = 0; f < F; ++f)
+        if (!(feat_std[f] > 0.f)) return hci_fail(c, HCI_ERR_ARG, "hci_mlp_set_params: feature std must be > 0");
+    HIP_TRY(c, hipMemcpy(c->mlp->P, params, HCI_MLP_NPARAM * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->mlp->mean, feat_mean, F * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->mlp->stdv, feat_std, F * 4, hipMemcpyHostToDevice));
+    c->mlp->ready = true;
+    return HCI_OK;
+}
+
+int hci_mlp_get_params(hci_ctx* c, float* params, float* feat_mean, float* feat_std) {
+    if (!c) return HCI_ERR_ARG;
+    if (!c->mlp) return hci_fail(c, HCI_ERR_STATE, "hci_mlp_get_params: no network");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (params) HIP_TRY(c, hipMemcpy(params, c->mlp->P, HCI_MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+    if (feat_mean) HIP_TRY(c, hipMemcpy(feat_mean, c->mlp->mean, F * 4, hipMemcpyDeviceToHost));
+    if (feat_std) HIP_TRY(c, hipMemcpy(feat_std, c->mlp->stdv, F * 4, hipMemcpyDeviceToHost));
+    return HCI_OK;
+}
+
+int hci_mlp_forward(hci_ctx* c, const float* X, int64_t n, float* out, int mem) {
+    if (!c || n < 0 || (n > 0 && (!X || !out))) return c ? hci_fail(c, HCI_ERR_ARG, "hci_mlp_forward: bad arguments") : HCI_ERR_ARG;
+    if (!mlp_ready(c)) return hci_fail(c, HCI_ERR_STATE, "hci_mlp_forward: no network");
+    HIP_TRY(c, hipSetDevice(c->device));
+    DevIn dx;
+    int rc = dev_in(c, X, (size_t)n * F, mem, dx);
+    if (rc) return rc;
+    MlpState* s = c->mlp;
+    const int64_t chunk = 1 << 18;
+    rc = ensure_work(c, std::min<int64_t>(std::max<int64_t>(n, 1), chunk));
+    if (rc) return rc;
+    for (int64_t a = 0; a < n; a += chunk) {
+        const int64_t b = std::min<int64_t>(n, a + chunk) - a;
+        hipLaunchKernelGGL(k_normalize, grid_for(b * F, 256), 256, 0, c->stream, dx.p + a * F, s->mean, s->stdv, b, s->Xn);
+        forward(c, b);
+        HIP_TRY(c, hipMemcpyAsync(out + a, s->out, (size_t)b * 4,
+                                  mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HCI_OK;
+}
+
+int hci_mlp_loss_grad(hci_ctx* c, const float* X, const float* m, const float* y, int64_t n, float* grad,
+                      double* loss, int mem) {
+    if (!c || n < 1 || !X || !m || !y) return c ? hci_fail(c, HCI_ERR_ARG, "hci_mlp_loss_grad: bad arguments") : HCI_ERR_ARG;
+    if (!mlp_ready(c)) return hci_fail(c, HCI_ERR_STATE, "hci_mlp_loss_grad: no network");
+    if (n > (1 << 20)) return hci_fail(c, HCI_ERR_ARG, "hci_mlp_loss_grad: at most 2^20 rows");
+    HIP_TRY(c, hipSetDevice(c->device));
+    DevIn dx, dm, dy;
+    int rc = dev_in(c, X, (size_t)n * F, mem, dx);
+    if (!rc) rc = dev_in(c, m, (size_t)n, mem, dm);
+    if (!rc) rc = dev_in(c, y, (size_t)n, mem, dy);
+    if (!rc) rc = ensure_work(c, n);
+    if (rc) return rc;
+    MlpState* s = c->mlp;
+    hipLaunchKernelGGL(k_normalize, grid_for(n * F, 256), 256, 0, c->stream, dx.p, s->mean, s->stdv, n, s->Xn);
+    HIP_TRY(c, hipMemcpyAsync(s->mb, dm.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(s->yb, dy.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+    forward(c, n);
+    rc = backward(c, n, loss);
+    if (rc) return rc;
+    if (grad) HIP_TRY(c, hipMemcpy(grad, s->G, HCI_MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+    return HCI_OK;
+}
+
+void hci_mlp_default_train_opts(hci_mlp_train_opts* o) {
+    if (!o) return;
+    o->epochs = 20;
+    o->batch = 4096;
+    o->lr = 1e-3f;
+    o->seed = 7;
+    o->standardize = 1;
+    o->mem = HCI_MEM_HOST;
+}
+
+int hci_mlp_train(hci_ctx* c, const float* X, const float* m, const float* y, int64_t n,
+                  const hci_mlp_train_opts* o, hci_mlp_train_stats* stats) {
+    if (!c || !o || n < 1 || !X || !m || !y) return c ? hci_fail(c, HCI_ERR_ARG, "hci_mlp_train: bad arguments") : HCI_ERR_ARG;
+    if (o->batch < 16 || o->batch > (1 << 16) || o->epochs < 0)
+        return hci_fail(c, HCI_ERR_ARG, "hci_mlp_train: batch in [16, 65536], epochs >= 0");
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!mlp_ready(c)) {
+        int rc = hci_mlp_init(c, o->seed);
+        if (rc) return rc;
+    }
+    DevIn dx, dm, dy;
+    int rc = dev_in(c, X, (size_t)n * F, o->mem, dx);
+    if (!rc) rc = dev_in(c, m, (size_t)n, o->mem, dm);
+    if (!rc) rc = dev_in(c, y, (size_t)n, o->mem, dy);
+    if (!rc) rc = ensure_work(c, o->batch);
+    if (rc) return rc;
+    MlpState* s = c->mlp;
+    hipStream_t st = c->stream;
+    if (o->standardize) {
+        // feature means / standard deviations from (a sample of) the training set
+        const int64_t ns = std::min<int64_t>(n, 1 << 20);
+        std::vector<float> hx((size_t)ns * F);
+        HIP_TRY(c, hipMemcpyAsync(hx.data(), dx.p, hx.size() * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipStreamSynchronize(st));
+        std::vector<double> mu(F, 0.0), var(F, 0.0);
+        for (int64_t i = 0; i < ns; ++i)
+            for (int f = 0; f < F; ++f) mu[f] += hx[(size_t)i * F + f];
+        for (int f = 0; f < F; ++f) mu[f] /= (double)ns;
+        for (int64_t i = 0; i < ns; ++i)
+            for (int f = 0; f < F; ++f) {
+                const double d = hx[(size_t)i * F + f] - mu[f];
+                var[f] += d * d;
+            }
+        std::vector<float> fm(F), fs(F);
+        for (int f = 0; f < F; ++f) {
+            fm[f] = (float)mu[f];
+            fs[f] = (float)std::max(std::sqrt(var[f] / (double)ns), 1e-3);
+        }
+        HIP_TRY(c, hipMemcpy(s->mean, fm.data(), F * 4, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(s->stdv, fs.data(), F * 4, hipMemcpyHostToDevice));
+    }
+    Rng rng{o->seed * 0x2545F4914F6CDD1Dull + 1};
+    std::vector<int64_t> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) order[i] = i;
+    double first = -1.0, last = 0.0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < o->epochs; ++ep) {
+        for (int64_t i = n - 1; i > 0; --i) std::swap(order[i], order[(int64_t)(rng.uni() * (i + 1))]);
+        double ep_loss = 0.0;
+        int64_t nb = 0;
+        for (int64_t a = 0; a + o->batch <= n || (a == 0 && n < o->batch); a += o->batch) {
+            const int64_t b = std::min<int64_t>(o->batch, n - a);
+            HIP_TRY(c, hipMemcpyAsync(s->idx, order.data() + a, (size_t)b * 8, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_gather_rows, grid_for(b * F, 256), 256, 0, st, dx.p, dm.p, dy.p, s->idx, b, s->Xb,
+                               s->mb, s->yb);
+            hipLaunchKernelGGL(k_normalize, grid_for(b * F, 256), 256, 0, st, s->Xb, s->mean, s->stdv, b, s->Xn);
+            forward(c, b);
+            double l = 0.0;
+            rc = backward(c, b, &l);
+            if (rc) return rc;
+            ++s->adam_t;
+            const float c1 = 1.f - std::pow(b1, (float)s->adam_t), c2 = 1.f - std::pow(b2, (float)s->adam_t);
+            hipLaunchKernelGGL(k_adam, grid_for(HCI_MLP_NPARAM, 256), 256, 0, st, s->P, s->G, s->A1, s->A2,
+                               HCI_MLP_NPARAM, o->lr, b1, b2, eps, c1, c2);
+            ep_loss += l;
+            ++nb;
+        }
+        last = ep_loss / std::max<int64_t>(nb, 1);
+        if (first < 0) first = last;
+    }
+    HIP_TRY(c, hipStreamSynchronize(st));
+    s->ready = true;
+    if (stats) {
+        stats->first_epoch_loss = first;
+        stats->last_epoch_loss = last;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return HCI_OK;
+}
+
+}  // extern "C"

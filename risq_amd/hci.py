@@ -56,7 +56,7 @@ class BuildStats(C.Structure):
 class DavOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("etol", C.c_double), ("max_iters", C.c_int),
                 ("max_sub", C.c_int), ("precond", C.c_int), ("mixed", C.c_int),
-                ("refine_tol", C.c_double), ("mem", C.c_int)]
+                ("refine_tol", C.c_double), ("mem", C.c_int), ("restart_keep", C.c_int)]
 
 
 class DavStats(C.Structure):
@@ -67,6 +67,20 @@ class DavStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MLP_NFEAT = 24
+MLP_HIDDEN = 128
+MLP_NPARAM = 19841
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("seed", C.c_uint64),
+                ("standardize", C.c_int), ("mem", C.c_int)]
+
+
+class MlpTrainStats(C.Structure):
+    _fields_ = [("first_epoch_loss", C.c_double), ("last_epoch_loss", C.c_double), ("seconds", C.c_double)]
 
 
 # every symbol include/hci_gpu.h declares: name -> (restype, argtypes)
@@ -89,6 +103,15 @@ SIGNATURES = {
     "hci_dav_default_opts": (None, [C.POINTER(DavOpts)]),
     "hci_davidson": (C.c_int, [_P, _P, _P, C.POINTER(DavOpts), C.POINTER(C.c_double), _P,
                                C.POINTER(DavStats)]),
+    "hci_mlp_init": (C.c_int, [_P, C.c_uint64]),
+    "hci_mlp_set_params": (C.c_int, [_P, _P, _P, _P]),
+    "hci_mlp_get_params": (C.c_int, [_P, _P, _P, _P]),
+    "hci_mlp_forward": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int]),
+    "hci_mlp_loss_grad": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, C.POINTER(C.c_double), C.c_int]),
+    "hci_mlp_default_train_opts": (None, [C.POINTER(MlpTrainOpts)]),
+    "hci_mlp_train": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.POINTER(MlpTrainOpts), C.POINTER(MlpTrainStats)]),
+    "hci_mlp_collect": (C.c_int, [_P, _P, _P, C.POINTER(DavOpts), C.c_int64, _P, _P, _P, C.c_int64,
+                                  C.POINTER(C.c_int64)]),
     "hci_dev_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "hci_dev_free": (C.c_int, [_P, _P]),
     "hci_memcpy": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
@@ -245,14 +268,77 @@ class HciContext:
                                          n_stored, C.byref(opts), C.byref(out), C.byref(st)))
         return SparseMatUpperTri(self, out.value, st)
 
+    # -- learned preconditioner ------------------------------------------------------
+    def mlp_init(self, seed: int = 7) -> None:
+        self._check(self.lib.hci_mlp_init(self.handle, seed))
+
+    def mlp_set_params(self, params: np.ndarray, mean: np.ndarray, std: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        mu = np.ascontiguousarray(mean, dtype=np.float32)
+        sd = np.ascontiguousarray(std, dtype=np.float32)
+        assert p.size == MLP_NPARAM and mu.size == MLP_NFEAT and sd.size == MLP_NFEAT
+        self._check(self.lib.hci_mlp_set_params(self.handle, _ptr(p), _ptr(mu), _ptr(sd)))
+
+    def mlp_get_params(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        p = np.empty(MLP_NPARAM, dtype=np.float32)
+        mu = np.empty(MLP_NFEAT, dtype=np.float32)
+        sd = np.empty(MLP_NFEAT, dtype=np.float32)
+        self._check(self.lib.hci_mlp_get_params(self.handle, _ptr(p), _ptr(mu), _ptr(sd)))
+        return p, mu, sd
+
+    def mlp_forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.empty(X.shape[0], dtype=np.float32)
+        self._check(self.lib.hci_mlp_forward(self.handle, _ptr(X), X.shape[0], _ptr(out), MEM_HOST))
+        return out
+
+    def mlp_loss_grad(self, X, m, y) -> tuple[float, np.ndarray]:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        m = np.ascontiguousarray(m, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.hci_mlp_loss_grad(self.handle, _ptr(X), _ptr(m), _ptr(y), X.shape[0], _ptr(g),
+                                               C.byref(loss), MEM_HOST))
+        return loss.value, g
+
+    def mlp_train(self, X, m, y, epochs: int = 20, batch: int = 4096, lr: float = 1e-3, seed: int = 7,
+                  standardize: bool = True) -> dict:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        m = np.ascontiguousarray(m, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        o = MlpTrainOpts()
+        self.lib.hci_mlp_default_train_opts(C.byref(o))
+        o.epochs, o.batch, o.lr, o.seed, o.standardize, o.mem = epochs, batch, lr, seed, int(standardize), MEM_HOST
+        st = MlpTrainStats()
+        self._check(self.lib.hci_mlp_train(self.handle, _ptr(X), _ptr(m), _ptr(y), X.shape[0], C.byref(o),
+                                           C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def mlp_collect(self, h: "SparseMatUpperTri", x_exact: np.ndarray, per_iter: int, cap: int,
+                    max_iters: int = 300, tol: float = 1e-5, etol: float = 1e-9):
+        x_exact = np.ascontiguousarray(x_exact, dtype=np.float64)
+        opts = DavOpts()
+        self.lib.hci_dav_default_opts(C.byref(opts))
+        opts.tol, opts.etol, opts.max_iters = tol, etol, max_iters
+        X = np.empty((cap, MLP_NFEAT), dtype=np.float32)
+        m = np.empty(cap, dtype=np.float32)
+        y = np.empty(cap, dtype=np.float32)
+        cnt = C.c_int64()
+        self._check(self.lib.hci_mlp_collect(self.handle, h.handle, _ptr(x_exact), C.byref(opts), per_iter,
+                                             _ptr(X), _ptr(m), _ptr(y), cap, C.byref(cnt)))
+        k = cnt.value
+        return X[:k], m[:k], y[:k]
+
     def davidson(self, h: "SparseMatUpperTri", init: np.ndarray | None = None, tol: float = 1e-4,
                  etol: float = 1e-8, max_iters: int = 50, max_sub: int = 11,
                  precond: int = PRECOND_DPR, mixed: bool = False,
-                 refine_tol: float = 1e-4) -> "DavidsonResult":
+                 refine_tol: float = 1e-4, restart_keep: int = 1) -> "DavidsonResult":
         opts = DavOpts()
         self.lib.hci_dav_default_opts(C.byref(opts))
         opts.tol, opts.etol, opts.max_iters, opts.max_sub = tol, etol, max_iters, max_sub
         opts.precond, opts.mixed, opts.refine_tol, opts.mem = precond, int(mixed), refine_tol, MEM_HOST
+        opts.restart_keep = restart_keep
         coeffs = np.zeros(h.n)
         e0 = C.c_double(0.0)
         st = DavStats()

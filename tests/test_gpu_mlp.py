@@ -1,0 +1,112 @@
+"""Learned preconditioner on the GPU: BF16-MFMA forward/backward against the
+FP64 oracle (with the same bf16 rounding points), training, and its use inside
+Davidson (same E0 as the diagonal correction, fallback when it fails)."""
+import numpy as np
+import pytest
+
+from helpers import csr_to_dense, mol_space
+from oracle import mlp_oracle as mo
+from risq_amd.hci import PRECOND_MLP, HciError
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_net(seed, scale=0.15):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(mo.NPARAM) * scale).astype(np.float32)
+    mu = (rng.standard_normal(mo.F) * 0.2).astype(np.float32)
+    sd = (1.0 + rng.random(mo.F)).astype(np.float32)
+    return p, mu, sd
+
+
+def test_mfma_forward_matches_oracle(gpu_ctx):
+    p, mu, sd = _rand_net(1)
+    gpu_ctx.mlp_set_params(p, mu, sd)
+    rng = np.random.default_rng(2)
+    for n in (1, 17, 1000):
+        X = rng.standard_normal((n, mo.F)).astype(np.float32)
+        out = gpu_ctx.mlp_forward(X)
+        ref, _ = mo.forward(p, X, mu, sd, emulate=True)
+        err = np.abs(out - ref) / (np.abs(ref) + 1e-3)
+        assert np.median(err) < 1e-4 and err.max() < 2e-2, (n, np.median(err), err.max())
+        # and close to the unrounded FP64 network at bf16 precision
+        ref64, _ = mo.forward(p, X, mu, sd, emulate=False)
+        assert np.max(np.abs(out - ref64)) < 5e-2 * (np.max(np.abs(ref64)) + 1e-3)
+
+
+def test_mfma_gradients_match_oracle(gpu_ctx):
+    p, mu, sd = _rand_net(3)
+    gpu_ctx.mlp_set_params(p, mu, sd)
+    rng = np.random.default_rng(4)
+    n = 777
+    X = rng.standard_normal((n, mo.F)).astype(np.float32)
+    m = rng.standard_normal(n).astype(np.float32)
+    y = rng.standard_normal(n).astype(np.float32)
+    loss, g = gpu_ctx.mlp_loss_grad(X, m, y)
+    rloss, rg = mo.loss_grad(p, X, m, y, mu, sd, emulate=True)
+    assert abs(loss - rloss) < 1e-4 * abs(rloss)
+    offs = [0, mo.F * mo.H, mo.F * mo.H + mo.H, mo.F * mo.H + mo.H + mo.H * mo.H,
+            mo.F * mo.H + 2 * mo.H + mo.H * mo.H, mo.NPARAM - 1, mo.NPARAM]
+    for a, b in zip(offs[:-1], offs[1:]):
+        rel = np.linalg.norm(g[a:b] - rg[a:b]) / (np.linalg.norm(rg[a:b]) + 1e-12)
+        assert rel < 2e-3, (a, b, rel)
+
+
+def test_training_reduces_loss(gpu_ctx):
+    rng = np.random.default_rng(5)
+    n = 60000
+    X = rng.standard_normal((n, mo.F)).astype(np.float32)
+    m = rng.standard_normal(n).astype(np.float32)
+    g_true = 1.0 + 0.8 * np.tanh(X[:, 0]) - 0.3 * X[:, 3] * (X[:, 7] > 0)
+    y = (g_true * m).astype(np.float32)
+    gpu_ctx.mlp_init(11)
+    st = gpu_ctx.mlp_train(X, m, y, epochs=8, batch=2048, lr=2e-3)
+    diagonal = float(np.mean((m - y) ** 2))  # loss of g = 1, i.e. the plain diagonal correction
+    assert st["last_epoch_loss"] < 0.1 * diagonal, (st, diagonal)
+    # the trained network also predicts held-out rows
+    Xt = rng.standard_normal((5000, mo.F)).astype(np.float32)
+    gt = 1.0 + 0.8 * np.tanh(Xt[:, 0]) - 0.3 * Xt[:, 3] * (Xt[:, 7] > 0)
+    pred = 1.0 + 2.0 * np.tanh(gpu_ctx.mlp_forward(Xt))
+    assert np.mean((pred - gt) ** 2) < 0.1 * np.mean((gt - 1.0) ** 2)
+
+
+def test_mlp_in_davidson_keeps_the_answer(gpu_ctx):
+    ints, dets = mol_space(16, 32, 0, 4, 4, 1500)
+    gpu_ctx.set_integrals(ints)
+    h = gpu_ctx.build_h(dets, gpu_ctx.ham_diag(dets))
+    ref = np.linalg.eigvalsh(csr_to_dense(*h.download()))[0]
+    # collect training rows from this system and train briefly
+    base = gpu_ctx.davidson(h, tol=1e-7, etol=1e-11, max_iters=400)
+    X, m, y = gpu_ctx.mlp_collect(h, base.coeffs, per_iter=1500, cap=200000)
+    assert X.shape[0] > 10000 and np.all(np.isfinite(X)) and np.all(np.isfinite(y))
+    gpu_ctx.mlp_init(3)
+    gpu_ctx.mlp_train(X, m, y, epochs=5, batch=2048)
+    res = gpu_ctx.davidson(h, tol=1e-5, etol=1e-9, max_iters=400, precond=PRECOND_MLP)
+    assert res.converged and abs(res.energy - ref) < 1e-6
+    assert res.stats["mlp_applied"] > 0
+
+
+def test_bad_network_falls_back(gpu_ctx):
+    """An untrained network with large random weights scales every component of
+    the correction by noise; steps that fail to reduce the residual are replaced
+    by the diagonal correction, and the answer is unchanged."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 1500)
+    gpu_ctx.set_integrals(ints)
+    h = gpu_ctx.build_h(dets, gpu_ctx.ham_diag(dets))
+    ref = np.linalg.eigvalsh(csr_to_dense(*h.download()))[0]
+    p, mu, sd = _rand_net(9, scale=1.0)
+    gpu_ctx.mlp_set_params(p, mu, sd)
+    res = gpu_ctx.davidson(h, tol=1e-5, etol=1e-9, max_iters=800, precond=PRECOND_MLP)
+    assert res.converged and abs(res.energy - ref) < 1e-6
+    assert res.stats["mlp_fallbacks"] > 0
+
+
+def test_mlp_requires_a_network():
+    from risq_amd.hci import HciContext
+    ctx = HciContext(0)
+    ints, dets = mol_space(16, 32, 0, 4, 4, 300)
+    ctx.set_integrals(ints)
+    h = ctx.build_h(dets, ctx.ham_diag(dets))
+    with pytest.raises(HciError, match="no network"):
+        ctx.davidson(h, precond=PRECOND_MLP)
+    ctx.close()

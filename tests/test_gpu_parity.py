@@ -286,3 +286,15 @@ def test_row_accessor_matches_download(gpu_ctx, oracle_lib):
         c, v = h.row(i)
         assert np.array_equal(c, col[rp[i]:rp[i + 1]])
         assert np.array_equal(v, val[rp[i]:rp[i + 1]])
+
+
+def test_thick_restart_same_energy(gpu_ctx, oracle_lib):
+    ints, dets = mol_space(16, 32, 0, 4, 4, 1500)
+    gpu_ctx.set_integrals(ints)
+    h = gpu_ctx.build_h(dets, gpu_ctx.ham_diag(dets))
+    ref = np.linalg.eigvalsh(csr_to_dense(*h.download()))[0]
+    a = gpu_ctx.davidson(h, tol=1e-5, etol=1e-9, max_iters=400)
+    b = gpu_ctx.davidson(h, tol=1e-5, etol=1e-9, max_iters=400, restart_keep=2)
+    assert a.converged and b.converged
+    assert abs(a.energy - ref) < 1e-6 and abs(b.energy - ref) < 1e-6
+    assert b.stats["iters"] <= a.stats["iters"]
