@@ -65,6 +65,12 @@ extern "C" {
                              reproducible, 2x the value memory */
 #define HCI_STORE_UPPER 1 /* upper triangle only: sigma scatters the transpose
                              half with FP64 atomics */
+#define HCI_STORE_TILED 2 /* both triangles in string-group tiles: a tile of rows
+                             of one alpha (or beta) string group holds its
+                             entries per column group with 16-bit local row and
+                             column indices; sigma stages x slices in LDS and
+                             sums rows in LDS (order-dependent rounding).  No
+                             incremental builds and no hci_csr_row. */
 
 /* which singles are stored (doubles are always stored iff H != 0,
  * src/var/off_diag.rs:33) */
@@ -86,7 +92,7 @@ typedef struct hci_build_opts {
     int value_prec;  /* HCI_PREC_F64 | HCI_PREC_F32 */
     int single_rule; /* HCI_SINGLES_* */
     int mem;         /* where dets/diag live */
-    int storage;     /* HCI_STORE_FULL | HCI_STORE_UPPER */
+    int storage;     /* HCI_STORE_FULL | HCI_STORE_UPPER | HCI_STORE_TILED */
 } hci_build_opts;
 
 typedef struct hci_build_stats {

@@ -33,6 +33,7 @@
 
 #include <rocprim/rocprim.hpp>
 
+#include <algorithm>
 #include <chrono>
 
 #include "hci_internal.h"
@@ -234,6 +235,74 @@ __global__ void __launch_bounds__(kBlock)
     }
 #undef HCI_EMIT
     if (!FILL && lane == 0) row_cnt[p] = cnt + pc;
+}
+
+// ---- TILED storage --------------------------------------------------------------
+
+// One wavefront per sub-block; pairs are enumerated column-major (f = col * nr
+// + row), so the emitted entries of a sub-block are ordered by column, then row.
+template <class S, bool FILL>
+__global__ void __launch_bounds__(kBlock)
+    k_tile_connect(const SubBlock* __restrict__ sbs, int64_t n_sb, BuildView<S> B, const S* ub, DevInts T,
+                   int single_rule, int64_t* cnt_out, FillOut out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (w >= n_sb) return;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const SubBlock sb = sbs[w];
+    const int32_t P = sb.nr * sb.nc;
+    int64_t cnt = 0;
+    const int64_t base = FILL ? sb.off : 0;
+    for (int32_t fb = 0; fb < P; fb += 64) {
+        const int32_t f = fb + lane;
+        bool keep = false;
+        double v = 0.0;
+        int32_t rl = 0, cl = 0;
+        if (f < P) {
+            cl = f / sb.nr;
+            rl = f - cl * sb.nr;
+            const int32_t p = sb.r0 + rl, q = sb.c0 + cl;
+            if (sb.kind == 2) {
+                const int32_t i = B.sb_idx[p], j = B.sb_idx[q];
+                S a = B.sb_up[p], b = B.sb_up[q];
+                if (popc(a ^ b) == 4) {
+                    swap_if(j < i, a, b);
+                    v = elem_same_double<S>(T, a, b);
+                    keep = (v != 0.0);
+                }
+            } else {
+                const int32_t i = B.sa_idx[p], j = B.sa_idx[q];
+                S da = B.sa_dn[p], db = B.sa_dn[q];
+                const int c = popc(da ^ db);
+                if (sb.kind == 0) {
+                    const S up = B.ua[sb.g_row];
+                    swap_if(j < i, da, db);
+                    if (c == 2) {
+                        v = elem_single<S>(T, da, db, up);
+                        keep = keep_single(v, single_rule);
+                    } else if (c == 4) {
+                        v = elem_same_double<S>(T, da, db);
+                        keep = (v != 0.0);
+                    }
+                } else {
+                    S ua = B.ua[sb.g_row], uc = B.ua[sb.g_col];
+                    swap_if(j < i, ua, uc);
+                    swap_if(j < i, da, db);
+                    if (c == 0) {
+                        v = elem_single<S>(T, ua, uc, da);
+                        keep = keep_single(v, single_rule);
+                    } else if (c == 2) {
+                        v = elem_opp_double<S>(T, ua, uc, da, db);
+                        keep = (v != 0.0);
+                    }
+                }
+            }
+        }
+        const uint64_t m = __ballot(keep);
+        if (FILL && keep) put(out, base + cnt + __popcll(m & lt_mask), (int32_t)(((uint32_t)rl << 16) | (uint32_t)cl), v);
+        cnt += __popcll(m);
+    }
+    if (!FILL && lane == 0) cnt_out[w] = cnt;
 }
 
 // ---- sorting / bucketing helpers ------------------------------------------------
@@ -521,6 +590,113 @@ int rank_strings(hci_ctx* c, Scratch& tmp, DevBufs& db, const S* str, int64_t N,
     return HCI_OK;
 }
 
+// Sub-block bookkeeping on the host from the group tables, then count and
+// fill passes on the device.  Parts: alpha-major tiles of every alpha group
+// (columns: the group itself and its single-connected groups) and beta-major
+// tiles of every beta group (columns: the group itself).
+template <class S>
+int build_tiled(hci_ctx* c, Scratch& tmp, const BuildView<S>& B, const S* ub, int32_t Ua, int32_t Ub,
+                int32_t n_pairs, const hci_build_opts& o, hci_csr* H) {
+    hipStream_t s = c->stream;
+    const int64_t N = B.N;
+    std::vector<int32_t> ga((size_t)Ua + 1), gb((size_t)Ub + 1), no((size_t)Ua + 1), nb((size_t)n_pairs + 1);
+    HIP_TRY(c, hipMemcpy(ga.data(), B.ga_off, ga.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(gb.data(), B.gb_off, gb.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(no.data(), B.nbr_off, no.size() * 4, hipMemcpyDeviceToHost));
+    if (n_pairs > 0) HIP_TRY(c, hipMemcpy(nb.data(), B.nbr, (size_t)n_pairs * 4, hipMemcpyDeviceToHost));
+    std::vector<SubBlock> sbs;
+    std::vector<TileTask> ta, tb;
+    sbs.reserve((size_t)n_pairs + (size_t)Ua + (size_t)Ub + 16);
+    auto add_cols = [&](int32_t r0, int32_t nr, int32_t g_row, int32_t g_col, int kind, int32_t c0, int32_t c1) {
+        for (int32_t cs = c0; cs < c1; cs += kTileCols) {
+            SubBlock b{};
+            b.r0 = r0;
+            b.nr = nr;
+            b.c0 = cs;
+            b.nc = std::min(kTileCols, c1 - cs);
+            b.g_row = g_row;
+            b.g_col = g_col;
+            b.kind = kind;
+            sbs.push_back(b);
+        }
+    };
+    std::vector<int32_t> cols;
+    for (int32_t g = 0; g < Ua; ++g) {
+        cols.assign(nb.begin() + no[g], nb.begin() + no[g + 1]);
+        cols.insert(std::lower_bound(cols.begin(), cols.end(), g), g);
+        for (int32_t r0 = ga[g]; r0 < ga[g + 1]; r0 += kTileRows) {
+            const int32_t nr = std::min(kTileRows, ga[g + 1] - r0);
+            TileTask t{r0, nr, (int32_t)sbs.size(), 0};
+            for (int32_t cg : cols) add_cols(r0, nr, g, cg, cg == g ? 0 : 1, ga[cg], ga[cg + 1]);
+            t.sb_end = (int32_t)sbs.size();
+            ta.push_back(t);
+        }
+    }
+    for (int32_t g = 0; g < Ub; ++g) {
+        for (int32_t r0 = gb[g]; r0 < gb[g + 1]; r0 += kTileRows) {
+            const int32_t nr = std::min(kTileRows, gb[g + 1] - r0);
+            TileTask t{r0, nr, (int32_t)sbs.size(), 0};
+            if (gb[g + 1] - gb[g] > 1) add_cols(r0, nr, g, g, 2, gb[g], gb[g + 1]);
+            t.sb_end = (int32_t)sbs.size();
+            tb.push_back(t);
+        }
+    }
+    const int64_t n_sb = (int64_t)sbs.size();
+    if (n_sb >= (int64_t)INT32_MAX) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many tiles");
+    HIP_TRY(c, hipMalloc(&H->d_sb, (size_t)n_sb * sizeof(SubBlock) + 16));
+    HIP_TRY(c, hipMemcpy(H->d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice));
+    int64_t* cnt = nullptr;
+    HIP_TRY(c, hipMalloc(&cnt, (size_t)n_sb * 8 + 16));
+    const DevInts T = c->ints();
+    FillOut none{nullptr, nullptr, nullptr, nullptr};
+    if (n_sb > 0)
+        hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, H->d_sb, n_sb, B, ub, T,
+                           o.single_rule, cnt, none);
+    std::vector<int64_t> hc((size_t)n_sb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && n_sb > 0) e = hipMemcpy(hc.data(), cnt, (size_t)n_sb * 8, hipMemcpyDeviceToHost);
+    hipFree(cnt);
+    HIP_TRY(c, e);
+    int64_t nnz = 0;
+    for (int64_t k = 0; k < n_sb; ++k) {
+        sbs[k].off = nnz;
+        sbs[k].cnt = hc[k];
+        sbs[k].stage = hc[k] >= sbs[k].nc ? 1 : 0;
+        nnz += hc[k];
+    }
+    HIP_TRY(c, hipMemcpy(H->d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice));
+    H->nnz_s = nnz;
+    H->nnz_u = nnz / 2;
+    H->n_sb = n_sb;
+    HIP_TRY(c, hipMalloc(&H->d_col, (size_t)nnz * 4 + 16));
+    FillOut fo{H->d_col, nullptr, nullptr, nullptr};
+    if (o.value_prec == HCI_PREC_F64) {
+        HIP_TRY(c, hipMalloc(&H->d_val, (size_t)nnz * 8 + 16));
+        fo.val = H->d_val;
+    } else {
+        HIP_TRY(c, hipMalloc(&H->d_hi, (size_t)nnz * 4 + 16));
+        HIP_TRY(c, hipMalloc(&H->d_lo, (size_t)nnz * 4 + 16));
+        fo.hi = H->d_hi;
+        fo.lo = H->d_lo;
+    }
+    if (n_sb > 0)
+        hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, H->d_sb, n_sb, B, ub, T,
+                           o.single_rule, nullptr, fo);
+    HIP_TRY(c, hipGetLastError());
+    H->n_task_a = (int64_t)ta.size();
+    H->n_task_b = (int64_t)tb.size();
+    HIP_TRY(c, hipMalloc(&H->d_task_a, ta.size() * sizeof(TileTask) + 16));
+    HIP_TRY(c, hipMalloc(&H->d_task_b, tb.size() * sizeof(TileTask) + 16));
+    HIP_TRY(c, hipMemcpy(H->d_task_a, ta.data(), ta.size() * sizeof(TileTask), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(H->d_task_b, tb.data(), tb.size() * sizeof(TileTask), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMalloc(&H->d_bpos, (size_t)N * 4 + 16));
+    HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(c, hipMalloc(&H->d_xb, (size_t)N * 8 + 16));
+    HIP_TRY(c, hipMalloc(&H->d_yb, (size_t)N * 8 + 16));
+    (void)tmp;
+    return HCI_OK;
+}
+
 template <class S>
 int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int64_t N,
                const hci_csr* prev, int64_t n_stored, const hci_build_opts& o, hci_csr** out,
@@ -693,6 +869,24 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     B.nbr = nbr;
     const DevInts T = c->ints();
 
+    if (o.storage == HCI_STORE_TILED) {
+        int trc = build_tiled<S>(c, tmp, B, ub, Ua, Ub, n_pairs, o, H);
+        if (trc) return fail_free(trc);
+        BTH(hipMalloc(&H->d_diag, (size_t)N * 8 + 16));
+        hipLaunchKernelGGL(k_gather_f64, grid_for(N, 256), 256, 0, s, sa_idx, diag_dev, N, H->d_diag);
+        BTH(hipMalloc(&H->d_words, (size_t)N * 32 + 16));
+        hipLaunchKernelGGL(k_gather_words, grid_for(4 * N, 256), 256, 0, s, sa_idx, dets_dev, N, H->d_words);
+        BTH(hipStreamSynchronize(s));
+        if (stats) {
+            stats->nnz_u = H->nnz_u;
+            stats->n_alpha = Ua;
+            stats->n_beta = Ub;
+            stats->alpha_pairs = n_pairs;
+        }
+        *out = H;
+        return HCI_OK;
+    }
+
     // previous build (incremental form): per-row counts of its renumbered entries
     int64_t* prev_cnt = nullptr;
     const int32_t n_prev = prev ? (int32_t)prev->n : 0;
@@ -831,8 +1025,10 @@ extern "C" int hci_build_h(hci_ctx* c, const uint64_t* dets, const double* diag,
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: value_prec must be 64 or 32");
     if (opts->single_rule != HCI_SINGLES_ABS_GT_1E12 && opts->single_rule != HCI_SINGLES_NONZERO)
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: bad single_rule");
-    if (opts->storage != HCI_STORE_FULL && opts->storage != HCI_STORE_UPPER)
+    if (opts->storage != HCI_STORE_FULL && opts->storage != HCI_STORE_UPPER && opts->storage != HCI_STORE_TILED)
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: bad storage");
+    if (opts->storage == HCI_STORE_TILED && prev)
+        return hci_fail(c, HCI_ERR_ARG, "hci_build_h: HCI_STORE_TILED builds are not incremental");
     if (prev) {
         if (prev->ctx != c) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: prev from another context");
         if (n_stored != prev->n || n < prev->n)

@@ -283,7 +283,7 @@ int fetch_internal(const hci_csr* h, std::vector<int64_t>& rp, std::vector<int32
     col.resize((size_t)h->nnz_s);
     val.resize((size_t)h->nnz_s);
     perm.resize((size_t)h->n);
-    HIP_TRY(c, hipMemcpy(rp.data(), h->d_row_ptr, rp.size() * 8, hipMemcpyDeviceToHost));
+    if (h->d_row_ptr) HIP_TRY(c, hipMemcpy(rp.data(), h->d_row_ptr, rp.size() * 8, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(perm.data(), h->d_perm, perm.size() * 4, hipMemcpyDeviceToHost));
     if (h->nnz_s == 0) return HCI_OK;
     HIP_TRY(c, hipMemcpy(col.data(), h->d_col, col.size() * 4, hipMemcpyDeviceToHost));
@@ -312,13 +312,34 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::vector<int64_t> rp;
-    std::vector<int32_t> ic, perm;
+    std::vector<int32_t> ic, perm, bpos;
     std::vector<double> iv;
+    std::vector<SubBlock> sbs;
     int rc = fetch_internal(h, rp, ic, iv, perm);
     if (rc) return rc;
     const int64_t n = h->n;
+    if (h->storage == HCI_STORE_TILED) {
+        sbs.resize((size_t)h->n_sb);
+        bpos.resize((size_t)n);
+        if (h->n_sb) HIP_TRY(c, hipMemcpy(sbs.data(), h->d_sb, sbs.size() * sizeof(SubBlock), hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(bpos.data(), h->d_bpos, (size_t)n * 4, hipMemcpyDeviceToHost));
+    }
     std::vector<int64_t> cnt((size_t)n + 1, 0);
     auto each = [&](auto f) {
+        if (h->storage == HCI_STORE_TILED) {
+            for (const SubBlock& b : sbs)
+                for (int64_t e = b.off; e < b.off + b.cnt; ++e) {
+                    const uint32_t u = (uint32_t)ic[e];
+                    int32_t p = b.r0 + (int32_t)(u >> 16), q = b.c0 + (int32_t)(u & 0xFFFFu);
+                    if (b.kind == 2) {
+                        p = bpos[p];
+                        q = bpos[q];
+                    }
+                    const int32_t i = perm[p], j = perm[q];
+                    if (i < j) f(i, j, iv[e]);
+                }
+            return;
+        }
         for (int64_t p = 0; p < n; ++p)
             for (int64_t e = rp[p]; e < rp[p + 1]; ++e) {
                 int32_t i = perm[p], j = perm[ic[e]];
@@ -417,6 +438,12 @@ void hci_csr_free(hci_csr* h) {
     hipFree(h->d_perm);
     hipFree(h->d_pos);
     hipFree(h->d_words);
+    hipFree(h->d_sb);
+    hipFree(h->d_task_a);
+    hipFree(h->d_task_b);
+    hipFree(h->d_bpos);
+    hipFree(h->d_xb);
+    hipFree(h->d_yb);
     delete h;
 }
 

@@ -11,6 +11,28 @@
 
 struct MlpState;  // learned preconditioner (mlp.hip)
 
+// TILED storage.  A sub-block holds the entries between nr rows starting at r0
+// and nc columns starting at c0 (both in the part's order: alpha-major for
+// kinds 0/1, beta-major for kind 2), column-major, each entry a packed
+// (local_row << 16 | local_col) index plus its value.
+//   kind 0: rows and columns in the same alpha group (beta singles/doubles)
+//   kind 1: columns in a single-connected alpha group (alpha singles with equal
+//           beta, opposite-spin doubles)
+//   kind 2: rows and columns in the same beta group, beta-major (alpha doubles)
+// stage = 1 when the x slice is worth staging in LDS (entries >= columns).
+constexpr int kTileRows = 256;
+constexpr int kTileCols = 4096;
+struct SubBlock {
+    int32_t r0, nr, c0, nc;
+    int32_t g_row, g_col;
+    int32_t kind, stage;
+    int64_t off, cnt;
+};
+// one workgroup: rows [r0, r0+nr) and sub-blocks [sb_begin, sb_end)
+struct TileTask {
+    int32_t r0, nr, sb_begin, sb_end;
+};
+
 struct hci_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -69,6 +91,16 @@ struct hci_csr {
     int64_t n_items = 0;
     // per-row absolute sums (learned-preconditioner feature), lazily built
     double* d_rownorm = nullptr;
+    // TILED storage (d_col holds the packed local indices)
+    SubBlock* d_sb = nullptr;
+    int64_t n_sb = 0;
+    TileTask* d_task_a = nullptr;  // alpha-major tiles (kinds 0, 1)
+    int64_t n_task_a = 0;
+    TileTask* d_task_b = nullptr;  // beta-major tiles (kind 2)
+    int64_t n_task_b = 0;
+    int32_t* d_bpos = nullptr;     // beta-major position -> internal position
+    double* d_xb = nullptr;        // work vectors in beta-major order
+    double* d_yb = nullptr;
 };
 
 // status helpers ------------------------------------------------------------

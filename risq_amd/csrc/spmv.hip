@@ -12,6 +12,7 @@
 //     reproducible.
 //   UPPER storage: the transpose term H_pq x_p goes to y_q with a hardware FP64
 //     atomic (order-dependent rounding).
+//   TILED storage: k_spmv_tiles, x slices staged in LDS, rows summed in LDS.
 // y starts as d o x in both cases.
 #include "hci_internal.h"
 
@@ -203,6 +204,67 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// TILED storage: one workgroup per tile.  For each sub-block the x slice is
+// staged in LDS when dense enough, and every entry adds H_pq x_q to an LDS row
+// accumulator (hardware FP64 LDS atomics; entries are column-major, so the
+// lanes of one step mostly hit different rows).  The tile then writes its
+// rows once: y_p = d_p x_p + acc_p (alpha-major part) or acc_p (beta-major part).
+template <int MODE, bool ABS>
+__global__ void __launch_bounds__(256)
+    k_spmv_tiles(const TileTask* __restrict__ tasks, const SubBlock* __restrict__ sbs,
+                 const int32_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
+                 const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
+                 double* __restrict__ y) {
+    __shared__ double acc[kTileRows];
+    __shared__ double xs[kTileCols];
+    const TileTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x;
+    for (int r = tid; r < t.nr; r += 256) acc[r] = 0.0;
+    __syncthreads();
+    for (int32_t k = t.sb_begin; k < t.sb_end; ++k) {
+        const SubBlock b = sbs[k];
+        if (b.cnt == 0) continue;
+        if (b.stage) {
+            for (int j = tid; j < b.nc; j += 256) xs[j] = ABS ? 1.0 : x[b.c0 + j];
+            __syncthreads();
+        }
+        const int64_t e1 = b.off + b.cnt;
+        int64_t e = b.off + tid;
+        for (; e + 768 < e1; e += 1024) {
+            uint32_t u[4];
+            double v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                u[q] = (uint32_t)__builtin_nontemporal_load(idx + e + 256 * q);
+                v[q] = load_val<MODE>(val, hi, lo, e + 256 * q);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int lc = (int)(u[q] & 0xFFFFu), lr = (int)(u[q] >> 16);
+                const double xv = ABS ? 1.0 : (b.stage ? xs[lc] : x[b.c0 + lc]);
+                atomicAdd(&acc[lr], ABS ? fabs(v[q]) : v[q] * xv);
+            }
+        }
+        for (; e < e1; e += 256) {
+            const uint32_t uu = (uint32_t)__builtin_nontemporal_load(idx + e);
+            const double vv = load_val<MODE>(val, hi, lo, e);
+            const int lc = (int)(uu & 0xFFFFu), lr = (int)(uu >> 16);
+            const double xv = ABS ? 1.0 : (b.stage ? xs[lc] : x[b.c0 + lc]);
+            atomicAdd(&acc[lr], ABS ? fabs(vv) : vv * xv);
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < t.nr; r += 256) {
+        const int64_t p = (int64_t)t.r0 + r;
+        y[p] = (diag && !ABS) ? diag[p] * x[p] + acc[r] : acc[r];
+    }
+}
+
+__global__ void k_scatter_add(const int32_t* pos, const double* src, int64_t n, double* dst) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[pos[q]] += src[q];
+}
+
 __global__ void k_gather(const int32_t* perm, const double* src, int64_t n, double* dst) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) dst[p] = src[perm[p]];
@@ -229,6 +291,7 @@ int spmv_prepare(hci_csr* h) {
     hipFree(h->d_item_row);
     h->d_chunk_row = nullptr;
     h->d_item_row = nullptr;
+    if (h->storage == HCI_STORE_TILED) return HCI_OK;
     if (h->storage == HCI_STORE_UPPER) {
         h->n_chunks = (h->nnz_s + kSpmvChunk - 1) / kSpmvChunk;
         HIP_TRY(c, hipMalloc(&h->d_chunk_row, (size_t)(h->n_chunks + 1) * 4 + 16));
@@ -268,6 +331,28 @@ namespace {
 template <int MODE>
 void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
     hipStream_t s = h->ctx->stream;
+    if (h->storage == HCI_STORE_TILED) {
+        // beta-major part first (x gathered to beta-major order), then the
+        // alpha-major part writes y, then the beta-major rows are added
+        if (!absval) hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, s, h->d_bpos, x, h->n, h->d_xb);
+        if (h->n_task_b > 0) {
+            if (absval)
+                hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_b, 256, 0, s, h->d_task_b, h->d_sb,
+                                   h->d_col, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);
+            else
+                hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_b, 256, 0, s, h->d_task_b,
+                                   h->d_sb, h->d_col, h->d_val, h->d_hi, h->d_lo, h->d_xb, nullptr, h->d_yb);
+        }
+        if (absval)
+            hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_a, 256, 0, s, h->d_task_a, h->d_sb,
+                               h->d_col, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, y);
+        else
+            hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_a, 256, 0, s, h->d_task_a, h->d_sb,
+                               h->d_col, h->d_val, h->d_hi, h->d_lo, x, h->d_diag, y);
+        if (h->n_task_b > 0)
+            hipLaunchKernelGGL(k_scatter_add, grid_for(h->n, 256), 256, 0, s, h->d_bpos, h->d_yb, h->n, y);
+        return;
+    }
     if (h->storage == HCI_STORE_FULL) {
         const unsigned grid = grid_for(h->n_items, 4);
         if (absval)

@@ -36,6 +36,7 @@ PRECOND_DPR = 0
 PRECOND_MLP = 1
 STORE_FULL = 0
 STORE_UPPER = 1
+STORE_TILED = 2
 
 
 class HciError(RuntimeError):

@@ -9,7 +9,7 @@ import pytest
 
 from helpers import csr_to_dense, hubbard_space, mol_space
 from risq_amd.dets import from_orbitals, full_space
-from risq_amd.hci import PREC_F32, SIGMA_F32, SINGLES_NONZERO, STORE_FULL, STORE_UPPER, HciError
+from risq_amd.hci import PREC_F32, SIGMA_F32, SINGLES_NONZERO, STORE_FULL, STORE_TILED, STORE_UPPER, HciError
 
 pytestmark = pytest.mark.gpu
 
@@ -57,7 +57,7 @@ def test_elements_bit_exact(gpu_ctx, oracle_lib):
     assert np.array_equal(gv[nz].view(np.uint64), gv2[nz].view(np.uint64))
 
 
-@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER])
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER, STORE_TILED])
 @pytest.mark.parametrize("space", [MOL0, (16, 32, 3, 3, 5, 2500)])
 def test_build_matches_oracle(gpu_ctx, oracle_lib, space, storage):
     ints, dets = mol_space(*space)
@@ -156,7 +156,7 @@ def test_small_n_single_rule(gpu_ctx, oracle_lib):
     assert np.array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER])
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER, STORE_TILED])
 def test_sigma_matches_oracle(gpu_ctx, oracle_lib, storage):
     ints, dets = mol_space(*MOL0)
     gpu_ctx.set_integrals(ints)
@@ -298,3 +298,35 @@ def test_thick_restart_same_energy(gpu_ctx, oracle_lib):
     assert a.converged and b.converged
     assert abs(a.energy - ref) < 1e-6 and abs(b.energy - ref) < 1e-6
     assert b.stats["iters"] <= a.stats["iters"]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_tiled_storage_large_groups(gpu_ctx, oracle_lib, prec):
+    """Alpha groups larger than a tile (256 rows) and column groups wider than a
+    column window (4096) split into several sub-blocks; sigma and download agree
+    with full storage."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 40000)
+    gpu_ctx.set_integrals(ints)
+    diag = gpu_ctx.ham_diag(dets)
+    f = gpu_ctx.build_h(dets, diag, value_prec=prec)
+    t = gpu_ctx.build_h(dets, diag, value_prec=prec, storage=STORE_TILED)
+    assert t.nnz == f.nnz
+    assert t.build_stats["n_alpha"] == f.build_stats["n_alpha"]
+    x = np.random.default_rng(8).standard_normal(dets.shape[0])
+    for mode in ([0, 1] if prec == 32 else [0]):
+        a, b = f.matrix_vector_prod(x, mode=mode), t.matrix_vector_prod(x, mode=mode)
+        assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
+    for u, v in zip(f.download(), t.download()):
+        assert np.array_equal(u, v)
+    ra = gpu_ctx.davidson(f, tol=1e-5, etol=1e-9, max_iters=400)
+    rb = gpu_ctx.davidson(t, tol=1e-5, etol=1e-9, max_iters=400)
+    assert abs(ra.energy - rb.energy) < 1e-8
+
+
+def test_tiled_rejects_incremental(gpu_ctx):
+    ints, dets = mol_space(*MOL0)
+    gpu_ctx.set_integrals(ints)
+    diag = gpu_ctx.ham_diag(dets)
+    h1 = gpu_ctx.build_h(dets[:100], diag[:100], storage=STORE_TILED)
+    with pytest.raises(HciError, match="not incremental"):
+        gpu_ctx.build_h(dets, diag, prev=h1, n_stored=100, storage=STORE_TILED)
