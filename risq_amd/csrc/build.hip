@@ -58,6 +58,12 @@ struct BuildView {
     const int32_t* sb_pos;  // internal position at each beta-sorted position
     const int32_t* nbr_off;  // single-connected alpha groups of group g: nbr[nbr_off[g]..nbr_off[g+1])
     const int32_t* nbr;      // ascending within each group
+    // tiled builds: for every internal position q, its beta string with one
+    // electron removed (n_beta keys), sorted by key within each alpha group;
+    // group g owns [ga_off[g]*n_beta, ga_off[g+1]*n_beta)
+    const S* bkey;
+    const int32_t* bkq;
+    int n_beta;
 };
 
 __device__ __forceinline__ bool keep_single(double v, int rule) {
@@ -250,9 +256,77 @@ __global__ void __launch_bounds__(kBlock)
     if (w >= n_sb) return;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const SubBlock sb = sbs[w];
-    const int32_t P = sb.nr * sb.nc;
     int64_t cnt = 0;
     const int64_t base = FILL ? sb.off : 0;
+    if (sb.kind == 3) {
+        // work item g = key * nr + row: the row's beta string minus its key-th
+        // electron; every determinant of the column group whose beta string
+        // contains that key has an equal or single-connected beta string
+        const int nbk = B.n_beta;
+        const int32_t G = sb.nr * nbk;
+        const int64_t seg_lo = (int64_t)B.ga_off[sb.g_col] * nbk, seg_hi = (int64_t)B.ga_off[sb.g_col + 1] * nbk;
+        const S ua_row = B.ua[sb.g_row], ua_col = B.ua[sb.g_col];
+        for (int32_t gb = 0; gb < G; gb += 64) {
+            const int32_t g = gb + lane;
+            int32_t rl = 0, kk = 0, i = 0;
+            S b = 0, key = 0;
+            int64_t cur = seg_hi;
+            if (g < G) {
+                kk = g / sb.nr;
+                rl = g - kk * sb.nr;
+                const int32_t p = sb.r0 + rl;
+                i = B.sa_idx[p];
+                b = B.sa_dn[p];
+                S m = b;
+                for (int t = 0; t < kk; ++t) m &= m - (S)1;
+                key = b & ~(((S)1) << ctz(m));
+                int64_t lo = seg_lo, hi = seg_hi;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (B.bkey[mid] < key)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                cur = lo;
+            }
+            while (true) {
+                const bool pending = cur < seg_hi && B.bkey[cur] == key;
+                if (!__ballot(pending)) break;
+                bool keep = false;
+                double v = 0.0;
+                int32_t cl = 0;
+                if (pending) {
+                    const int32_t q = B.bkq[cur];
+                    ++cur;
+                    if (q >= sb.c0 && q < sb.c0 + sb.nc) {
+                        cl = q - sb.c0;
+                        const int32_t j = B.sa_idx[q];
+                        S da = b, db = B.sa_dn[q], u1 = ua_row, u2 = ua_col;
+                        const bool same = (da == db);
+                        if (!same || kk == 0) {  // an equal beta string matches every key: count it once
+                            swap_if(j < i, u1, u2);
+                            swap_if(j < i, da, db);
+                            if (same) {
+                                v = elem_single<S>(T, u1, u2, da);
+                                keep = keep_single(v, single_rule);
+                            } else {
+                                v = elem_opp_double<S>(T, u1, u2, da, db);
+                                keep = (v != 0.0);
+                            }
+                        }
+                    }
+                }
+                const uint64_t mk = __ballot(keep);
+                if (FILL && keep)
+                    put(out, base + cnt + __popcll(mk & lt_mask), (int32_t)(((uint32_t)rl << 16) | (uint32_t)cl), v);
+                cnt += __popcll(mk);
+            }
+        }
+        if (!FILL && lane == 0) cnt_out[w] = cnt;
+        return;
+    }
+    const int32_t P = sb.nr * sb.nc;
     for (int32_t fb = 0; fb < P; fb += 64) {
         const int32_t f = fb + lane;
         bool keep = false;
@@ -457,6 +531,25 @@ __global__ void k_prev_fill(const int64_t* prp, const int32_t* pcol, const doubl
     }
 }
 
+template <class S>
+__global__ void k_beta_r1(const S* sa_dn, int64_t n, int nbk, S* key, int32_t* kq) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const S b = sa_dn[q];
+    S m = b;
+    for (int t = 0; t < nbk; ++t) {
+        const int o = ctz(m);
+        m &= m - (S)1;
+        key[q * nbk + t] = b & ~(((S)1) << o);
+        kq[q * nbk + t] = (int32_t)q;
+    }
+}
+
+__global__ void k_scale_i32(const int32_t* a, int64_t n, int f, int32_t* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] * f;
+}
+
 __global__ void k_split(const double* v, int64_t n, float* hi, float* lo) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -594,11 +687,42 @@ int rank_strings(hci_ctx* c, Scratch& tmp, DevBufs& db, const S* str, int64_t N,
 // fill passes on the device.  Parts: alpha-major tiles of every alpha group
 // (columns: the group itself and its single-connected groups) and beta-major
 // tiles of every beta group (columns: the group itself).
+constexpr int32_t kKeyedCols = 48;  // column groups wider than this use key lookups
+
 template <class S>
-int build_tiled(hci_ctx* c, Scratch& tmp, const BuildView<S>& B, const S* ub, int32_t Ua, int32_t Ub,
+int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t Ua, int32_t Ub,
                 int32_t n_pairs, const hci_build_opts& o, hci_csr* H) {
     hipStream_t s = c->stream;
     const int64_t N = B.N;
+    // beta-minus-one-electron keys, sorted within each alpha group
+    int nbk = 0;
+    {
+        uint64_t w[2];
+        HIP_TRY(c, hipMemcpy(w, B.sa_dn, sizeof(S) <= 8 ? 8 : 16, hipMemcpyDeviceToHost));
+        nbk = __builtin_popcountll(w[0]) + (sizeof(S) > 8 ? __builtin_popcountll(w[1]) : 0);
+    }
+    DevBufs kb;
+    const int64_t NK = N * (int64_t)std::max(nbk, 1);
+    S *k_in = nullptr, *k_out = nullptr;
+    int32_t *q_in = nullptr, *q_out = nullptr, *seg = nullptr;
+    HIP_TRY(c, kb.alloc(&k_in, (size_t)NK));
+    HIP_TRY(c, kb.alloc(&k_out, (size_t)NK));
+    HIP_TRY(c, kb.alloc(&q_in, (size_t)NK));
+    HIP_TRY(c, kb.alloc(&q_out, (size_t)NK));
+    HIP_TRY(c, kb.alloc(&seg, (size_t)Ua + 1));
+    if (nbk > 0) {
+        hipLaunchKernelGGL(k_beta_r1<S>, grid_for(N, 256), 256, 0, s, B.sa_dn, N, nbk, k_in, q_in);
+        hipLaunchKernelGGL(k_scale_i32, grid_for(Ua + 1, 256), 256, 0, s, B.ga_off, (int64_t)Ua + 1, nbk, seg);
+        size_t need = 0;
+        HIP_TRY(c, rocprim::segmented_radix_sort_pairs(nullptr, need, k_in, k_out, q_in, q_out, (size_t)NK,
+                                                       (unsigned)Ua, seg, seg + 1, 0, c->n_orb, s));
+        HIP_TRY(c, tmp.ensure(need));
+        HIP_TRY(c, rocprim::segmented_radix_sort_pairs(tmp.p, need, k_in, k_out, q_in, q_out, (size_t)NK,
+                                                       (unsigned)Ua, seg, seg + 1, 0, c->n_orb, s));
+    }
+    B.bkey = k_out;
+    B.bkq = q_out;
+    B.n_beta = nbk;
     std::vector<int32_t> ga((size_t)Ua + 1), gb((size_t)Ub + 1), no((size_t)Ua + 1), nb((size_t)n_pairs + 1);
     HIP_TRY(c, hipMemcpy(ga.data(), B.ga_off, ga.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(gb.data(), B.gb_off, gb.size() * 4, hipMemcpyDeviceToHost));
@@ -627,7 +751,8 @@ int build_tiled(hci_ctx* c, Scratch& tmp, const BuildView<S>& B, const S* ub, in
         for (int32_t r0 = ga[g]; r0 < ga[g + 1]; r0 += kTileRows) {
             const int32_t nr = std::min(kTileRows, ga[g + 1] - r0);
             TileTask t{r0, nr, (int32_t)sbs.size(), 0};
-            for (int32_t cg : cols) add_cols(r0, nr, g, cg, cg == g ? 0 : 1, ga[cg], ga[cg + 1]);
+            for (int32_t cg : cols)
+                add_cols(r0, nr, g, cg, cg == g ? 0 : (ga[cg + 1] - ga[cg] > kKeyedCols ? 3 : 1), ga[cg], ga[cg + 1]);
             t.sb_end = (int32_t)sbs.size();
             ta.push_back(t);
         }
@@ -867,6 +992,9 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     B.sb_pos = sb_pos;
     B.nbr_off = nbr_off;
     B.nbr = nbr;
+    B.bkey = nullptr;
+    B.bkq = nullptr;
+    B.n_beta = 0;
     const DevInts T = c->ints();
 
     if (o.storage == HCI_STORE_TILED) {

@@ -19,6 +19,8 @@ struct MlpState;  // learned preconditioner (mlp.hip)
 //   kind 1: columns in a single-connected alpha group (alpha singles with equal
 //           beta, opposite-spin doubles)
 //   kind 2: rows and columns in the same beta group, beta-major (alpha doubles)
+//   kind 3: as kind 1, but the build finds the partners through sorted
+//           "beta minus one electron" keys instead of scanning the group
 // stage = 1 when the x slice is worth staging in LDS (entries >= columns).
 constexpr int kTileRows = 256;
 constexpr int kTileCols = 4096;
