@@ -5,12 +5,17 @@ Workload (one "step" = one pass of the hot path over one determinant list):
   recipe (h_pp = -1 + 2p/15, h_pq ~ N(0, 0.05^2), (pq|rs) = sum_K L^K_pq L^K_rs,
   K = 80 Cholesky vectors, L ~ N(0, 0.08^2), seed 1), over a 2e6-determinant
   space from the deterministic energy-truncated generator (risq_amd.dets);
-  the step is the sparse H build from HBM-resident determinants followed by
+  a step is the sparse H build from HBM-resident determinants followed by
   Davidson to E0 (lowest root, residual 1e-4 and energy change 1e-8).
 
-Reported: seconds to E0 per step (value), H-build nonzeros/s, SpMV GB/s with
-its HBM-roofline fraction (algorithmic bytes / event-timed kernel time), and
-the CPU oracle timed on a bounded sample on this host.
+Timed path (value): tiled storage with FP32 hi+lo values, Davidson with FP32
+sigmas and FP64 refinement, restarts that keep the previous Ritz vector, and the
+correction chosen by --precond.  Reported beside it from the same run: the
+plain FP64 GPU path (FP64 values, the reference's restart-to-one-vector control
+flow, diagonal correction) and its E0, the other correction, the H-build
+nonzeros/s, the sigma products' HBM-roofline fraction (algorithmic bytes / HIP
+event time of the products inside the timed steps), and the CPU oracle timed
+on a bounded sample on this host.
 """
 from __future__ import annotations
 
@@ -46,7 +51,7 @@ def spmv_bytes(nnz_u: int, n: int, value_bytes: int) -> int:
     return nnz_u * (value_bytes + 4) + 8 * (n + 1) + 8 * n + 8 * n + 8 * n
 
 
-def cpu_baseline(ints, dets, gpu_e0_full: float, budget_s: float = 20.0) -> dict:
+def cpu_baseline(ints, dets) -> dict:
     """The oracle (C++ restatement of the reference path, one core) on a bounded
     prefix of the same determinant list: H build + SpMV + Davidson."""
     from oracle import pyoracle
@@ -86,7 +91,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-dets", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--mixed", action="store_true", help="FP32-value sigmas with FP64 refinement")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the untimed comparison runs (FP64 reference path, other preconditioner)")
+    ap.add_argument("--precond", choices=["mlp", "dpr"], default="dpr",
+                    help="correction of the timed path (the other one is reported beside it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has one MI355X; "
@@ -94,7 +102,9 @@ def main() -> int:
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
-    from risq_amd.hci import PREC_F32, PREC_F64, DeviceBuffer, HciContext
+    from risq_amd import precond_train
+    from risq_amd.hci import (PREC_F32, PREC_F64, PRECOND_DPR, PRECOND_MLP, STORE_TILED, DeviceBuffer,
+                              HciContext)
 
     t0 = time.perf_counter()
     ints, dets = make_inputs(args.n_dets)
@@ -102,6 +112,8 @@ def main() -> int:
     log(f"inputs ready: n={n} ({time.perf_counter() - t0:.1f}s)")
     ctx = HciContext(0)
     ctx.set_integrals(ints)
+    precond_train.load(ctx)
+    precond = PRECOND_MLP if args.precond == "mlp" else PRECOND_DPR
     diag = ctx.ham_diag(dets)
     d_dets = DeviceBuffer.from_array(ctx, dets)
     d_diag = DeviceBuffer.from_array(ctx, diag)
@@ -111,20 +123,22 @@ def main() -> int:
     ctx._check(ctx.lib.hci_memcpy(ctx.handle, cal_b.ptr, cal_a.ptr, 256 << 20, 1, 1))
     cal_a.free()
     cal_b.free()
-    prec = PREC_F32 if args.mixed else PREC_F64
 
-    def step():
+    def run(prec, mixed, keep, pc):
         ts = time.perf_counter()
-        h = ctx.build_h(d_dets, d_diag, n=n, value_prec=prec)
+        h = ctx.build_h(d_dets, d_diag, n=n, value_prec=prec, storage=STORE_TILED)
         tb = time.perf_counter() - ts
-        res = ctx.davidson(h, tol=1e-4, etol=1e-8, max_iters=400, mixed=args.mixed)
+        res = ctx.davidson(h, tol=1e-4, etol=1e-8, max_iters=400, mixed=mixed, restart_keep=keep, precond=pc)
         ctx.sync()
         return h, res, tb, time.perf_counter() - ts
+
+    def step():
+        return run(PREC_F32, True, 2, precond)
 
     for w in range(args.warmup):
         h, res, tb, tt = step()
         log(f"warmup {w}: build {tb:.3f}s total {tt:.3f}s E0={res.energy:.10f} iters={res.stats['iters']}")
-        del h
+        h.free()
     ctx.sync()
     times, builds, results = [], [], []
     t_start = time.perf_counter()
@@ -133,18 +147,18 @@ def main() -> int:
         times.append(tt)
         builds.append(tb)
         results.append(res)
-        log(f"step {k}: build {tb:.3f}s total {tt:.3f}s E0={res.energy:.10f} iters={res.stats['iters']} "
-            f"sigma {res.stats['sigma_ms'] / max(1, res.stats['n_sigma_full'] + res.stats['n_sigma_f32']):.3f} ms")
+        log(f"step {k}: build {tb:.3f}s total {tt:.3f}s E0={res.energy:.10f} {res.stats}")
         if k + 1 < args.steps:
-            del h
+            h.free()
     ctx.sync()
     wall = time.perf_counter() - t_start
     nnz = h.nnz
+    h.free()
     last = results[-1].stats
-    n_sig = last["n_sigma_full"] + last["n_sigma_f32"]
-    sigma_ms = last["sigma_ms"] / max(1, n_sig)
-    vb = 4 if args.mixed else 8
-    achieved = spmv_bytes(nnz, n, vb) / (sigma_ms * 1e-3) / 1e9
+    n32, nfull = last["n_sigma_f32"], last["n_sigma_full"]
+    # FP32 sigmas read col + hi (8 B/entry), refinement sigmas col + hi + lo (12 B)
+    alg_bytes = n32 * spmv_bytes(nnz, n, 4) + nfull * spmv_bytes(nnz, n, 8)
+    achieved = alg_bytes / (last["sigma_ms"] * 1e-3) / 1e9
     out = {
         "metric": "s to E0 within 1µHa (plain FP64 vs FP32+learned precond); H-build nnz/s; SpMV GB/s",
         "value": round(float(np.mean(times)), 4),
@@ -156,26 +170,51 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if args.mixed else "f64",
+        "dtype": "f32 values + f64 accumulation, f64 refinement",
         "data": "synthetic (seeded integrals; deterministic energy-truncated determinant space)",
         "config": {"workload": "config-1 recipe: n_orb=40, 6a6b, K=80, seed 1, "
-                               f"{n} dets, Davidson to E0 (tol 1e-4, etol 1e-8)",
-                   "n_dets": n, "nnz_u": nnz, "value_storage": "fp32 hi+lo" if args.mixed else "fp64",
+                               f"{n} dets, H build + Davidson to E0 (tol 1e-4, etol 1e-8)",
+                   "n_dets": n, "nnz_u": nnz, "storage": "tiled, fp32 hi+lo",
+                   "davidson": f"mixed precision, restart keeps 2 vectors, correction={args.precond}",
                    "parallelism": "single GPU"},
         "E0": results[-1].energy,
-        "davidson": {"iters": last["iters"], "n_sigma": n_sig, "seconds": last["seconds"],
-                     "converged": bool(last["converged"])},
+        "davidson": {"iters": last["iters"], "n_sigma_f32": n32, "n_sigma_full": nfull,
+                     "seconds": last["seconds"], "converged": bool(last["converged"]),
+                     "mlp_applied": last["mlp_applied"], "mlp_fallbacks": last["mlp_fallbacks"]},
         "h_build_nnz_per_s": nnz / float(np.mean(builds)),
         "h_build_s": float(np.mean(builds)),
-        "spmv_ms": sigma_ms,
+        "spmv_ms": last["sigma_ms"] / max(1, n32 + nfull),
         "spmv_gbps": achieved,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmv_items (full-storage sigma product)"},
+                     "kernel": "k_spmv_tiles (sigma product: FP32-value and FP32 hi+lo launches)"},
     }
+    if not args.no_compare:
+        h, ref, tb, tr = run(PREC_F64, False, 1, PRECOND_DPR)
+        st = ref.stats
+        r_ms = st["sigma_ms"] / max(1, st["n_sigma_full"])
+        r_ach = spmv_bytes(h.nnz, n, 8) / (r_ms * 1e-3) / 1e9
+        h.free()
+        out["fp64_reference_path"] = {
+            "s_to_E0": round(tr, 4), "build_s": round(tb, 4), "E0": ref.energy, "iters": st["iters"],
+            "converged": bool(st["converged"]), "sigma_ms": r_ms, "sigma_gbps": r_ach,
+            "sigma_roofline_frac": round(r_ach / HBM_PEAK_GBS, 4),
+            "what": "FP64 values, restart to the Ritz vector (reference control flow), diagonal correction"}
+        out["speedup_vs_fp64_reference"] = round(tr / out["value"], 3)
+        out["abs_dE_vs_fp64_reference"] = abs(ref.energy - out["E0"])
+        log(f"fp64 reference path: {tr:.3f}s E0={ref.energy:.10f} iters={st['iters']} sigma {r_ms:.2f} ms")
+        other = PRECOND_DPR if precond == PRECOND_MLP else PRECOND_MLP
+        h, alt, tb, ta = run(PREC_F32, True, 2, other)
+        h.free()
+        out["other_correction"] = {
+            "correction": "dpr" if other == PRECOND_DPR else "mlp", "s_to_E0": round(ta, 4),
+            "E0": alt.energy, "iters": alt.stats["iters"], "n_sigma_f32": alt.stats["n_sigma_f32"],
+            "n_sigma_full": alt.stats["n_sigma_full"], "mlp_applied": alt.stats["mlp_applied"],
+            "mlp_fallbacks": alt.stats["mlp_fallbacks"]}
+        log(f"other correction: {ta:.3f}s E0={alt.energy:.10f} {alt.stats}")
     if rank == 0 and not args.no_cpu:
         log("cpu baseline (oracle, 1 core, bounded sample)...")
-        out["cpu_baseline"] = cpu_baseline(ints, dets, results[-1].energy)
+        out["cpu_baseline"] = cpu_baseline(ints, dets)
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0

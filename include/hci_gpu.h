@@ -103,17 +103,25 @@ typedef struct hci_build_stats {
     double seconds;         /* wall time of the call (device work included) */
 } hci_build_stats;
 
+#define HCI_DAV_MAX_SUB 32
+#define HCI_DAV_MAX_KEEP 8
+
 typedef struct hci_dav_opts {
     double tol;        /* residual norm (reference coeff_eps = 1e-4, src/var/mod.rs:89) */
     double etol;       /* eigenvalue change (reference energy_eps = 1e-8, src/var/mod.rs:90) */
     int max_iters;     /* 50 (davidson.rs:75) */
-    int max_sub;       /* subspace limit, reference nvalues + 10 = 11 (davidson.rs:64-68) */
+    int max_sub;       /* subspace limit, reference nvalues + 10 = 11 (davidson.rs:64-68);
+                          at most HCI_DAV_MAX_SUB */
     int precond;       /* HCI_PRECOND_* */
     int mixed;         /* 0: full-precision sigmas; 1: FP32 sigmas, then FP64 refinement */
-    double refine_tol; /* mixed: residual below which refinement starts (default 1e-4) */
+    double refine_tol; /* mixed: residual below which refinement starts; 0 (default):
+                          when the FP32 iteration has converged or stalls */
     int mem;           /* where init / coeffs live */
-    int restart_keep;  /* vectors kept at a restart: 1 = the Ritz vector (reference,
-                          davidson.rs:258-269); 2 = also the previous Ritz vector */
+    int restart_keep;  /* vectors kept at a restart: 1 = the Ritz vector, whose product
+                          is recomputed (reference, davidson.rs:258-269); m >= 2 = the
+                          m - 1 lowest Ritz vectors plus the previous iteration's Ritz
+                          vector, formed inside the subspace with no new product;
+                          m < max_sub, m <= HCI_DAV_MAX_KEEP */
 } hci_dav_opts;
 
 typedef struct hci_dav_stats {

@@ -782,25 +782,27 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (e == hipSuccess && n_sb > 0) e = hipMemcpy(hc.data(), cnt, (size_t)n_sb * 8, hipMemcpyDeviceToHost);
     hipFree(cnt);
     HIP_TRY(c, e);
-    int64_t nnz = 0;
+    int64_t nnz = 0, end = 0;
     for (int64_t k = 0; k < n_sb; ++k) {
-        sbs[k].off = nnz;
+        sbs[k].off = end;
         sbs[k].cnt = hc[k];
         sbs[k].stage = hc[k] >= sbs[k].nc ? 1 : 0;
         nnz += hc[k];
+        end += (hc[k] + kTileAlign - 1) / kTileAlign * kTileAlign;
     }
     HIP_TRY(c, hipMemcpy(H->d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice));
     H->nnz_s = nnz;
     H->nnz_u = nnz / 2;
+    H->nnz_alloc = end;
     H->n_sb = n_sb;
-    HIP_TRY(c, hipMalloc(&H->d_col, (size_t)nnz * 4 + 16));
+    HIP_TRY(c, hipMalloc(&H->d_col, (size_t)end * 4 + 16));
     FillOut fo{H->d_col, nullptr, nullptr, nullptr};
     if (o.value_prec == HCI_PREC_F64) {
-        HIP_TRY(c, hipMalloc(&H->d_val, (size_t)nnz * 8 + 16));
+        HIP_TRY(c, hipMalloc(&H->d_val, (size_t)end * 8 + 16));
         fo.val = H->d_val;
     } else {
-        HIP_TRY(c, hipMalloc(&H->d_hi, (size_t)nnz * 4 + 16));
-        HIP_TRY(c, hipMalloc(&H->d_lo, (size_t)nnz * 4 + 16));
+        HIP_TRY(c, hipMalloc(&H->d_hi, (size_t)end * 4 + 16));
+        HIP_TRY(c, hipMalloc(&H->d_lo, (size_t)end * 4 + 16));
         fo.hi = H->d_hi;
         fo.lo = H->d_lo;
     }
@@ -1052,6 +1054,7 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     BTH(hipStreamSynchronize(s));
     db.release(row_cnt);
     H->nnz_s = nnz;
+    H->nnz_alloc = nnz;
     H->nnz_u = full ? nnz / 2 : nnz;
 
     // fill: straight into the final arrays, except an incremental UPPER build,

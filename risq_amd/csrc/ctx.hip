@@ -280,19 +280,20 @@ int fetch_internal(const hci_csr* h, std::vector<int64_t>& rp, std::vector<int32
                    std::vector<double>& val, std::vector<int32_t>& perm) {
     hci_ctx* c = h->ctx;
     rp.resize((size_t)h->n + 1);
-    col.resize((size_t)h->nnz_s);
-    val.resize((size_t)h->nnz_s);
+    col.resize((size_t)h->nnz_alloc);
+    val.resize((size_t)h->nnz_alloc);
     perm.resize((size_t)h->n);
     if (h->d_row_ptr) HIP_TRY(c, hipMemcpy(rp.data(), h->d_row_ptr, rp.size() * 8, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(perm.data(), h->d_perm, perm.size() * 4, hipMemcpyDeviceToHost));
-    if (h->nnz_s == 0) return HCI_OK;
+    if (h->nnz_alloc == 0) return HCI_OK;
     HIP_TRY(c, hipMemcpy(col.data(), h->d_col, col.size() * 4, hipMemcpyDeviceToHost));
     if (h->prec == HCI_PREC_F64) {
         HIP_TRY(c, hipMemcpy(val.data(), h->d_val, val.size() * 8, hipMemcpyDeviceToHost));
     } else {
         double* tmp = nullptr;
         HIP_TRY(c, hipMalloc(&tmp, val.size() * 8));
-        hipLaunchKernelGGL(k_widen, grid_for(h->nnz_s, 256), 256, 0, c->stream, h->d_hi, h->d_lo, tmp, h->nnz_s);
+        hipLaunchKernelGGL(k_widen, grid_for(h->nnz_alloc, 256), 256, 0, c->stream, h->d_hi, h->d_lo, tmp,
+                           h->nnz_alloc);
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(val.data(), tmp, val.size() * 8, hipMemcpyDeviceToHost);
         hipFree(tmp);

@@ -15,10 +15,17 @@
 // done as classical Gram-Schmidt applied twice (two HBM passes instead of k),
 // which gives the same orthogonality to FP64 rounding.  The <= 11 x 11
 // projected eigenproblem is solved on the host with cyclic Jacobi.
+// Thick restart (restart_keep = m >= 2): the new basis is V Q, where the
+// columns of Q (in the k-dimensional coefficient space) are the m - 1 lowest
+// Ritz vectors and the previous iteration's Ritz coefficients orthonormalised
+// against them.  W Q and Q^T P Q follow without any new product, because the
+// previous Ritz vector lies in span(V) (the basis only grew since it was
+// formed).
 // Mixed precision: sigmas read FP32 values (FP64 vectors and accumulation)
-// until the residual drops below refine_tol or stalls, then the solver
-// restarts from the Ritz vector with full-precision sigmas, so the converged
-// eigenvalue is that of the full-precision H.
+// until the FP32 iteration converges (or the residual drops below refine_tol,
+// or stalls), then the solver restarts from the Ritz vector with
+// full-precision sigmas, so the converged eigenvalue is that of the
+// full-precision H.
 #include <chrono>
 #include <cmath>
 
@@ -27,7 +34,8 @@
 
 namespace {
 
-constexpr int kMaxSub = 16;
+constexpr int kMaxSub = HCI_DAV_MAX_SUB;
+constexpr int kMaxKeep = HCI_DAV_MAX_KEEP;
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 
@@ -151,6 +159,32 @@ __global__ void __launch_bounds__(kRedThreads)
     block_store<kMaxSub + 1>(acc, k + 1, partial);
 }
 
+// In place: V[:, j] = sum_c V[:, c] q[c * kMaxKeep + j] for j < m (same for W).
+// Each thread reads its row of all k columns before writing m of them.
+__global__ void __launch_bounds__(kRedThreads)
+    k_rotate(double* V, double* W, int64_t ld, int k, int m, const double* __restrict__ q, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double vo[kMaxKeep], wo[kMaxKeep];
+#pragma unroll
+        for (int j = 0; j < kMaxKeep; ++j) vo[j] = wo[j] = 0.0;
+        for (int c = 0; c < k; ++c) {
+            const double v = V[(size_t)c * ld + i], w = W[(size_t)c * ld + i];
+#pragma unroll
+            for (int j = 0; j < kMaxKeep; ++j) {
+                const double qc = q[c * kMaxKeep + j];
+                vo[j] += v * qc;
+                wo[j] += w * qc;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kMaxKeep; ++j)
+            if (j < m) {
+                V[(size_t)j * ld + i] = vo[j];
+                W[(size_t)j * ld + i] = wo[j];
+            }
+    }
+}
+
 __global__ void k_scale(const double* t, double s, double* out, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = t[i] * s;
@@ -217,7 +251,8 @@ struct DavWork {
     double* W = nullptr;
     double* x = nullptr;
     double* wx = nullptr;
-    double* xp = nullptr;  // previous Ritz vector
+    double* q = nullptr;    // restart coefficients, kMaxSub x kMaxKeep
+    double* h_q = nullptr;  // pinned staging for q
     double* r = nullptr;
     double* t = nullptr;
     double* partial = nullptr;
@@ -228,12 +263,13 @@ struct DavWork {
         hipFree(W);
         hipFree(x);
         hipFree(wx);
-        hipFree(xp);
+        hipFree(q);
         hipFree(r);
         hipFree(t);
         hipFree(partial);
         hipFree(red);
         if (h_red) hipHostFree(h_red);
+        if (h_q) hipHostFree(h_q);
     }
     unsigned red_grid() const {
         int64_t g = (n + kRedThreads - 1) / kRedThreads;
@@ -315,7 +351,7 @@ extern "C" void hci_dav_default_opts(hci_dav_opts* o) {
     o->max_sub = 11;
     o->precond = HCI_PRECOND_DPR;
     o->mixed = 0;
-    o->refine_tol = 1e-4;
+    o->refine_tol = 0.0;
     o->mem = HCI_MEM_HOST;
     o->restart_keep = 1;
 }
@@ -327,7 +363,10 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
     const int64_t N = h->n;
     if (N < 1) return hci_fail(c, HCI_ERR_ARG, "hci_davidson: empty matrix");
     if (opts->max_sub < 2 || opts->max_sub > kMaxSub)
-        return hci_fail(c, HCI_ERR_ARG, "hci_davidson: max_sub must be in [2, 16]");
+        return hci_fail(c, HCI_ERR_ARG, "hci_davidson: max_sub must be in [2, 32]");
+    if (opts->restart_keep < 1 || opts->restart_keep > kMaxKeep ||
+        (opts->restart_keep > 1 && opts->restart_keep >= opts->max_sub))
+        return hci_fail(c, HCI_ERR_ARG, "hci_davidson: restart_keep must be in [1, min(8, max_sub - 1)]");
     if (opts->precond == HCI_PRECOND_MLP && !mlp_ready(c))
         return hci_fail(c, HCI_ERR_STATE, "hci_davidson: MLP preconditioner requested but no network loaded");
     const bool use_f32 = opts->mixed && h->prec == HCI_PREC_F32;
@@ -346,7 +385,8 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
     HIP_TRY(c, hipMalloc(&w.W, vb * (size_t)std::max(max_sub, 2) + 16));
     HIP_TRY(c, hipMalloc(&w.x, vb + 16));
     HIP_TRY(c, hipMalloc(&w.wx, vb + 16));
-    HIP_TRY(c, hipMalloc(&w.xp, vb + 16));
+    HIP_TRY(c, hipMalloc(&w.q, (size_t)kMaxSub * kMaxKeep * 8));
+    HIP_TRY(c, hipHostMalloc(&w.h_q, (size_t)kMaxSub * kMaxKeep * 8));
     HIP_TRY(c, hipMalloc(&w.r, vb + 16));
     HIP_TRY(c, hipMalloc(&w.t, vb + 16));
     HIP_TRY(c, hipMalloc(&w.partial, (size_t)kRedBlocks * (kMaxSub + 1) * 8));
@@ -421,7 +461,7 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
     double err_prev = 1e300, best_err = 1e300;
     int since_best = 0;
     MlpState* net = (opts->precond == HCI_PRECOND_MLP) ? c->mlp : nullptr;
-    bool have_prev = false;
+    std::vector<double> y_prev;  // previous Ritz coefficients in the current basis
     if (net) DT(csr_rownorm(const_cast<hci_csr*>(h)));
     int it;
     for (it = 0; it < opts->max_iters; ++it) {
@@ -445,14 +485,17 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         if (net && mlp_prev && !(err < err_prev)) ++st.mlp_fallbacks;
         const bool mlp_now = net && st.mlp_fallbacks < 2;
         last_eig = lam;
+        // stall counter: iterations without a 30% residual improvement, counted
+        // only near convergence (an early plateau while the ground state is
+        // still being found is not a precision floor)
         if (err < best_err * 0.7) {
             best_err = err;
             since_best = 0;
-        } else {
+        } else if (err < 100.0 * opts->tol) {
             ++since_best;
         }
-        // FP32-value sigmas converge on the rounded matrix; refine once the
-        // residual reaches refine_tol, or if it has stopped improving
+        // FP32-value sigmas converge on the rounded matrix; refine once that
+        // iteration converges, the residual reaches refine_tol, or it stalls
         if (sigma_mode == HCI_SIGMA_F32 && (conv || err < opts->refine_tol || since_best >= 25)) {
             // switch to full-precision sigmas: restart from the Ritz vector
             sigma_mode = HCI_SIGMA_FULL;
@@ -464,6 +507,7 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             best_err = 1e300;
             since_best = 0;
             mlp_prev = false;
+            y_prev.assign(1, 1.0);
             continue;
         }
         err_prev = err;
@@ -511,45 +555,67 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             }
             P.swap(np);
             dim_sub = nd;
-        } else if (opts->restart_keep >= 2 && have_prev) {
-            // thick restart: the Ritz vector (W column from W y, no product) and
-            // the previous Ritz vector orthonormalised against it
-            HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
-            HIP_TRY(c, hipMemcpyAsync(w.W, w.wx, vb, hipMemcpyDeviceToDevice, s));
-            double* v1 = w.col(w.V, 1);
-            HIP_TRY(c, hipMemcpyAsync(v1, w.xp, vb, hipMemcpyDeviceToDevice, s));
-            DT(w.multidot(w.V, 1, v1));
-            Coef hc{};
-            hc.c[0] = w.h_red[0];
-            hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, 1, hc, v1, N, w.partial);
-            DT(w.reduce(2));
-            hc.c[0] = w.h_red[0];
-            hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, 1, hc, v1, N, w.partial);
-            DT(w.reduce(2));
-            const double nrm = std::sqrt(w.h_red[1]);
-            dim_sub = 1;
-            if (nrm > 1e-9) {
-                hipLaunchKernelGGL(k_scale, grid_for(N, 256), 256, 0, s, v1, 1.0 / nrm, v1, N);
-                DT(sigma(v1, w.col(w.W, 1)));
-                dim_sub = 2;
+        } else if (opts->restart_keep >= 2) {
+            // thick restart inside the subspace: Q = [y_0 .. y_{l-1}, y_prev orthonormalised]
+            const int k = dim_sub;
+            const int l = std::min(opts->restart_keep - 1, k - 1);
+            std::vector<double> Q((size_t)k * kMaxKeep, 0.0);  // Q[c * kMaxKeep + j]
+            for (int j = 0; j < l; ++j)
+                for (int c = 0; c < k; ++c) Q[(size_t)c * kMaxKeep + j] = vecs[(size_t)c * k + j];
+            int m = l;
+            if (!y_prev.empty()) {
+                std::vector<double> u(k, 0.0);
+                for (size_t c = 0; c < y_prev.size() && (int)c < k; ++c) u[c] = y_prev[c];
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int j = 0; j < l; ++j) {
+                        double d = 0.0;
+                        for (int c = 0; c < k; ++c) d += Q[(size_t)c * kMaxKeep + j] * u[c];
+                        for (int c = 0; c < k; ++c) u[c] -= d * Q[(size_t)c * kMaxKeep + j];
+                    }
+                double nu = 0.0;
+                for (int c = 0; c < k; ++c) nu += u[c] * u[c];
+                nu = std::sqrt(nu);
+                if (nu > 1e-8) {
+                    for (int c = 0; c < k; ++c) Q[(size_t)c * kMaxKeep + m] = u[c] / nu;
+                    ++m;
+                }
             }
-            P.assign((size_t)dim_sub * dim_sub, 0.0);
-            for (int k = 0; k < dim_sub; ++k) {
-                DT(w.multidot(w.V, dim_sub, w.col(w.W, k)));
-                for (int q = 0; q < dim_sub; ++q) P[(size_t)q * dim_sub + k] = w.h_red[q];
-            }
-            for (int q = 0; q < dim_sub; ++q)
-                for (int k = q + 1; k < dim_sub; ++k) P[(size_t)k * dim_sub + q] = P[(size_t)q * dim_sub + k];
+            // projected matrix in the new basis: Q^T P Q
+            std::vector<double> PQ((size_t)k * m, 0.0), np((size_t)m * m, 0.0);
+            for (int r = 0; r < k; ++r)
+                for (int j = 0; j < m; ++j) {
+                    double a = 0.0;
+                    for (int c = 0; c < k; ++c) a += P[(size_t)r * k + c] * Q[(size_t)c * kMaxKeep + j];
+                    PQ[(size_t)r * m + j] = a;
+                }
+            for (int i2 = 0; i2 < m; ++i2)
+                for (int j = 0; j < m; ++j) {
+                    double a = 0.0;
+                    for (int r = 0; r < k; ++r) a += Q[(size_t)r * kMaxKeep + i2] * PQ[(size_t)r * m + j];
+                    np[(size_t)i2 * m + j] = a;
+                }
+            for (int i2 = 0; i2 < m; ++i2)
+                for (int j = i2 + 1; j < m; ++j) np[(size_t)j * m + i2] = np[(size_t)i2 * m + j];
+            std::copy(Q.begin(), Q.end(), w.h_q);
+            HIP_TRY(c, hipMemcpyAsync(w.q, w.h_q, Q.size() * 8, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_rotate, w.red_grid(), kRedThreads, 0, s, w.V, w.W, N, k, m, w.q, N);
+            HIP_TRY(c, hipGetLastError());
+            HIP_TRY(c, hipStreamSynchronize(s));  // h_q is reused by the next restart
+            P.swap(np);
+            dim_sub = m;
+            // the current Ritz vector is the first new column
+            y.c[0] = 1.0;
+            for (int c2 = 1; c2 < kMaxSub; ++c2) y.c[c2] = 0.0;
             mlp_prev = false;
         } else {
             // restart from the Ritz vector (davidson.rs:258-269)
             dim_sub = 1;
             HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
             DT(rebuild_projection());
+            y.c[0] = 1.0;
             mlp_prev = false;
         }
-        HIP_TRY(c, hipMemcpyAsync(w.xp, w.x, vb, hipMemcpyDeviceToDevice, s));
-        have_prev = true;
+        y_prev.assign(y.c, y.c + std::min(dim_sub, kMaxSub));
     }
 #undef DT
     st.iters = it;

@@ -22,8 +22,12 @@ struct MlpState;  // learned preconditioner (mlp.hip)
 //   kind 3: as kind 1, but the build finds the partners through sorted
 //           "beta minus one electron" keys instead of scanning the group
 // stage = 1 when the x slice is worth staging in LDS (entries >= columns).
+// Every sub-block starts at an entry offset that is a multiple of 4, so the
+// product reads its entries as 16-byte vectors.
 constexpr int kTileRows = 256;
 constexpr int kTileCols = 4096;
+constexpr int kTileAlign = 4;
+constexpr int kTileThreads = 512;
 struct SubBlock {
     int32_t r0, nr, c0, nc;
     int32_t g_row, g_col;
@@ -72,6 +76,7 @@ struct hci_csr {
     int64_t n = 0;
     int64_t nnz_u = 0;             // reference count (upper pairs)
     int64_t nnz_s = 0;             // stored entries (2*nnz_u for FULL)
+    int64_t nnz_alloc = 0;         // length of the entry arrays (nnz_s plus TILED alignment gaps)
     int prec = HCI_PREC_F64;
     int storage = HCI_STORE_FULL;
     int32_t* d_perm = nullptr;     // internal -> caller index

@@ -209,8 +209,52 @@ __global__ void __launch_bounds__(256)
 // accumulator (hardware FP64 LDS atomics; entries are column-major, so the
 // lanes of one step mostly hit different rows).  The tile then writes its
 // rows once: y_p = d_p x_p + acc_p (alpha-major part) or acc_p (beta-major part).
+// Entries are read four per lane as 16-byte vectors (sub-blocks start at
+// multiples of 4), two vectors in flight per lane, so one wave keeps 2-3 KB
+// of the matrix stream outstanding; the last cnt mod 4 entries of a
+// sub-block are read one per lane.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <int MODE>
+__device__ __forceinline__ void load_quad(const int32_t* idx, const double* val, const float* hi, const float* lo,
+                                          int64_t e, u32x4& u, double (&v)[4]) {
+    u = __builtin_nontemporal_load((const u32x4*)(idx + e));
+    if (MODE == 0) {
+        const f64x2 a = __builtin_nontemporal_load((const f64x2*)(val + e));
+        const f64x2 b = __builtin_nontemporal_load((const f64x2*)(val + e + 2));
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = b.x;
+        v[3] = b.y;
+    } else {
+        const f32x4 h = __builtin_nontemporal_load((const f32x4*)(hi + e));
+        if (MODE == 1) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = (double)h[q];
+        } else {
+            const f32x4 l = __builtin_nontemporal_load((const f32x4*)(lo + e));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = (double)h[q] + (double)l[q];
+        }
+    }
+}
+
+template <bool ABS>
+__device__ __forceinline__ void add_entry(double* acc, const double* xs, const double* xg, bool staged, uint32_t u,
+                                          double v) {
+    const int lc = (int)(u & 0xFFFFu), lr = (int)(u >> 16);
+    if (ABS) {
+        atomicAdd(&acc[lr], fabs(v));
+    } else {
+        const double xv = staged ? xs[lc] : xg[lc];
+        atomicAdd(&acc[lr], v * xv);
+    }
+}
+
 template <int MODE, bool ABS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTileThreads)
     k_spmv_tiles(const TileTask* __restrict__ tasks, const SubBlock* __restrict__ sbs,
                  const int32_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
                  const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
@@ -219,42 +263,47 @@ __global__ void __launch_bounds__(256)
     __shared__ double xs[kTileCols];
     const TileTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x;
-    for (int r = tid; r < t.nr; r += 256) acc[r] = 0.0;
+    for (int r = tid; r < t.nr; r += kTileThreads) acc[r] = 0.0;
     __syncthreads();
+    constexpr int64_t kStep = (int64_t)kTileAlign * kTileThreads;
     for (int32_t k = t.sb_begin; k < t.sb_end; ++k) {
         const SubBlock b = sbs[k];
         if (b.cnt == 0) continue;
-        if (b.stage) {
-            for (int j = tid; j < b.nc; j += 256) xs[j] = ABS ? 1.0 : x[b.c0 + j];
+        const bool staged = !ABS && b.stage;
+        if (staged) {
+            __syncthreads();  // previous readers of xs are done
+            for (int j = tid; j < b.nc; j += kTileThreads) xs[j] = x[b.c0 + j];
             __syncthreads();
         }
-        const int64_t e1 = b.off + b.cnt;
-        int64_t e = b.off + tid;
-        for (; e + 768 < e1; e += 1024) {
-            uint32_t u[4];
-            double v[4];
+        const double* xg = ABS ? nullptr : x + b.c0;
+        const int64_t qend = b.off + (b.cnt & ~(int64_t)(kTileAlign - 1));
+        int64_t e = b.off + (int64_t)kTileAlign * tid;
+        for (; e + kStep < qend; e += 2 * kStep) {
+            u32x4 u0, u1;
+            double v0[4], v1[4];
+            load_quad<MODE>(idx, val, hi, lo, e, u0, v0);
+            load_quad<MODE>(idx, val, hi, lo, e + kStep, u1, v1);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                u[q] = (uint32_t)__builtin_nontemporal_load(idx + e + 256 * q);
-                v[q] = load_val<MODE>(val, hi, lo, e + 256 * q);
-            }
+            for (int q = 0; q < 4; ++q) add_entry<ABS>(acc, xs, xg, staged, u0[q], v0[q]);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int lc = (int)(u[q] & 0xFFFFu), lr = (int)(u[q] >> 16);
-                const double xv = ABS ? 1.0 : (b.stage ? xs[lc] : x[b.c0 + lc]);
-                atomicAdd(&acc[lr], ABS ? fabs(v[q]) : v[q] * xv);
-            }
+            for (int q = 0; q < 4; ++q) add_entry<ABS>(acc, xs, xg, staged, u1[q], v1[q]);
         }
-        for (; e < e1; e += 256) {
-            const uint32_t uu = (uint32_t)__builtin_nontemporal_load(idx + e);
-            const double vv = load_val<MODE>(val, hi, lo, e);
-            const int lc = (int)(uu & 0xFFFFu), lr = (int)(uu >> 16);
-            const double xv = ABS ? 1.0 : (b.stage ? xs[lc] : x[b.c0 + lc]);
-            atomicAdd(&acc[lr], ABS ? fabs(vv) : vv * xv);
+        if (e < qend) {
+            u32x4 u0;
+            double v0[4];
+            load_quad<MODE>(idx, val, hi, lo, e, u0, v0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) add_entry<ABS>(acc, xs, xg, staged, u0[q], v0[q]);
         }
-        __syncthreads();
+        const int rem = (int)(b.cnt & (kTileAlign - 1));
+        if (tid < rem) {
+            const int64_t et = qend + tid;
+            add_entry<ABS>(acc, xs, xg, staged, (uint32_t)__builtin_nontemporal_load(idx + et),
+                           load_val<MODE>(val, hi, lo, et));
+        }
     }
-    for (int r = tid; r < t.nr; r += 256) {
+    __syncthreads();
+    for (int r = tid; r < t.nr; r += kTileThreads) {
         const int64_t p = (int64_t)t.r0 + r;
         y[p] = (diag && !ABS) ? diag[p] * x[p] + acc[r] : acc[r];
     }
@@ -337,17 +386,17 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
         if (!absval) hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, s, h->d_bpos, x, h->n, h->d_xb);
         if (h->n_task_b > 0) {
             if (absval)
-                hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_b, 256, 0, s, h->d_task_b, h->d_sb,
+                hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b, h->d_sb,
                                    h->d_col, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);
             else
-                hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_b, 256, 0, s, h->d_task_b,
+                hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
                                    h->d_sb, h->d_col, h->d_val, h->d_hi, h->d_lo, h->d_xb, nullptr, h->d_yb);
         }
         if (absval)
-            hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_a, 256, 0, s, h->d_task_a, h->d_sb,
+            hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a, h->d_sb,
                                h->d_col, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, y);
         else
-            hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_a, 256, 0, s, h->d_task_a, h->d_sb,
+            hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a, h->d_sb,
                                h->d_col, h->d_val, h->d_hi, h->d_lo, x, h->d_diag, y);
         if (h->n_task_b > 0)
             hipLaunchKernelGGL(k_scatter_add, grid_for(h->n, 256), 256, 0, s, h->d_bpos, h->d_yb, h->n, y);

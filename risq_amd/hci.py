@@ -334,7 +334,7 @@ class HciContext:
     def davidson(self, h: "SparseMatUpperTri", init: np.ndarray | None = None, tol: float = 1e-4,
                  etol: float = 1e-8, max_iters: int = 50, max_sub: int = 11,
                  precond: int = PRECOND_DPR, mixed: bool = False,
-                 refine_tol: float = 1e-4, restart_keep: int = 1) -> "DavidsonResult":
+                 refine_tol: float = 0.0, restart_keep: int = 1) -> "DavidsonResult":
         opts = DavOpts()
         self.lib.hci_dav_default_opts(C.byref(opts))
         opts.tol, opts.etol, opts.max_iters, opts.max_sub = tol, etol, max_iters, max_sub
