@@ -96,6 +96,27 @@ __device__ __forceinline__ void put(const FillOut& o, int64_t e, int32_t q, doub
     }
 }
 
+// TILED fill target: window w holds wcnt[w] entries from slot woff[w]
+struct TileOut {
+    uint16_t* idx;
+    double* val;
+    float* hi;
+    float* lo;
+    const int64_t* woff;
+    const int64_t* wcnt;
+};
+
+__device__ __forceinline__ void put16(const TileOut& o, int64_t e, uint32_t u, double v) {
+    o.idx[e] = (uint16_t)u;
+    if (o.val) {
+        o.val[e] = v;
+    } else {
+        const float h = (float)v;
+        o.hi[e] = h;
+        o.lo[e] = (float)(v - (double)h);
+    }
+}
+
 // One wavefront per internal row p (caller determinant i = sa_idx[p]).
 // A candidate q (caller j) is kept iff its pair is connected, its element passes
 // the storage rule, max(i, j) >= n_stored, and (UPPER) p < q.
@@ -245,19 +266,60 @@ __global__ void __launch_bounds__(kBlock)
 
 // ---- TILED storage --------------------------------------------------------------
 
-// One wavefront per sub-block; pairs are enumerated column-major (f = col * nr
-// + row), so the emitted entries of a sub-block are ordered by column, then row.
+// One wavefront per sub-block.  Every kept pair goes to the window of its
+// column slice (local column >> 8); the wave keeps one running count per
+// window in LDS.  A step whose kept pairs share one window (kinds 0, 1 and 2
+// enumerate pairs column-major, f = col * nr + row, so nearly all do) places
+// them in lane order; a step that spreads over several windows (kind 3 emits
+// in key order) takes positions with LDS atomics.  The count pass leaves the
+// per-window totals; the fill pass stores each pair at its window's offset +
+// position.  Positions within a window are a permutation of its slots; the
+// product does not depend on their order.
 template <class S, bool FILL>
 __global__ void __launch_bounds__(kBlock)
     k_tile_connect(const SubBlock* __restrict__ sbs, int64_t n_sb, BuildView<S> B, const S* ub, DevInts T,
-                   int single_rule, int64_t* cnt_out, FillOut out) {
+                   int single_rule, int64_t* win_cnt, TileOut out) {
+    __shared__ int32_t s_cur[kWaves][kWinPerSb];
+    __shared__ int64_t s_off[kWaves][kWinPerSb];
+    __shared__ int64_t s_cnt[kWaves][kWinPerSb];
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
     if (w >= n_sb) return;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const SubBlock sb = sbs[w];
-    int64_t cnt = 0;
-    const int64_t base = FILL ? sb.off : 0;
+    const int n_slices = (sb.nc + kWinCols - 1) / kWinCols;
+    int32_t* wcur = s_cur[threadIdx.x >> 6];
+    int64_t* woff = s_off[threadIdx.x >> 6];
+    int64_t* wcnt = s_cnt[threadIdx.x >> 6];
+    if (lane < kWinPerSb) wcur[lane] = 0;
+    if (FILL && lane < n_slices) {
+        woff[lane] = out.woff[sb.win_begin + lane];
+        wcnt[lane] = out.wcnt[sb.win_begin + lane];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    auto emit = [&](bool keep, int32_t rl, int32_t cl, double v) {
+        const uint64_t mk = __ballot(keep);
+        if (!mk) return;
+        const int slice = cl / kWinCols;
+        const int s0 = __shfl(slice, (int)__builtin_ctzll(mk));
+        const uint64_t ms = __ballot(keep && slice == s0);
+        int32_t pos = 0;
+        if (ms == mk) {
+            pos = wcur[s0] + __popcll(ms & lt_mask);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane == (int)__builtin_ctzll(mk)) wcur[s0] += __popcll(ms);
+        } else if (keep) {
+            pos = atomicAdd(&wcur[slice], 1);
+        }
+        if (FILL && keep)
+            put16(out, woff[slice] + win_slot(pos, wcnt[slice]), ((uint32_t)rl << 8) | (uint32_t)(cl % kWinCols), v);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     if (sb.kind == 3) {
         // work item g = key * nr + row: the row's beta string minus its key-th
         // electron; every determinant of the column group whose beta string
@@ -317,13 +379,10 @@ __global__ void __launch_bounds__(kBlock)
                         }
                     }
                 }
-                const uint64_t mk = __ballot(keep);
-                if (FILL && keep)
-                    put(out, base + cnt + __popcll(mk & lt_mask), (int32_t)(((uint32_t)rl << 16) | (uint32_t)cl), v);
-                cnt += __popcll(mk);
+                emit(keep, rl, cl, v);
             }
         }
-        if (!FILL && lane == 0) cnt_out[w] = cnt;
+        if (!FILL && lane < n_slices) win_cnt[sb.win_begin + lane] = wcur[lane];
         return;
     }
     const int32_t P = sb.nr * sb.nc;
@@ -372,11 +431,47 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
         }
-        const uint64_t m = __ballot(keep);
-        if (FILL && keep) put(out, base + cnt + __popcll(m & lt_mask), (int32_t)(((uint32_t)rl << 16) | (uint32_t)cl), v);
-        cnt += __popcll(m);
+        emit(keep, rl, cl, v);
     }
-    if (!FILL && lane == 0) cnt_out[w] = cnt;
+    if (!FILL && lane < n_slices) win_cnt[sb.win_begin + lane] = wcur[lane];
+}
+
+// window k of sub-block b covers columns [c0 + 256 s, ...) of b
+__global__ void k_win_init(const SubBlock* sbs, int64_t n_sb, int32_t* wc0, int32_t* wnc) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_sb) return;
+    const SubBlock sb = sbs[b];
+    for (int32_t s = 0, k = sb.win_begin; s < sb.nc; s += kWinCols, ++k) {
+        wc0[k] = sb.c0 + s;
+        wnc[k] = min(kWinCols, sb.nc - s);
+    }
+}
+
+// slots per window (multiple of kWinAlign) and non-empty flags; entry n_win is 0
+__global__ void k_win_sizes(const int64_t* wc, int64_t n_win, int64_t* sz, int32_t* flag) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n_win) return;
+    const int64_t c = k < n_win ? wc[k] : 0;
+    sz[k] = (c + kWinAlign - 1) / kWinAlign * kWinAlign;
+    flag[k] = c > 0 ? 1 : 0;
+}
+
+__global__ void k_win_compact(const int64_t* wc, const int64_t* woff, const int32_t* nidx, const int32_t* wc0,
+                              const int32_t* wnc, int64_t n_win, Window* out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_win || wc[k] == 0) return;
+    Window w;
+    w.off = woff[k];
+    w.c0 = wc0[k];
+    w.cnt_nc = (uint32_t)wc[k] | ((uint32_t)wnc[k] << 20);
+    out[nidx[k]] = w;
+}
+
+__global__ void k_task_remap(TileTask* t, int64_t n, const int32_t* nidx) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    t[k].w_begin = nidx[t[k].w_begin];
+    t[k].w_end = nidx[t[k].w_end];
 }
 
 // ---- sorting / bucketing helpers ------------------------------------------------
@@ -730,6 +825,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (n_pairs > 0) HIP_TRY(c, hipMemcpy(nb.data(), B.nbr, (size_t)n_pairs * 4, hipMemcpyDeviceToHost));
     std::vector<SubBlock> sbs;
     std::vector<TileTask> ta, tb;
+    int64_t n_wall = 0;
     sbs.reserve((size_t)n_pairs + (size_t)Ua + (size_t)Ub + 16);
     auto add_cols = [&](int32_t r0, int32_t nr, int32_t g_row, int32_t g_col, int kind, int32_t c0, int32_t c1) {
         for (int32_t cs = c0; cs < c1; cs += kTileCols) {
@@ -741,6 +837,8 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
             b.g_row = g_row;
             b.g_col = g_col;
             b.kind = kind;
+            b.win_begin = (int32_t)n_wall;
+            n_wall += (b.nc + kWinCols - 1) / kWinCols;
             sbs.push_back(b);
         }
     };
@@ -750,77 +848,98 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         cols.insert(std::lower_bound(cols.begin(), cols.end(), g), g);
         for (int32_t r0 = ga[g]; r0 < ga[g + 1]; r0 += kTileRows) {
             const int32_t nr = std::min(kTileRows, ga[g + 1] - r0);
-            TileTask t{r0, nr, (int32_t)sbs.size(), 0};
+            TileTask t{r0, nr, (int32_t)n_wall, 0};
             for (int32_t cg : cols)
                 add_cols(r0, nr, g, cg, cg == g ? 0 : (ga[cg + 1] - ga[cg] > kKeyedCols ? 3 : 1), ga[cg], ga[cg + 1]);
-            t.sb_end = (int32_t)sbs.size();
+            t.w_end = (int32_t)n_wall;
             ta.push_back(t);
         }
     }
     for (int32_t g = 0; g < Ub; ++g) {
         for (int32_t r0 = gb[g]; r0 < gb[g + 1]; r0 += kTileRows) {
             const int32_t nr = std::min(kTileRows, gb[g + 1] - r0);
-            TileTask t{r0, nr, (int32_t)sbs.size(), 0};
+            TileTask t{r0, nr, (int32_t)n_wall, 0};
             if (gb[g + 1] - gb[g] > 1) add_cols(r0, nr, g, g, 2, gb[g], gb[g + 1]);
-            t.sb_end = (int32_t)sbs.size();
+            t.w_end = (int32_t)n_wall;
             tb.push_back(t);
         }
     }
     const int64_t n_sb = (int64_t)sbs.size();
-    if (n_sb >= (int64_t)INT32_MAX) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many tiles");
-    HIP_TRY(c, hipMalloc(&H->d_sb, (size_t)n_sb * sizeof(SubBlock) + 16));
-    HIP_TRY(c, hipMemcpy(H->d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice));
-    int64_t* cnt = nullptr;
-    HIP_TRY(c, hipMalloc(&cnt, (size_t)n_sb * 8 + 16));
-    const DevInts T = c->ints();
-    FillOut none{nullptr, nullptr, nullptr, nullptr};
-    if (n_sb > 0)
-        hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, H->d_sb, n_sb, B, ub, T,
-                           o.single_rule, cnt, none);
-    std::vector<int64_t> hc((size_t)n_sb);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess && n_sb > 0) e = hipMemcpy(hc.data(), cnt, (size_t)n_sb * 8, hipMemcpyDeviceToHost);
-    hipFree(cnt);
-    HIP_TRY(c, e);
-    int64_t nnz = 0, end = 0;
-    for (int64_t k = 0; k < n_sb; ++k) {
-        sbs[k].off = end;
-        sbs[k].cnt = hc[k];
-        sbs[k].stage = hc[k] >= sbs[k].nc ? 1 : 0;
-        nnz += hc[k];
-        end += (hc[k] + kTileAlign - 1) / kTileAlign * kTileAlign;
-    }
-    HIP_TRY(c, hipMemcpy(H->d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice));
-    H->nnz_s = nnz;
-    H->nnz_u = nnz / 2;
-    H->nnz_alloc = end;
-    H->n_sb = n_sb;
-    HIP_TRY(c, hipMalloc(&H->d_col, (size_t)end * 4 + 16));
-    FillOut fo{H->d_col, nullptr, nullptr, nullptr};
-    if (o.value_prec == HCI_PREC_F64) {
-        HIP_TRY(c, hipMalloc(&H->d_val, (size_t)end * 8 + 16));
-        fo.val = H->d_val;
-    } else {
-        HIP_TRY(c, hipMalloc(&H->d_hi, (size_t)end * 4 + 16));
-        HIP_TRY(c, hipMalloc(&H->d_lo, (size_t)end * 4 + 16));
-        fo.hi = H->d_hi;
-        fo.lo = H->d_lo;
-    }
-    if (n_sb > 0)
-        hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, H->d_sb, n_sb, B, ub, T,
-                           o.single_rule, nullptr, fo);
-    HIP_TRY(c, hipGetLastError());
+    if (n_wall >= (int64_t)INT32_MAX) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many tiles");
+    DevBufs wb;
+    SubBlock* d_sb = nullptr;
+    int64_t *d_wc = nullptr, *d_sz = nullptr, *d_woff = nullptr, *d_sum = nullptr;
+    int32_t *d_flag = nullptr, *d_nidx = nullptr, *d_wc0 = nullptr, *d_wnc = nullptr;
+    HIP_TRY(c, wb.alloc(&d_sb, (size_t)n_sb + 1));
+    HIP_TRY(c, wb.alloc(&d_wc, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_sz, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_woff, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_sum, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_flag, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_nidx, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_wc0, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_wnc, (size_t)n_wall + 1));
+    if (n_sb > 0) HIP_TRY(c, hipMemcpyAsync(d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice, s));
     H->n_task_a = (int64_t)ta.size();
     H->n_task_b = (int64_t)tb.size();
     HIP_TRY(c, hipMalloc(&H->d_task_a, ta.size() * sizeof(TileTask) + 16));
     HIP_TRY(c, hipMalloc(&H->d_task_b, tb.size() * sizeof(TileTask) + 16));
-    HIP_TRY(c, hipMemcpy(H->d_task_a, ta.data(), ta.size() * sizeof(TileTask), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(H->d_task_b, tb.data(), tb.size() * sizeof(TileTask), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpyAsync(H->d_task_a, ta.data(), ta.size() * sizeof(TileTask), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(H->d_task_b, tb.data(), tb.size() * sizeof(TileTask), hipMemcpyHostToDevice, s));
+    const DevInts T = c->ints();
+    TileOut none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (n_sb > 0) {
+        hipLaunchKernelGGL(k_win_init, grid_for(n_sb, 256), 256, 0, s, d_sb, n_sb, d_wc0, d_wnc);
+        hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
+                           o.single_rule, d_wc, none);
+    }
+    // window slots, offsets, the non-empty windows' new indices and the entry total
+    hipLaunchKernelGGL(k_win_sizes, grid_for(n_wall + 1, 256), 256, 0, s, d_wc, n_wall, d_sz, d_flag);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, exclusive_scan(tmp, d_sz, d_woff, (size_t)n_wall + 1, s));
+    HIP_TRY(c, exclusive_scan(tmp, d_flag, d_nidx, (size_t)n_wall + 1, s));
+    HIP_TRY(c, hipMemsetAsync(d_wc + n_wall, 0, 8, s));
+    HIP_TRY(c, exclusive_scan(tmp, d_wc, d_sum, (size_t)n_wall + 1, s));
+    int64_t end = 0, nnz = 0;
+    int32_t n_win = 0;
+    HIP_TRY(c, hipMemcpyAsync(&end, d_woff + n_wall, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&nnz, d_sum + n_wall, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&n_win, d_nidx + n_wall, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    H->nnz_s = nnz;
+    H->nnz_u = nnz / 2;
+    H->nnz_alloc = end;
+    H->n_win = n_win;
+    const size_t cap = (size_t)end + kWinAlign;
+    HIP_TRY(c, hipMalloc(&H->d_idx16, cap * 2));
+    TileOut fo{H->d_idx16, nullptr, nullptr, nullptr, d_woff, d_wc};
+    if (o.value_prec == HCI_PREC_F64) {
+        HIP_TRY(c, hipMalloc(&H->d_val, cap * 8));
+        fo.val = H->d_val;
+    } else {
+        HIP_TRY(c, hipMalloc(&H->d_hi, cap * 4));
+        HIP_TRY(c, hipMalloc(&H->d_lo, cap * 4));
+        fo.hi = H->d_hi;
+        fo.lo = H->d_lo;
+    }
+    HIP_TRY(c, hipMalloc(&H->d_win, (size_t)n_win * sizeof(Window) + 16));
+    if (n_sb > 0)
+        hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
+                           o.single_rule, nullptr, fo);
+    hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
+                       H->d_win);
+    if (!ta.empty())
+        hipLaunchKernelGGL(k_task_remap, grid_for((int64_t)ta.size(), 256), 256, 0, s, H->d_task_a,
+                           (int64_t)ta.size(), d_nidx);
+    if (!tb.empty())
+        hipLaunchKernelGGL(k_task_remap, grid_for((int64_t)tb.size(), 256), 256, 0, s, H->d_task_b,
+                           (int64_t)tb.size(), d_nidx);
+    HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMalloc(&H->d_bpos, (size_t)N * 4 + 16));
     HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
     HIP_TRY(c, hipMalloc(&H->d_xb, (size_t)N * 8 + 16));
     HIP_TRY(c, hipMalloc(&H->d_yb, (size_t)N * 8 + 16));
-    (void)tmp;
+    HIP_TRY(c, hipStreamSynchronize(s));  // the passes read the work buffers freed on return
     return HCI_OK;
 }
 

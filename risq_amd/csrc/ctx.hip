@@ -286,7 +286,13 @@ int fetch_internal(const hci_csr* h, std::vector<int64_t>& rp, std::vector<int32
     if (h->d_row_ptr) HIP_TRY(c, hipMemcpy(rp.data(), h->d_row_ptr, rp.size() * 8, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(perm.data(), h->d_perm, perm.size() * 4, hipMemcpyDeviceToHost));
     if (h->nnz_alloc == 0) return HCI_OK;
-    HIP_TRY(c, hipMemcpy(col.data(), h->d_col, col.size() * 4, hipMemcpyDeviceToHost));
+    if (h->d_idx16) {
+        std::vector<uint16_t> u(col.size());
+        HIP_TRY(c, hipMemcpy(u.data(), h->d_idx16, u.size() * 2, hipMemcpyDeviceToHost));
+        std::copy(u.begin(), u.end(), col.begin());
+    } else {
+        HIP_TRY(c, hipMemcpy(col.data(), h->d_col, col.size() * 4, hipMemcpyDeviceToHost));
+    }
     if (h->prec == HCI_PREC_F64) {
         HIP_TRY(c, hipMemcpy(val.data(), h->d_val, val.size() * 8, hipMemcpyDeviceToHost));
     } else {
@@ -315,30 +321,43 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
     std::vector<int64_t> rp;
     std::vector<int32_t> ic, perm, bpos;
     std::vector<double> iv;
-    std::vector<SubBlock> sbs;
+    std::vector<Window> wins;
+    std::vector<TileTask> tasks;
     int rc = fetch_internal(h, rp, ic, iv, perm);
     if (rc) return rc;
     const int64_t n = h->n;
     if (h->storage == HCI_STORE_TILED) {
-        sbs.resize((size_t)h->n_sb);
+        wins.resize((size_t)h->n_win);
+        tasks.resize((size_t)(h->n_task_a + h->n_task_b));
         bpos.resize((size_t)n);
-        if (h->n_sb) HIP_TRY(c, hipMemcpy(sbs.data(), h->d_sb, sbs.size() * sizeof(SubBlock), hipMemcpyDeviceToHost));
+        if (h->n_win) HIP_TRY(c, hipMemcpy(wins.data(), h->d_win, wins.size() * sizeof(Window), hipMemcpyDeviceToHost));
+        if (h->n_task_a)
+            HIP_TRY(c, hipMemcpy(tasks.data(), h->d_task_a, (size_t)h->n_task_a * sizeof(TileTask), hipMemcpyDeviceToHost));
+        if (h->n_task_b)
+            HIP_TRY(c, hipMemcpy(tasks.data() + h->n_task_a, h->d_task_b, (size_t)h->n_task_b * sizeof(TileTask),
+                                 hipMemcpyDeviceToHost));
         HIP_TRY(c, hipMemcpy(bpos.data(), h->d_bpos, (size_t)n * 4, hipMemcpyDeviceToHost));
     }
     std::vector<int64_t> cnt((size_t)n + 1, 0);
     auto each = [&](auto f) {
         if (h->storage == HCI_STORE_TILED) {
-            for (const SubBlock& b : sbs)
-                for (int64_t e = b.off; e < b.off + b.cnt; ++e) {
-                    const uint32_t u = (uint32_t)ic[e];
-                    int32_t p = b.r0 + (int32_t)(u >> 16), q = b.c0 + (int32_t)(u & 0xFFFFu);
-                    if (b.kind == 2) {
-                        p = bpos[p];
-                        q = bpos[q];
+            for (size_t t = 0; t < tasks.size(); ++t) {
+                const bool beta_major = (int64_t)t >= h->n_task_a;
+                for (int32_t k = tasks[t].w_begin; k < tasks[t].w_end; ++k) {
+                    const Window& b = wins[(size_t)k];
+                    for (int64_t j = 0; j < b.cnt(); ++j) {
+                        const int64_t e = b.off + win_slot(j, b.cnt());
+                        const uint32_t u = (uint32_t)ic[(size_t)e];
+                        int32_t p = tasks[t].r0 + (int32_t)(u >> 8), q = b.c0 + (int32_t)(u & 0xFFu);
+                        if (beta_major) {
+                            p = bpos[p];
+                            q = bpos[q];
+                        }
+                        const int32_t i = perm[p], j2 = perm[q];
+                        if (i < j2) f(i, j2, iv[(size_t)e]);
                     }
-                    const int32_t i = perm[p], j = perm[q];
-                    if (i < j) f(i, j, iv[e]);
                 }
+            }
             return;
         }
         for (int64_t p = 0; p < n; ++p)
@@ -439,7 +458,8 @@ void hci_csr_free(hci_csr* h) {
     hipFree(h->d_perm);
     hipFree(h->d_pos);
     hipFree(h->d_words);
-    hipFree(h->d_sb);
+    hipFree(h->d_idx16);
+    hipFree(h->d_win);
     hipFree(h->d_task_a);
     hipFree(h->d_task_b);
     hipFree(h->d_bpos);

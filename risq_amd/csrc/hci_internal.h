@@ -11,32 +11,48 @@
 
 struct MlpState;  // learned preconditioner (mlp.hip)
 
-// TILED storage.  A sub-block holds the entries between nr rows starting at r0
-// and nc columns starting at c0 (both in the part's order: alpha-major for
-// kinds 0/1, beta-major for kind 2), column-major, each entry a packed
-// (local_row << 16 | local_col) index plus its value.
+// TILED storage.  The build enumerates sub-blocks: the pairs between nr <= 256
+// rows starting at r0 and nc <= 4096 columns starting at c0 (both in the
+// part's order: alpha-major for kinds 0/1/3, beta-major for kind 2).
 //   kind 0: rows and columns in the same alpha group (beta singles/doubles)
 //   kind 1: columns in a single-connected alpha group (alpha singles with equal
 //           beta, opposite-spin doubles)
 //   kind 2: rows and columns in the same beta group, beta-major (alpha doubles)
 //   kind 3: as kind 1, but the build finds the partners through sorted
 //           "beta minus one electron" keys instead of scanning the group
-// stage = 1 when the x slice is worth staging in LDS (entries >= columns).
-// Every sub-block starts at an entry offset that is a multiple of 4, so the
-// product reads its entries as 16-byte vectors.
+// Storage is by window: the entries of one sub-block whose columns fall in one
+// 256-wide slice, each a 16-bit (local_row << 8 | local_col) index plus its
+// value.  A window of cnt entries occupies R = ceil(cnt / 8) groups of 8 slots
+// starting at a multiple of 8: entry j sits in group j mod R, slot j div R, so
+// lane l of a wave reads its 8 entries {l, l + R, ..., l + 7R} as one 16-byte
+// index vector, and in each of the 8 steps the lanes of a wave handle 64
+// consecutive entries.
 constexpr int kTileRows = 256;
-constexpr int kTileCols = 4096;
-constexpr int kTileAlign = 4;
-constexpr int kTileThreads = 512;
+constexpr int kTileCols = 4096;  // build sub-block width
+constexpr int kWinCols = 256;
+constexpr int kWinPerSb = kTileCols / kWinCols;
+constexpr int kWinAlign = 8;
+constexpr int kTileWaves = 8;
+constexpr int kTileThreads = 64 * kTileWaves;
 struct SubBlock {
     int32_t r0, nr, c0, nc;
     int32_t g_row, g_col;
-    int32_t kind, stage;
-    int64_t off, cnt;
+    int32_t kind, win_begin;  // first of its ceil(nc / 256) windows
 };
-// one workgroup: rows [r0, r0+nr) and sub-blocks [sb_begin, sb_end)
+struct Window {
+    int64_t off;      // first slot (multiple of kWinAlign)
+    int32_t c0;       // first column, in the part's order
+    uint32_t cnt_nc;  // entries | columns << 20
+    __host__ __device__ int cnt() const { return (int)(cnt_nc & 0xFFFFFu); }
+    __host__ __device__ int nc() const { return (int)(cnt_nc >> 20); }
+};
+__host__ __device__ inline int64_t win_slot(int64_t j, int64_t cnt) {
+    const int64_t R = (cnt + kWinAlign - 1) / kWinAlign;
+    return (j % R) * kWinAlign + j / R;
+}
+// one workgroup: rows [r0, r0+nr) and windows [w_begin, w_end)
 struct TileTask {
-    int32_t r0, nr, sb_begin, sb_end;
+    int32_t r0, nr, w_begin, w_end;
 };
 
 struct hci_ctx {
@@ -98,10 +114,11 @@ struct hci_csr {
     int64_t n_items = 0;
     // per-row absolute sums (learned-preconditioner feature), lazily built
     double* d_rownorm = nullptr;
-    // TILED storage (d_col holds the packed local indices)
-    SubBlock* d_sb = nullptr;
-    int64_t n_sb = 0;
-    TileTask* d_task_a = nullptr;  // alpha-major tiles (kinds 0, 1)
+    // TILED storage
+    uint16_t* d_idx16 = nullptr;   // nnz_alloc packed (local_row << 8 | local_col)
+    Window* d_win = nullptr;       // non-empty windows, tile by tile
+    int64_t n_win = 0;
+    TileTask* d_task_a = nullptr;  // alpha-major tiles (kinds 0, 1, 3)
     int64_t n_task_a = 0;
     TileTask* d_task_b = nullptr;  // beta-major tiles (kind 2)
     int64_t n_task_b = 0;

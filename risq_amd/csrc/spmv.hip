@@ -12,7 +12,7 @@
 //     reproducible.
 //   UPPER storage: the transpose term H_pq x_p goes to y_q with a hardware FP64
 //     atomic (order-dependent rounding).
-//   TILED storage: k_spmv_tiles, x slices staged in LDS, rows summed in LDS.
+//   TILED storage: k_spmv_win, x slices staged in LDS, rows summed in LDS.
 // y starts as d o x in both cases.
 #include "hci_internal.h"
 
@@ -204,103 +204,126 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// TILED storage: one workgroup per tile.  For each sub-block the x slice is
-// staged in LDS when dense enough, and every entry adds H_pq x_q to an LDS row
-// accumulator (hardware FP64 LDS atomics; entries are column-major, so the
-// lanes of one step mostly hit different rows).  The tile then writes its
-// rows once: y_p = d_p x_p + acc_p (alpha-major part) or acc_p (beta-major part).
-// Entries are read four per lane as 16-byte vectors (sub-blocks start at
-// multiples of 4), two vectors in flight per lane, so one wave keeps 2-3 KB
-// of the matrix stream outstanding; the last cnt mod 4 entries of a
-// sub-block are read one per lane.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// TILED storage: one workgroup per 256-row tile, one wavefront per window at
+// a time (windows handed out through an LDS counter).  The wave stages its
+// window's x slice (<= 256 values) in its own LDS region, then streams the
+// window's entries 8 per lane (16-byte index and value vectors, two vectors
+// in flight per lane) and adds H_pq x_q to the tile's LDS row accumulator with
+// hardware FP64 LDS atomics.  In each of a lane's 8 steps the wave handles 64
+// consecutive entries of the window (column-major for kinds 0-2), so the x
+// reads mostly broadcast and the row atomics mostly hit distinct rows.  The tile then writes its rows once:
+// y_p = d_p x_p + acc_p (alpha-major part) or acc_p (beta-major part).
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
+// 8 consecutive entries of one lane, values kept in their stored width until use
 template <int MODE>
-__device__ __forceinline__ void load_quad(const int32_t* idx, const double* val, const float* hi, const float* lo,
-                                          int64_t e, u32x4& u, double (&v)[4]) {
-    u = __builtin_nontemporal_load((const u32x4*)(idx + e));
-    if (MODE == 0) {
-        const f64x2 a = __builtin_nontemporal_load((const f64x2*)(val + e));
-        const f64x2 b = __builtin_nontemporal_load((const f64x2*)(val + e + 2));
-        v[0] = a.x;
-        v[1] = a.y;
-        v[2] = b.x;
-        v[3] = b.y;
-    } else {
-        const f32x4 h = __builtin_nontemporal_load((const f32x4*)(hi + e));
-        if (MODE == 1) {
+struct Oct {
+    u16x8 u;
+    f64x2 d[MODE == 0 ? 4 : 1];
+    f32x4 h[MODE == 0 ? 1 : 2];
+    f32x4 l[MODE == 2 ? 2 : 1];
+    __device__ __forceinline__ void load(const uint16_t* idx, const double* val, const float* hi, const float* lo,
+                                         int64_t e) {
+        u = __builtin_nontemporal_load((const u16x8*)(idx + e));
+        if (MODE == 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = (double)h[q];
+            for (int k = 0; k < 4; ++k) d[k] = __builtin_nontemporal_load((const f64x2*)(val + e + 2 * k));
         } else {
-            const f32x4 l = __builtin_nontemporal_load((const f32x4*)(lo + e));
 #pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = (double)h[q] + (double)l[q];
+            for (int k = 0; k < 2; ++k) {
+                h[k] = __builtin_nontemporal_load((const f32x4*)(hi + e + 4 * k));
+                if (MODE == 2) l[k] = __builtin_nontemporal_load((const f32x4*)(lo + e + 4 * k));
+            }
         }
     }
-}
+    __device__ __forceinline__ double value(int q) const {
+        if (MODE == 0) return d[q >> 1][q & 1];
+        if (MODE == 1) return (double)h[q >> 2][q & 3];
+        return (double)h[q >> 2][q & 3] + (double)l[q >> 2][q & 3];
+    }
+};
 
 template <bool ABS>
-__device__ __forceinline__ void add_entry(double* acc, const double* xs, const double* xg, bool staged, uint32_t u,
-                                          double v) {
-    const int lc = (int)(u & 0xFFFFu), lr = (int)(u >> 16);
-    if (ABS) {
-        atomicAdd(&acc[lr], fabs(v));
-    } else {
-        const double xv = staged ? xs[lc] : xg[lc];
-        atomicAdd(&acc[lr], v * xv);
-    }
+__device__ __forceinline__ void add_entry(double* acc, const double* xw, uint32_t u, double v) {
+    if (ABS)
+        atomicAdd(&acc[u >> 8], fabs(v));
+    else
+        atomicAdd(&acc[u >> 8], v * xw[u & 0xFFu]);
 }
 
 template <int MODE, bool ABS>
 __global__ void __launch_bounds__(kTileThreads)
-    k_spmv_tiles(const TileTask* __restrict__ tasks, const SubBlock* __restrict__ sbs,
-                 const int32_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
-                 const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
-                 double* __restrict__ y) {
+    k_spmv_win(const TileTask* __restrict__ tasks, const Window* __restrict__ wins,
+               const uint16_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
+               const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
+               double* __restrict__ y) {
     __shared__ double acc[kTileRows];
-    __shared__ double xs[kTileCols];
+    __shared__ double xs[kTileWaves][kWinCols];
+    __shared__ int next;
     const TileTask t = tasks[blockIdx.x];
-    const int tid = threadIdx.x;
-    for (int r = tid; r < t.nr; r += kTileThreads) acc[r] = 0.0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    double* xw = xs[tid >> 6];
+    for (int r = tid; r < kTileRows; r += kTileThreads) acc[r] = 0.0;
+    if (tid == 0) next = 0;
     __syncthreads();
-    constexpr int64_t kStep = (int64_t)kTileAlign * kTileThreads;
-    for (int32_t k = t.sb_begin; k < t.sb_end; ++k) {
-        const SubBlock b = sbs[k];
-        if (b.cnt == 0) continue;
-        const bool staged = !ABS && b.stage;
-        if (staged) {
-            __syncthreads();  // previous readers of xs are done
-            for (int j = tid; j < b.nc; j += kTileThreads) xs[j] = x[b.c0 + j];
-            __syncthreads();
-        }
-        const double* xg = ABS ? nullptr : x + b.c0;
-        const int64_t qend = b.off + (b.cnt & ~(int64_t)(kTileAlign - 1));
-        int64_t e = b.off + (int64_t)kTileAlign * tid;
-        for (; e + kStep < qend; e += 2 * kStep) {
-            u32x4 u0, u1;
-            double v0[4], v1[4];
-            load_quad<MODE>(idx, val, hi, lo, e, u0, v0);
-            load_quad<MODE>(idx, val, hi, lo, e + kStep, u1, v1);
+    const int n_w = t.w_end - t.w_begin;
+    auto grab = [&]() {
+        int k = 0;
+        if (lane == 0) k = atomicAdd(&next, 1);
+        return __shfl(k, 0);
+    };
+    int k = grab();
+    Window W{};
+    if (k < n_w) W = wins[t.w_begin + k];
+    while (k < n_w) {
+        // the next window's descriptor is requested before this window's data
+        const int kn = grab();
+        Window Wn{};
+        if (kn < n_w) Wn = wins[t.w_begin + kn];
+        // group g of the window (8 slots from off + 8 g) holds entries
+        // g, g + R, ..., g + 7R; lane l takes groups l, l + 64, ...
+        const int cnt = W.cnt(), nc = W.nc();
+        const int R = (cnt + kWinAlign - 1) / kWinAlign;
+        int g = lane;
+        // the first two groups' entries and the x slice are requested together
+        Oct<MODE> o0{}, o1{};
+        if (g < R) o0.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * g);
+        if (g + 64 < R) o1.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * (g + 64));
+        if (!ABS) {
+            double xv[kWinCols / 64];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) add_entry<ABS>(acc, xs, xg, staged, u0[q], v0[q]);
+            for (int j = 0; j < kWinCols / 64; ++j) {
+                const int c = lane + 64 * j;
+                xv[j] = c < nc ? x[W.c0 + c] : 0.0;
+            }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) add_entry<ABS>(acc, xs, xg, staged, u1[q], v1[q]);
+            for (int j = 0; j < kWinCols / 64; ++j) xw[lane + 64 * j] = xv[j];
         }
-        if (e < qend) {
-            u32x4 u0;
-            double v0[4];
-            load_quad<MODE>(idx, val, hi, lo, e, u0, v0);
+        // the slice is written and read by this wave only: order its LDS
+        // writes before the reads below
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        while (g < R) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) add_entry<ABS>(acc, xs, xg, staged, u0[q], v0[q]);
+            for (int q = 0; q < 8; ++q)
+                if (g + q * R < cnt) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+            if (g + 64 < R) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (g + 64 + q * R < cnt) add_entry<ABS>(acc, xw, o1.u[q], o1.value(q));
+            }
+            g += 128;
+            if (g < R) o0.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * g);
+            if (g + 64 < R) o1.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * (g + 64));
         }
-        const int rem = (int)(b.cnt & (kTileAlign - 1));
-        if (tid < rem) {
-            const int64_t et = qend + tid;
-            add_entry<ABS>(acc, xs, xg, staged, (uint32_t)__builtin_nontemporal_load(idx + et),
-                           load_val<MODE>(val, hi, lo, et));
-        }
+        // reads of this slice precede the next window's staging
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        W = Wn;
+        k = kn;
     }
     __syncthreads();
     for (int r = tid; r < t.nr; r += kTileThreads) {
@@ -386,18 +409,18 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
         if (!absval) hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, s, h->d_bpos, x, h->n, h->d_xb);
         if (h->n_task_b > 0) {
             if (absval)
-                hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b, h->d_sb,
-                                   h->d_col, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);
+                hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
+                                   h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);
             else
-                hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_sb, h->d_col, h->d_val, h->d_hi, h->d_lo, h->d_xb, nullptr, h->d_yb);
+                hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
+                                   h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, h->d_xb, nullptr, h->d_yb);
         }
         if (absval)
-            hipLaunchKernelGGL((k_spmv_tiles<MODE, true>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a, h->d_sb,
-                               h->d_col, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, y);
+            hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
+                               h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, y);
         else
-            hipLaunchKernelGGL((k_spmv_tiles<MODE, false>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a, h->d_sb,
-                               h->d_col, h->d_val, h->d_hi, h->d_lo, x, h->d_diag, y);
+            hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
+                               h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, x, h->d_diag, y);
         if (h->n_task_b > 0)
             hipLaunchKernelGGL(k_scatter_add, grid_for(h->n, 256), 256, 0, s, h->d_bpos, h->d_yb, h->n, y);
         return;
