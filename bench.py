@@ -187,7 +187,7 @@ def main() -> int:
         "spmv_gbps": achieved,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmv_tiles (sigma product: FP32-value and FP32 hi+lo launches)"},
+                     "kernel": "k_spmv_win (sigma product: FP32-value and FP32 hi+lo launches)"},
     }
     if not args.no_compare:
         h, ref, tb, tr = run(PREC_F64, False, 1, PRECOND_DPR)

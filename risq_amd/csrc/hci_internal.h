@@ -22,11 +22,12 @@ struct MlpState;  // learned preconditioner (mlp.hip)
 //           "beta minus one electron" keys instead of scanning the group
 // Storage is by window: the entries of one sub-block whose columns fall in one
 // 256-wide slice, each a 16-bit (local_row << 8 | local_col) index plus its
-// value.  A window of cnt entries occupies R = ceil(cnt / 8) groups of 8 slots
-// starting at a multiple of 8: entry j sits in group j mod R, slot j div R, so
-// lane l of a wave reads its 8 entries {l, l + R, ..., l + 7R} as one 16-byte
-// index vector, and in each of the 8 steps the lanes of a wave handle 64
-// consecutive entries.
+// value.  A window of cnt entries starts at a multiple of 8 and occupies
+// ceil(cnt / 8) groups of 8 slots.  Each full chunk of 512 entries is stored
+// lane-major: entry 64 q + l of the chunk sits in group l, slot q, so lane l of
+// a wave reads its 8 entries as one 16-byte index vector and in each of the 8
+// steps the wave handles 64 consecutive entries.  The last m < 512 entries use
+// R = ceil(m / 8) groups: entry j sits in group j mod R, slot j div R.
 constexpr int kTileRows = 256;
 constexpr int kTileCols = 4096;  // build sub-block width
 constexpr int kWinCols = 256;
@@ -46,9 +47,12 @@ struct Window {
     __host__ __device__ int cnt() const { return (int)(cnt_nc & 0xFFFFFu); }
     __host__ __device__ int nc() const { return (int)(cnt_nc >> 20); }
 };
+constexpr int kWinChunk = 64 * kWinAlign;
 __host__ __device__ inline int64_t win_slot(int64_t j, int64_t cnt) {
-    const int64_t R = (cnt + kWinAlign - 1) / kWinAlign;
-    return (j % R) * kWinAlign + j / R;
+    const int64_t full = cnt / kWinChunk * kWinChunk;
+    if (j < full) return (j & ~(int64_t)(kWinChunk - 1)) + ((j & 63) << 3) + ((j >> 6) & 7);
+    const int64_t t = j - full, R = (cnt - full + kWinAlign - 1) / kWinAlign;
+    return full + (t % R) * kWinAlign + t / R;
 }
 // one workgroup: rows [r0, r0+nr) and windows [w_begin, w_end)
 struct TileTask {

@@ -282,15 +282,21 @@ __global__ void __launch_bounds__(kTileThreads)
         const int kn = grab();
         Window Wn{};
         if (kn < n_w) Wn = wins[t.w_begin + kn];
-        // group g of the window (8 slots from off + 8 g) holds entries
-        // g, g + R, ..., g + 7R; lane l takes groups l, l + 64, ...
+        // full chunks: lane l's group of chunk c holds entries 512 c + 64 q + l;
+        // the tail of m entries: group g holds entries g, g + R, ..., g + 7R
         const int cnt = W.cnt(), nc = W.nc();
-        const int R = (cnt + kWinAlign - 1) / kWinAlign;
-        int g = lane;
+        const int n_full = cnt / kWinChunk;
+        const int m = cnt - n_full * kWinChunk;
+        const int R = (m + kWinAlign - 1) / kWinAlign;
+        const int64_t tail = W.off + (int64_t)n_full * kWinChunk;
+        const int64_t mine = W.off + (int64_t)kWinAlign * lane;
         // the first two groups' entries and the x slice are requested together
         Oct<MODE> o0{}, o1{};
-        if (g < R) o0.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * g);
-        if (g + 64 < R) o1.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * (g + 64));
+        if (n_full > 0)
+            o0.load(idx, val, hi, lo, mine);
+        else if (lane < R)
+            o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+        if (n_full > 1) o1.load(idx, val, hi, lo, mine + kWinChunk);
         if (!ABS) {
             double xv[kWinCols / 64];
 #pragma unroll
@@ -306,18 +312,26 @@ __global__ void __launch_bounds__(kTileThreads)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        while (g < R) {
+        int c = 0;
+        for (; c + 1 < n_full; c += 2) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+#pragma unroll
+            for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o1.u[q], o1.value(q));
+            if (c + 2 < n_full) o0.load(idx, val, hi, lo, mine + (int64_t)(c + 2) * kWinChunk);
+            if (c + 3 < n_full) o1.load(idx, val, hi, lo, mine + (int64_t)(c + 3) * kWinChunk);
+        }
+        if (c < n_full) {  // one full chunk left, in o0 (odd count) ...
+#pragma unroll
+            for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+            if (lane < R) o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+        } else if (n_full > 0 && lane < R) {  // ... or none: the tail is still to load
+            o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+        }
+        if (lane < R) {
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                if (g + q * R < cnt) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
-            if (g + 64 < R) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (g + 64 + q * R < cnt) add_entry<ABS>(acc, xw, o1.u[q], o1.value(q));
-            }
-            g += 128;
-            if (g < R) o0.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * g);
-            if (g + 64 < R) o1.load(idx, val, hi, lo, W.off + (int64_t)kWinAlign * (g + 64));
+                if (lane + q * R < m) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
         }
         // reads of this slice precede the next window's staging
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
