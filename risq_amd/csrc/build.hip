@@ -70,6 +70,11 @@ __device__ __forceinline__ bool keep_single(double v, int rule) {
     return rule == HCI_SINGLES_NONZERO ? (v != 0.0) : (fabs(v) > 1e-12);
 }
 
+// TILED builds place pairs by pattern; a single the storage rule rejects is
+// written as an explicit zero, which the product adds harmlessly and the
+// download skips (no stored element is ever exactly zero)
+__device__ __forceinline__ double single_or_zero(double v, int rule) { return keep_single(v, rule) ? v : 0.0; }
+
 template <class S>
 __device__ __forceinline__ void swap_if(bool c, S& a, S& b) {
     const S t = a;
@@ -278,7 +283,7 @@ __global__ void __launch_bounds__(kBlock)
 template <class S, bool FILL>
 __global__ void __launch_bounds__(kBlock)
     k_tile_connect(const SubBlock* __restrict__ sbs, int64_t n_sb, BuildView<S> B, const S* ub, DevInts T,
-                   int single_rule, int64_t* win_cnt, TileOut out) {
+                   int single_rule, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out) {
     __shared__ int32_t s_cur[kWaves][kWinPerSb];
     __shared__ int64_t s_off[kWaves][kWinPerSb];
     __shared__ int64_t s_cnt[kWaves][kWinPerSb];
@@ -299,9 +304,11 @@ __global__ void __launch_bounds__(kBlock)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    unsigned long long nz = 0;  // FILL: nonzero entries written
     auto emit = [&](bool keep, int32_t rl, int32_t cl, double v) {
         const uint64_t mk = __ballot(keep);
         if (!mk) return;
+        if (FILL) nz += __popcll(__ballot(keep && v != 0.0));
         const int slice = cl / kWinCols;
         const int s0 = __shfl(slice, (int)__builtin_ctzll(mk));
         const uint64_t ms = __ballot(keep && slice == s0);
@@ -366,16 +373,13 @@ __global__ void __launch_bounds__(kBlock)
                         const int32_t j = B.sa_idx[q];
                         S da = b, db = B.sa_dn[q], u1 = ua_row, u2 = ua_col;
                         const bool same = (da == db);
-                        if (!same || kk == 0) {  // an equal beta string matches every key: count it once
+                        // an equal beta string matches every key: count it once
+                        keep = same ? kk == 0 : dbl != 0;
+                        if (FILL && keep) {
                             swap_if(j < i, u1, u2);
                             swap_if(j < i, da, db);
-                            if (same) {
-                                v = elem_single<S>(T, u1, u2, da);
-                                keep = keep_single(v, single_rule);
-                            } else {
-                                v = elem_opp_double<S>(T, u1, u2, da, db);
-                                keep = (v != 0.0);
-                            }
+                            v = same ? single_or_zero(elem_single<S>(T, u1, u2, da), single_rule)
+                                     : elem_opp_double<S>(T, u1, u2, da, db);
                         }
                     }
                 }
@@ -383,6 +387,7 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
         if (!FILL && lane < n_slices) win_cnt[sb.win_begin + lane] = wcur[lane];
+        if (FILL && lane == 0) atomicAdd(nnz_out, nz);
         return;
     }
     const int32_t P = sb.nr * sb.nc;
@@ -398,35 +403,31 @@ __global__ void __launch_bounds__(kBlock)
             if (sb.kind == 2) {
                 const int32_t i = B.sb_idx[p], j = B.sb_idx[q];
                 S a = B.sb_up[p], b = B.sb_up[q];
-                if (popc(a ^ b) == 4) {
+                keep = popc(a ^ b) == 4;
+                if (FILL && keep) {
                     swap_if(j < i, a, b);
                     v = elem_same_double<S>(T, a, b);
-                    keep = (v != 0.0);
                 }
             } else {
                 const int32_t i = B.sa_idx[p], j = B.sa_idx[q];
                 S da = B.sa_dn[p], db = B.sa_dn[q];
                 const int c = popc(da ^ db);
                 if (sb.kind == 0) {
-                    const S up = B.ua[sb.g_row];
-                    swap_if(j < i, da, db);
-                    if (c == 2) {
-                        v = elem_single<S>(T, da, db, up);
-                        keep = keep_single(v, single_rule);
-                    } else if (c == 4) {
-                        v = elem_same_double<S>(T, da, db);
-                        keep = (v != 0.0);
+                    keep = c == 2 || (c == 4 && dbl);
+                    if (FILL && keep) {
+                        const S up = B.ua[sb.g_row];
+                        swap_if(j < i, da, db);
+                        v = c == 2 ? single_or_zero(elem_single<S>(T, da, db, up), single_rule)
+                                   : elem_same_double<S>(T, da, db);
                     }
                 } else {
-                    S ua = B.ua[sb.g_row], uc = B.ua[sb.g_col];
-                    swap_if(j < i, ua, uc);
-                    swap_if(j < i, da, db);
-                    if (c == 0) {
-                        v = elem_single<S>(T, ua, uc, da);
-                        keep = keep_single(v, single_rule);
-                    } else if (c == 2) {
-                        v = elem_opp_double<S>(T, ua, uc, da, db);
-                        keep = (v != 0.0);
+                    keep = c == 0 || (c == 2 && dbl);
+                    if (FILL && keep) {
+                        S ua = B.ua[sb.g_row], uc = B.ua[sb.g_col];
+                        swap_if(j < i, ua, uc);
+                        swap_if(j < i, da, db);
+                        v = c == 0 ? single_or_zero(elem_single<S>(T, ua, uc, da), single_rule)
+                                   : elem_opp_double<S>(T, ua, uc, da, db);
                     }
                 }
             }
@@ -434,6 +435,7 @@ __global__ void __launch_bounds__(kBlock)
         emit(keep, rl, cl, v);
     }
     if (!FILL && lane < n_slices) win_cnt[sb.win_begin + lane] = wcur[lane];
+    if (FILL && lane == 0) atomicAdd(nnz_out, nz);
 }
 
 // window k of sub-block b covers columns [c0 + 256 s, ...) of b
@@ -826,6 +828,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     std::vector<SubBlock> sbs;
     std::vector<TileTask> ta, tb;
     int64_t n_wall = 0;
+    const int dbl = c->doubles_vanish ? 0 : 1;  // no same-beta alpha doubles to place otherwise
     sbs.reserve((size_t)n_pairs + (size_t)Ua + (size_t)Ub + 16);
     auto add_cols = [&](int32_t r0, int32_t nr, int32_t g_row, int32_t g_col, int kind, int32_t c0, int32_t c1) {
         for (int32_t cs = c0; cs < c1; cs += kTileCols) {
@@ -859,7 +862,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         for (int32_t r0 = gb[g]; r0 < gb[g + 1]; r0 += kTileRows) {
             const int32_t nr = std::min(kTileRows, gb[g + 1] - r0);
             TileTask t{r0, nr, (int32_t)n_wall, 0};
-            if (gb[g + 1] - gb[g] > 1) add_cols(r0, nr, g, g, 2, gb[g], gb[g + 1]);
+            if (gb[g + 1] - gb[g] > 1 && dbl) add_cols(r0, nr, g, g, 2, gb[g], gb[g + 1]);
             t.w_end = (int32_t)n_wall;
             tb.push_back(t);
         }
@@ -868,7 +871,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (n_wall >= (int64_t)INT32_MAX) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many tiles");
     DevBufs wb;
     SubBlock* d_sb = nullptr;
-    int64_t *d_wc = nullptr, *d_sz = nullptr, *d_woff = nullptr, *d_sum = nullptr;
+    int64_t *d_wc = nullptr, *d_sz = nullptr, *d_woff = nullptr, *d_sum = nullptr, *d_nnz = nullptr;
     int32_t *d_flag = nullptr, *d_nidx = nullptr, *d_wc0 = nullptr, *d_wnc = nullptr;
     HIP_TRY(c, wb.alloc(&d_sb, (size_t)n_sb + 1));
     HIP_TRY(c, wb.alloc(&d_wc, (size_t)n_wall + 1));
@@ -879,6 +882,8 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, wb.alloc(&d_nidx, (size_t)n_wall + 1));
     HIP_TRY(c, wb.alloc(&d_wc0, (size_t)n_wall + 1));
     HIP_TRY(c, wb.alloc(&d_wnc, (size_t)n_wall + 1));
+    HIP_TRY(c, wb.alloc(&d_nnz, 1));
+    HIP_TRY(c, hipMemsetAsync(d_nnz, 0, 8, s));
     if (n_sb > 0) HIP_TRY(c, hipMemcpyAsync(d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice, s));
     H->n_task_a = (int64_t)ta.size();
     H->n_task_b = (int64_t)tb.size();
@@ -891,7 +896,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (n_sb > 0) {
         hipLaunchKernelGGL(k_win_init, grid_for(n_sb, 256), 256, 0, s, d_sb, n_sb, d_wc0, d_wnc);
         hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           o.single_rule, d_wc, none);
+                           o.single_rule, dbl, d_wc, none, nullptr);
     }
     // window slots, offsets, the non-empty windows' new indices and the entry total
     hipLaunchKernelGGL(k_win_sizes, grid_for(n_wall + 1, 256), 256, 0, s, d_wc, n_wall, d_sz, d_flag);
@@ -900,14 +905,13 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, exclusive_scan(tmp, d_flag, d_nidx, (size_t)n_wall + 1, s));
     HIP_TRY(c, hipMemsetAsync(d_wc + n_wall, 0, 8, s));
     HIP_TRY(c, exclusive_scan(tmp, d_wc, d_sum, (size_t)n_wall + 1, s));
-    int64_t end = 0, nnz = 0;
+    int64_t end = 0, placed = 0;
     int32_t n_win = 0;
     HIP_TRY(c, hipMemcpyAsync(&end, d_woff + n_wall, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipMemcpyAsync(&nnz, d_sum + n_wall, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&placed, d_sum + n_wall, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(&n_win, d_nidx + n_wall, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    H->nnz_s = nnz;
-    H->nnz_u = nnz / 2;
+    (void)placed;
     H->nnz_alloc = end;
     H->n_win = n_win;
     const size_t cap = (size_t)end + kWinAlign;
@@ -925,7 +929,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMalloc(&H->d_win, (size_t)n_win * sizeof(Window) + 16));
     if (n_sb > 0)
         hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           o.single_rule, nullptr, fo);
+                           o.single_rule, dbl, nullptr, fo, (unsigned long long*)d_nnz);
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
                        H->d_win);
     if (!ta.empty())
@@ -939,7 +943,11 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
     HIP_TRY(c, hipMalloc(&H->d_xb, (size_t)N * 8 + 16));
     HIP_TRY(c, hipMalloc(&H->d_yb, (size_t)N * 8 + 16));
+    int64_t nnz = 0;  // stored nonzero entries (both triangles)
+    HIP_TRY(c, hipMemcpyAsync(&nnz, d_nnz, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));  // the passes read the work buffers freed on return
+    H->nnz_s = nnz;
+    H->nnz_u = nnz / 2;
     return HCI_OK;
 }
 

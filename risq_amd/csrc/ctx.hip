@@ -170,9 +170,20 @@ int hci_set_integrals(hci_ctx* c, int n_orb, double e_core, const double* one_bo
     HIP_TRY(c, hipMemcpy(c->d_h1, h1.data(), N * N * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_J3, J3.data(), N * N * N * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_D3, D3.data(), N * N * N * sizeof(double), hipMemcpyHostToDevice));
+    // double excitations only read (pr|qs) with p != r and q != s
+    bool vanish = true;
+    for (size_t i = 1; i <= N && vanish; ++i)
+        for (size_t j = i + 1; j <= N && vanish; ++j)
+            for (size_t k = 1; k <= N && vanish; ++k)
+                for (size_t l = k + 1; l <= N; ++l)
+                    if (two_body[hc4(i, j, k, l)] != 0.0) {
+                        vanish = false;
+                        break;
+                    }
     c->n_orb = n_orb;
     c->e_core = e_core;
     c->n_two = need2;
+    c->doubles_vanish = vanish;
     return HCI_OK;
 }
 
@@ -354,7 +365,7 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
                             q = bpos[q];
                         }
                         const int32_t i = perm[p], j2 = perm[q];
-                        if (i < j2) f(i, j2, iv[(size_t)e]);
+                        if (i < j2 && iv[(size_t)e] != 0.0) f(i, j2, iv[(size_t)e]);
                     }
                 }
             }

@@ -71,6 +71,7 @@ struct hci_ctx {
     double* d_J3 = nullptr;
     double* d_D3 = nullptr;
     size_t n_two = 0;
+    bool doubles_vanish = false;  // every (pq|rs) with p != q and r != s is zero (Hubbard)
     MlpState* mlp = nullptr;
     hipEvent_t events[16] = {};
 

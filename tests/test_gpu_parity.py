@@ -97,17 +97,27 @@ def test_build_u128_strings(gpu_ctx, oracle_lib):
     assert np.array_equal(val.view(np.uint64), oval.view(np.uint64))
 
 
-def test_build_hubbard_doubles_vanish(gpu_ctx, oracle_lib):
-    """Hubbard: every double is exactly zero and dropped; singles are +-t hops."""
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER, STORE_TILED])
+def test_build_hubbard_doubles_vanish(gpu_ctx, oracle_lib, storage):
+    """Hubbard: every double is exactly zero and dropped; singles are +-t hops.
+    Hops between non-adjacent sites are exact zeros that the storage rule drops
+    (the tiled layout places them as explicit zeros, which must not show up in
+    the pattern, the count or the product)."""
     ints, dets = hubbard_space(3, 3, 4, 4, 2000)
     gpu_ctx.set_integrals(ints)
     diag = oracle_lib.OracleHam(ints).ham_diag(dets)
-    h = gpu_ctx.build_h(dets, diag)
+    h = gpu_ctx.build_h(dets, diag, storage=storage)
     rp, col, val, _ = h.download()
-    orp, ocol, oval, _ = _oracle_build(oracle_lib, ints, dets, diag).export()
+    S = _oracle_build(oracle_lib, ints, dets, diag)
+    orp, ocol, oval, _ = S.export()
+    assert h.nnz == S.nnz
     assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
     assert np.array_equal(val.view(np.uint64), oval.view(np.uint64))
     assert np.all(np.isin(np.abs(val), [1.0]))
+    x = np.random.default_rng(3).standard_normal(len(dets))
+    y = h.matrix_vector_prod(x)
+    yo = S.spmv(x)
+    assert np.max(np.abs(y - yo)) <= 1e-12 * np.max(np.abs(yo))
 
 
 @pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER])
