@@ -101,14 +101,14 @@ __device__ __forceinline__ void put(const FillOut& o, int64_t e, int32_t q, doub
     }
 }
 
-// TILED fill target: window w holds wcnt[w] entries from slot woff[w]
+// TILED fill target: the entries of window w go to woff[w] + position, in
+// the order of the enumeration; k_interleave then moves them to their slots
 struct TileOut {
     uint16_t* idx;
     double* val;
     float* hi;
     float* lo;
     const int64_t* woff;
-    const int64_t* wcnt;
 };
 
 __device__ __forceinline__ void put16(const TileOut& o, int64_t e, uint32_t u, double v) {
@@ -286,7 +286,6 @@ __global__ void __launch_bounds__(kBlock)
                    int single_rule, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out) {
     __shared__ int32_t s_cur[kWaves][kWinPerSb];
     __shared__ int64_t s_off[kWaves][kWinPerSb];
-    __shared__ int64_t s_cnt[kWaves][kWinPerSb];
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
     if (w >= n_sb) return;
@@ -295,12 +294,8 @@ __global__ void __launch_bounds__(kBlock)
     const int n_slices = (sb.nc + kWinCols - 1) / kWinCols;
     int32_t* wcur = s_cur[threadIdx.x >> 6];
     int64_t* woff = s_off[threadIdx.x >> 6];
-    int64_t* wcnt = s_cnt[threadIdx.x >> 6];
     if (lane < kWinPerSb) wcur[lane] = 0;
-    if (FILL && lane < n_slices) {
-        woff[lane] = out.woff[sb.win_begin + lane];
-        wcnt[lane] = out.wcnt[sb.win_begin + lane];
-    }
+    if (FILL && lane < n_slices) woff[lane] = out.woff[sb.win_begin + lane];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -322,7 +317,7 @@ __global__ void __launch_bounds__(kBlock)
             pos = atomicAdd(&wcur[slice], 1);
         }
         if (FILL && keep)
-            put16(out, woff[slice] + win_slot(pos, wcnt[slice]), ((uint32_t)rl << 8) | (uint32_t)(cl % kWinCols), v);
+            put16(out, woff[slice] + pos, ((uint32_t)rl << 8) | (uint32_t)(cl % kWinCols), v);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -467,6 +462,60 @@ __global__ void k_win_compact(const int64_t* wc, const int64_t* woff, const int3
     w.c0 = wc0[k];
     w.cnt_nc = (uint32_t)wc[k] | ((uint32_t)wnc[k] << 20);
     out[nidx[k]] = w;
+}
+
+// In place, one wavefront per window: every 512-entry piece (the last one
+// shorter) is read in enumeration order into the wave's LDS and written back
+// slot by slot (the inverse of win_slot, whose slots stay inside the piece's
+// own range), so both passes over memory are contiguous.  A window
+// of at most 8 entries is one group whose slots are its positions.
+constexpr int kIlWaves = 4;
+__global__ void __launch_bounds__(64 * kIlWaves)
+    k_interleave(const Window* __restrict__ wins, int64_t n_win, uint16_t* idx, double* val, float* hi,
+                 float* lo) {
+    __shared__ uint16_t si[kIlWaves][kWinChunk];
+    __shared__ double sv[kIlWaves][kWinChunk];
+    __shared__ float sh[kIlWaves][kWinChunk], sl[kIlWaves][kWinChunk];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t k = (int64_t)blockIdx.x * kIlWaves + wv; k < n_win; k += (int64_t)gridDim.x * kIlWaves) {
+    const Window W = wins[k];
+    const int cnt = W.cnt();
+    if (cnt <= kWinAlign) continue;
+    for (int base = 0; base < cnt; base += kWinChunk) {
+        const int n = min(kWinChunk, cnt - base);
+        for (int t = lane; t < n; t += 64) {
+            const int64_t e = W.off + base + t;
+            si[wv][t] = idx[e];
+            if (val) {
+                sv[wv][t] = val[e];
+            } else {
+                sh[wv][t] = hi[e];
+                sl[wv][t] = lo[e];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // slot s of the piece holds entry inv(s): writes stay contiguous
+        const bool full = n == kWinChunk;
+        const int R = (n + kWinAlign - 1) / kWinAlign;
+        for (int t = lane; t < R * kWinAlign; t += 64) {
+            const int j = full ? (t & 7) * 64 + (t >> 3) : (t >> 3) + (t & 7) * R;
+            if (j >= n) continue;
+            const int64_t e = W.off + base + t;
+            idx[e] = si[wv][j];
+            if (val) {
+                val[e] = sv[wv][j];
+            } else {
+                hi[e] = sh[wv][j];
+                lo[e] = sl[wv][j];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    }
 }
 
 __global__ void k_task_remap(TileTask* t, int64_t n, const int32_t* nidx) {
@@ -892,7 +941,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMemcpyAsync(H->d_task_a, ta.data(), ta.size() * sizeof(TileTask), hipMemcpyHostToDevice, s));
     HIP_TRY(c, hipMemcpyAsync(H->d_task_b, tb.data(), tb.size() * sizeof(TileTask), hipMemcpyHostToDevice, s));
     const DevInts T = c->ints();
-    TileOut none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    TileOut none{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (n_sb > 0) {
         hipLaunchKernelGGL(k_win_init, grid_for(n_sb, 256), 256, 0, s, d_sb, n_sb, d_wc0, d_wnc);
         hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
@@ -916,7 +965,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     H->n_win = n_win;
     const size_t cap = (size_t)end + kWinAlign;
     HIP_TRY(c, hipMalloc(&H->d_idx16, cap * 2));
-    TileOut fo{H->d_idx16, nullptr, nullptr, nullptr, d_woff, d_wc};
+    TileOut fo{H->d_idx16, nullptr, nullptr, nullptr, d_woff};
     if (o.value_prec == HCI_PREC_F64) {
         HIP_TRY(c, hipMalloc(&H->d_val, cap * 8));
         fo.val = H->d_val;
@@ -932,6 +981,9 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
                            o.single_rule, dbl, nullptr, fo, (unsigned long long*)d_nnz);
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
                        H->d_win);
+    if (n_win > 0)
+        hipLaunchKernelGGL(k_interleave, std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u), 64 * kIlWaves, 0, s,
+                           H->d_win, (int64_t)n_win, H->d_idx16, H->d_val, H->d_hi, H->d_lo);
     if (!ta.empty())
         hipLaunchKernelGGL(k_task_remap, grid_for((int64_t)ta.size(), 256), 256, 0, s, H->d_task_a,
                            (int64_t)ta.size(), d_nidx);
