@@ -93,7 +93,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the untimed comparison runs (FP64 reference path, other preconditioner)")
-    ap.add_argument("--precond", choices=["mlp", "dpr"], default="dpr",
+    ap.add_argument("--precond", choices=["mlp", "dpr"], default="mlp",
                     help="correction of the timed path (the other one is reported beside it)")
     args = ap.parse_args()
     if args.gpus != 1:

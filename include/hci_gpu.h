@@ -197,7 +197,7 @@ int hci_davidson(hci_ctx* ctx, const hci_csr* h, const double* init, const hci_d
 /* Learned preconditioner (no reference counterpart: it replaces the diagonal
  * correction of davidson.rs:391-408 when opts.precond == HCI_PRECOND_MLP).
  * Network 24 -> 128 -> 128 -> 1 (SiLU), output o gives the bounded factor
- * g = 1 + 2 tanh(o) that scales the diagonal correction:
+ * g = 1 + tanh(o) / 2 in (1/2, 3/2) that scales the diagonal correction:
  * t_p = g_p r_p / (lambda - H_pp).  Parameters are FP32, row-major
  * W1[24][128], b1[128], W2[128][128], b2[128], W3[128], b3 (HCI_MLP_NPARAM
  * floats); features are standardised with feat_mean / feat_std.  Products run

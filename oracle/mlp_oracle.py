@@ -1,7 +1,7 @@
 """CPU oracle of the learned preconditioner network -- TEST INFRASTRUCTURE ONLY.
 
 FP64 numpy forward/backward of the 24 -> 128 -> 128 -> 1 SiLU network with
-bounded factor g = 1 + 2 tanh(out) and loss mean((g m - y)^2).  With emulate=True every matrix-product operand
+bounded factor g = 1 + G_AMP tanh(out) (G_AMP = 1/2) and loss mean((g m - y)^2).  With emulate=True every matrix-product operand
 is rounded to bfloat16 (round to nearest even) and feature standardisation is
 done in FP32, at the same points as the GPU kernels, so the two agree to
 accumulation-order differences.  The backward pass is pinned by finite
@@ -10,6 +10,7 @@ differences (tests/test_mlp_oracle.py).
 import numpy as np
 
 F, H = 24, 128
+G_AMP = 0.5
 NPARAM = F * H + H + H * H + H + H + 1
 
 
@@ -62,9 +63,9 @@ def loss_grad(p, X, m, y, mean, std, emulate=False):
     out, (Xn, Z1, H1, S1, Z2, H2, S2) = forward(p, X, mean, std, emulate)
     n = X.shape[0]
     th = np.tanh(out)
-    r = (1.0 + 2.0 * th) * m - y
+    r = (1.0 + G_AMP * th) * m - y
     loss = float(np.mean(r * r))
-    dout = (2.0 * r * m * 2.0 * (1.0 - th * th) / n)[:, None]
+    dout = (2.0 * r * m * G_AMP * (1.0 - th * th) / n)[:, None]
     gW3 = _mm(H2.T, dout, emulate)
     gb3 = dout.sum(axis=0)
     dH2 = _mm(dout, W3.T, emulate)

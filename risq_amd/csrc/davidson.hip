@@ -482,8 +482,11 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         }
         // an MLP correction is judged by what it did to the residual: after two
         // steps that did not reduce it, the diagonal correction takes over
-        if (net && mlp_prev && !(err < err_prev)) ++st.mlp_fallbacks;
-        const bool mlp_now = net && st.mlp_fallbacks < 2;
+        // the step after a network correction that did not lower the residual
+        // norm takes the diagonal correction; the network is tried again after it
+        const bool failed = net && mlp_prev && !(err < err_prev);
+        if (failed) ++st.mlp_fallbacks;
+        const bool mlp_now = net && !failed;
         last_eig = lam;
         // stall counter: iterations without a 30% residual improvement, counted
         // only near convergence (an early plateau while the ground state is

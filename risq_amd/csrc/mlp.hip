@@ -1,6 +1,6 @@
 // Learned Davidson preconditioner: a 24 -> 128 -> 128 -> 1 SiLU network whose
-// output o_p gives a bounded factor g_p = 1 + 2 tanh(o_p) that rescales the
-// diagonal correction, t_p = g_p * r_p / (lambda - H_pp),
+// output o_p gives a bounded positive factor g_p = 1 + tanh(o_p) / 2 in
+// (1/2, 3/2) that rescales the diagonal correction, t_p = g_p * r_p / (lambda - H_pp),
 // for every determinant p.  Training pairs are recorded from the library's own
 // Davidson runs on small systems: per-determinant features and the exact
 // correction, i.e. the part of the converged eigenvector not yet in the
@@ -12,6 +12,9 @@
 // All matrix products (forward and backward) run on the matrix cores through
 // v_mfma_f32_16x16x16_bf16: operands are rounded to bfloat16 on load, products
 // accumulate in FP32.  Master weights, Adam moments and activations are FP32.
+// Training runs layer by layer (k_gemm_bf16); inference inside Davidson runs
+// all three layers in one kernel (k_mlp_infer) with the bfloat16 weights in
+// LDS and each wave's 16-row activations never leaving the CU.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -25,6 +28,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int F = HCI_MLP_NFEAT;
 constexpr int Hd = HCI_MLP_HIDDEN;
+constexpr float kGAmp = 0.5f;  // g = 1 + kGAmp tanh(out)
 // parameter offsets (row-major weights: W[in][out])
 constexpr int oW1 = 0, ob1 = oW1 + F * Hd, oW2 = ob1 + Hd, ob2 = oW2 + Hd * Hd, oW3 = ob2 + Hd, ob3 = oW3 + Hd;
 static_assert(ob3 + 1 == HCI_MLP_NPARAM, "parameter layout");
@@ -93,16 +97,17 @@ __global__ void k_silu_bwd(const float* dH, const float* Z, float* dZ, int64_t n
     dZ[i] = dH[i] * s * (1.f + Z[i] * (1.f - s));
 }
 
-// The network's scale factor is bounded: g = 1 + 2 tanh(out), in (-1, 3).
+// The network's scale factor is bounded and positive: g = 1 + kGAmp tanh(out).
+// A positive factor keeps every component of the diagonal correction's sign.
 // loss = (1/n) sum_i (g_i m_i - y_i)^2 ; dout_i = 2 (g_i m_i - y_i) m_i g'_i / n
 __global__ void k_loss(const float* out, const float* m, const float* y, int64_t n, float* dout, double* partial) {
     __shared__ double sh[256];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float th = tanhf(out[i]);
-        const float r = (1.f + 2.f * th) * m[i] - y[i];
+        const float r = (1.f + kGAmp * th) * m[i] - y[i];
         acc += (double)r * r;
-        dout[i] = 2.f * r * m[i] * 2.f * (1.f - th * th) / (float)n;
+        dout[i] = 2.f * r * m[i] * kGAmp * (1.f - th * th) / (float)n;
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -185,11 +190,139 @@ __global__ void k_features(const uint64_t* words, const double* diag, const doub
     o[23] = 0.125f * (float)(popc(up ^ hfu) + popc(dn ^ hfd));
 }
 
-__global__ void k_apply(const float* out, const double* dpr, int64_t n, double* t) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const float g = 1.f + 2.f * tanhf(out[p]);
-    t[p] = (double)g * dpr[p];
+// bfloat16 weight image for k_mlp_infer, in its LDS layout: W1 and W2
+// transposed (row n holds the weights into output n, so a lane's 4 K values
+// are contiguous), rows padded to kW1Row / kW2Row values against LDS bank
+// conflicts (W1's K padded from 24 to 32 with zeros), then W3.
+constexpr int kK1 = 32;
+constexpr int kW1Row = 40, kW2Row = Hd + 8;
+constexpr int kImgW1 = Hd * kW1Row, kImgW2 = Hd * kW2Row, kImg = kImgW1 + kImgW2 + Hd;
+
+__device__ __forceinline__ float from_bf16(short h) { return __uint_as_float((uint32_t)(uint16_t)h << 16); }
+
+__global__ void k_prep_weights(const float* P, short* img) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kImgW1) {
+        const int n = i / kW1Row, k = i % kW1Row;
+        img[i] = k < F ? to_bf16(P[oW1 + k * Hd + n]) : (short)0;
+    }
+    if (i < kImgW2) {
+        const int n = i / kW2Row, k = i % kW2Row;
+        img[kImgW1 + i] = k < Hd ? to_bf16(P[oW2 + k * Hd + n]) : (short)0;
+    }
+    if (i < Hd) img[kImgW1 + kImgW2 + i] = to_bf16(P[oW3 + i]);
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Fused inference: each wave takes 16 rows at a time: standardised features
+// (bf16, LDS) -> Z1 = X W1 + b1 (2 MFMA K-steps x 8 column tiles) -> SiLU ->
+// H1 (bf16, LDS) -> Z2 = H1 W2 + b2 (8 x 8 MFMAs) -> SiLU -> out = H2 . W3 + b3
+// (bf16 operands, FP32 sums over the 16 lanes of a row group).  Writes out
+// (if given) and t = (1 + kGAmp tanh(out)) * dpr (if given).  The same
+// roundings as the layer-by-layer path: operands to bf16, FP32 accumulation.
+constexpr int kInfWaves = 4;
+constexpr int kXRow = kW1Row, kHRow = kW2Row;
+__global__ void __launch_bounds__(64 * kInfWaves)
+    k_mlp_infer(const float* __restrict__ X, int64_t n, const float* __restrict__ mean,
+                const float* __restrict__ stdv, const short* __restrict__ img, const float* __restrict__ P,
+                const double* __restrict__ dpr, float* out, double* t) {
+    __shared__ __align__(16) short s_w[kImgW1 + kImgW2];
+    __shared__ __align__(16) short s_x[kInfWaves][16 * kXRow];
+    __shared__ __align__(16) short s_h[kInfWaves][16 * kHRow];
+    __shared__ float s_b1[Hd], s_b2[Hd], s_w3[Hd], s_mu[F], s_sd[F];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < (kImgW1 + kImgW2) / 2; i += blockDim.x) ((int*)s_w)[i] = ((const int*)img)[i];
+    for (int i = tid; i < Hd; i += blockDim.x) {
+        s_b1[i] = P[ob1 + i];
+        s_b2[i] = P[ob2 + i];
+        s_w3[i] = from_bf16(img[kImgW1 + kImgW2 + i]);
+    }
+    for (int i = tid; i < F; i += blockDim.x) {
+        s_mu[i] = mean[i];
+        s_sd[i] = stdv[i];
+    }
+    __syncthreads();
+    const float b3 = P[ob3];
+    const short* s_w1 = s_w;
+    const short* s_w2 = s_w + kImgW1;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int r16 = lane & 15, kq = (lane >> 4) * 4;
+    short* xs = s_x[wv];
+    short* hs = s_h[wv];
+    for (int64_t g0 = ((int64_t)blockIdx.x * kInfWaves + wv) * 16; g0 < n;
+         g0 += (int64_t)gridDim.x * kInfWaves * 16) {
+        for (int j = lane; j < 16 * kK1; j += 64) {
+            const int rr = j / kK1, k = j % kK1;
+            const int64_t row = g0 + rr;
+            float v = 0.f;
+            if (k < F && row < n) v = (X[row * F + k] - s_mu[k]) / s_sd[k];
+            xs[rr * kXRow + k] = to_bf16(v);
+        }
+        wave_sync();
+        f32x4 acc[Hd / 16];
+#pragma unroll
+        for (int nt = 0; nt < Hd / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < kK1; ks += 16) {
+            const bf16x4 a = *(const bf16x4*)&xs[r16 * kXRow + ks + kq];
+#pragma unroll
+            for (int nt = 0; nt < Hd / 16; ++nt) {
+                const bf16x4 b = *(const bf16x4*)&s_w1[(nt * 16 + r16) * kW1Row + ks + kq];
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < Hd / 16; ++nt) {
+            const int col = nt * 16 + r16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float z = acc[nt][i] + s_b1[col];
+                hs[(kq + i) * kHRow + col] = to_bf16(z * sigmoid(z));
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int nt = 0; nt < Hd / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < Hd; ks += 16) {
+            const bf16x4 a = *(const bf16x4*)&hs[r16 * kHRow + ks + kq];
+#pragma unroll
+            for (int nt = 0; nt < Hd / 16; ++nt) {
+                const bf16x4 b = *(const bf16x4*)&s_w2[(nt * 16 + r16) * kW2Row + ks + kq];
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+            }
+        }
+        float o[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int nt = 0; nt < Hd / 16; ++nt) {
+            const int col = nt * 16 + r16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float z = acc[nt][i] + s_b2[col];
+                o[i] += from_bf16(to_bf16(z * sigmoid(z))) * s_w3[col];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) o[i] += __shfl_xor(o[i], d);
+        if (r16 == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t row = g0 + kq + i;
+                if (row < n) {
+                    const float v = o[i] + b3;
+                    if (out) out[row] = v;
+                    if (t) t[row] = (double)(1.f + kGAmp * tanhf(v)) * dpr[row];
+                }
+            }
+        }
+        wave_sync();
+    }
 }
 
 unsigned gemm_grid(int64_t M, int N) { return grid_for(((M + 15) / 16) * ((N + 15) / 16), 4); }
@@ -210,6 +343,7 @@ struct MlpState {
     double* partial = nullptr;
     float* feat = nullptr;   // Davidson-time features (n x F) and outputs
     float* gout = nullptr;
+    short* img = nullptr;    // bfloat16 weight image for k_mlp_infer
     int64_t feat_cap = 0;
     bool ready = false;      // parameters and standardisation loaded
     int64_t adam_t = 0;
@@ -225,6 +359,7 @@ struct MlpState {
         free_work();
         for (float* q : {P, G, A1, A2, mean, stdv, feat, gout}) hipFree(q);
         hipFree(partial);
+        hipFree(img);
     }
 };
 
@@ -241,6 +376,7 @@ int ensure_state(hci_ctx* c) {
     HIP_TRY(c, hipMalloc(&s->mean, F * 4));
     HIP_TRY(c, hipMalloc(&s->stdv, F * 4));
     HIP_TRY(c, hipMalloc(&s->partial, 1024 * 8));
+    HIP_TRY(c, hipMalloc(&s->img, (size_t)kImg * 2 + 16));
     HIP_TRY(c, hipMemset(s->A1, 0, HCI_MLP_NPARAM * 4));
     HIP_TRY(c, hipMemset(s->A2, 0, HCI_MLP_NPARAM * 4));
     std::vector<float> one(F, 1.f), zero(F, 0.f);
@@ -279,6 +415,15 @@ void forward(hci_ctx* c, int64_t n) {
     gemm(st, n, Hd, Hd, s->H1, Hd, 1, P + oW2, Hd, 1, s->Z2, Hd, 1, P + ob2, 0);
     hipLaunchKernelGGL(k_silu_fwd, grid_for(n * Hd, 256), 256, 0, st, s->Z2, s->H2, n * Hd);
     gemm(st, n, 1, Hd, s->H2, Hd, 1, P + oW3, 1, 1, s->out, 1, 1, P + ob3, 0);
+}
+
+// fused inference on raw features X (n rows): out and/or t = g * dpr
+void infer(hci_ctx* c, const float* X, int64_t n, float* out, const double* dpr, double* t) {
+    MlpState* s = c->mlp;
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(k_prep_weights, grid_for(kImgW2, 256), 256, 0, st, s->P, s->img);
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for((n + 15) / 16, kInfWaves), 2048);
+    hipLaunchKernelGGL(k_mlp_infer, grid, 64 * kInfWaves, 0, st, X, n, s->mean, s->stdv, s->img, s->P, dpr, out, t);
 }
 
 // loss and gradients for the batch in Xn / mb / yb (n rows); returns the loss
@@ -371,16 +516,7 @@ int mlp_correction(hci_ctx* c, const hci_csr* h, double lam, double rnorm, doubl
     }
     int rc = mlp_features(c, h, lam, rnorm, tnorm, r, x, dpr, s->feat);
     if (rc) return rc;
-    const int64_t chunk = 1 << 18;
-    rc = ensure_work(c, std::min<int64_t>(n, chunk));
-    if (rc) return rc;
-    for (int64_t a = 0; a < n; a += chunk) {
-        const int64_t b = std::min<int64_t>(n, a + chunk) - a;
-        hipLaunchKernelGGL(k_normalize, grid_for(b * F, 256), 256, 0, st, s->feat + a * F, s->mean, s->stdv, b, s->Xn);
-        forward(c, b);
-        HIP_TRY(c, hipMemcpyAsync(s->gout + a, s->out, (size_t)b * 4, hipMemcpyDeviceToDevice, st));
-    }
-    hipLaunchKernelGGL(k_apply, grid_for(n, 256), 256, 0, st, s->gout, dpr, n, t);
+    infer(c, s->feat, n, nullptr, dpr, t);
     HIP_TRY(c, hipGetLastError());
     return HCI_OK;
 }
@@ -465,18 +601,19 @@ int hci_mlp_forward(hci_ctx* c, const float* X, int64_t n, float* out, int mem) 
     DevIn dx;
     int rc = dev_in(c, X, (size_t)n * F, mem, dx);
     if (rc) return rc;
-    MlpState* s = c->mlp;
-    const int64_t chunk = 1 << 18;
-    rc = ensure_work(c, std::min<int64_t>(std::max<int64_t>(n, 1), chunk));
-    if (rc) return rc;
-    for (int64_t a = 0; a < n; a += chunk) {
-        const int64_t b = std::min<int64_t>(n, a + chunk) - a;
-        hipLaunchKernelGGL(k_normalize, grid_for(b * F, 256), 256, 0, c->stream, dx.p + a * F, s->mean, s->stdv, b, s->Xn);
-        forward(c, b);
-        HIP_TRY(c, hipMemcpyAsync(out + a, s->out, (size_t)b * 4,
-                                  mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    if (n == 0) return HCI_OK;
+    float* dout = out;
+    float* own = nullptr;
+    if (mem != HCI_MEM_DEVICE) {
+        HIP_TRY(c, hipMalloc(&own, (size_t)n * 4 + 16));
+        dout = own;
     }
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    infer(c, dx.p, n, dout, nullptr, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && own) e = hipMemcpyAsync(out, own, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(own);
+    HIP_TRY(c, e);
     return HCI_OK;
 }
 

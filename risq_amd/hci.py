@@ -317,11 +317,12 @@ class HciContext:
         return {k: getattr(st, k) for k, _ in st._fields_}
 
     def mlp_collect(self, h: "SparseMatUpperTri", x_exact: np.ndarray, per_iter: int, cap: int,
-                    max_iters: int = 300, tol: float = 1e-5, etol: float = 1e-9):
+                    max_iters: int = 300, tol: float = 1e-5, etol: float = 1e-9, restart_keep: int = 1):
         x_exact = np.ascontiguousarray(x_exact, dtype=np.float64)
         opts = DavOpts()
         self.lib.hci_dav_default_opts(C.byref(opts))
         opts.tol, opts.etol, opts.max_iters = tol, etol, max_iters
+        opts.restart_keep = restart_keep
         X = np.empty((cap, MLP_NFEAT), dtype=np.float32)
         m = np.empty(cap, dtype=np.float32)
         y = np.empty(cap, dtype=np.float32)

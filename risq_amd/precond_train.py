@@ -1,12 +1,15 @@
 """Training of the learned Davidson preconditioner (run on the GPU).
 
-Training systems follow the small-system recipe of the synthetic molecular
-integrals (h_pp = -1 + 2p/15, h_pq ~ N(0, 0.05^2), (pq|rs) from K = 2 n_orb
-Cholesky vectors with L ~ N(0, 0.08^2)) at n_orb 20-28 and 8-10 electrons,
-seeds 100 + i, over energy-truncated determinant spaces.  For each system the
-library converges Davidson tightly, then re-runs it and records, at every
-iteration, per-determinant features with the diagonal correction and the exact
-correction (hci_mlp_collect).  The network is trained on the pooled rows with
+Training systems follow the recipe of the synthetic molecular integrals
+(h_pp = -1 + 2p/15, h_pq ~ N(0, 0.05^2), (pq|rs) from K = 2 n_orb Cholesky
+vectors with L ~ N(0, 0.08^2)) over energy-truncated determinant spaces: the
+small systems at n_orb 20-28 and 8-10 electrons (seeds 100 + i), and smaller
+spaces (1.5e5 determinants) of the n_orb 40, 12-electron system at seeds 2-5
+(the 2e6-determinant benchmark system is seed 1 and is not trained on).  For
+each system the library converges Davidson tightly, then re-runs it with the
+restart that keeps the previous Ritz vector (the fast path's) and records, at
+every iteration, per-determinant features with the diagonal correction and
+the exact correction (hci_mlp_collect).  The network is trained on the pooled rows with
 the library's BF16-MFMA kernels and saved as an .npz (parameters + feature
 standardisation).
 
@@ -28,7 +31,8 @@ DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "precond
 
 # (n_orb, electrons per spin, determinants, seed)
 TRAINING_SYSTEMS = [(20, 4, 30000, 100), (22, 5, 30000, 101), (24, 4, 40000, 102),
-                    (26, 5, 40000, 103), (28, 4, 50000, 104), (28, 5, 50000, 105)]
+                    (26, 5, 40000, 103), (28, 4, 50000, 104), (28, 5, 50000, 105),
+                    (40, 6, 150000, 2), (40, 6, 150000, 3), (40, 6, 150000, 4), (40, 6, 150000, 5)]
 
 
 def collect(ctx: HciContext, systems=TRAINING_SYSTEMS, per_iter: int = 4096, log=print):
@@ -42,7 +46,7 @@ def collect(ctx: HciContext, systems=TRAINING_SYSTEMS, per_iter: int = 4096, log
         h = ctx.build_h(dets, ctx.ham_diag(dets))
         exact = ctx.davidson(h, tol=1e-8, etol=1e-12, max_iters=2000)
         X, m, y = ctx.mlp_collect(h, exact.coeffs, per_iter=per_iter, cap=per_iter * 400, max_iters=400,
-                                  tol=1e-6, etol=1e-10)
+                                  tol=1e-6, etol=1e-10, restart_keep=2)
         Xs.append(X)
         ms.append(m)
         ys.append(y)
