@@ -31,3 +31,26 @@ def test_reference_energies_on_gpu(gpu_ctx, name):
           f"dets={len(r.dets)} iterations={r.iterations} {time.time() - t:.1f}s")
     assert r.converged
     assert abs(r.energy - c["energy"]) < 1e-6
+
+
+@pytest.mark.timeout(300)
+def test_synthetic_16_orbital_system_matches_oracle_run(gpu_ctx):
+    """The n_orb=16, 4a4b synthetic system (seed 0) at eps_var = 3e-3: the GPU
+    run of the selection loop against the oracle's run of the same loop
+    (tests/golden/config0_hci.json, made by tests/golden/make_config0_hci.py)."""
+    import numpy as np
+    from risq_amd.fcidump import synthetic_molecular
+    g = json.load(open(os.path.join(GOLD, "config0_hci.json")))
+    ints = synthetic_molecular(g["n_orb"], g["n_chol"], g["seed"])
+    solver = hs.GpuSolver(gpu_ctx, ints)
+    t = time.time()
+    r = hs.variational(solver, ints, g["nup"], g["ndn"], g["eps_var"])
+    solver.close()
+    print(f"synthetic n_orb=16: E={r.energy:.10f} (oracle run {g['energy']:.10f}) dets={len(r.dets)} "
+          f"(oracle {g['n_dets']}) iterations={r.iterations} {time.time() - t:.1f}s")
+    assert r.converged
+    assert abs(r.energy - g["energy"]) < 1e-6
+    assert len(r.dets) == g["n_dets"]
+    words = hs._words(r.dets)
+    chk = int(np.bitwise_xor.reduce(words[:, 0] * np.uint64(0x9E3779B97F4A7C15) ^ words[:, 2]))
+    assert chk == g["dets_checksum"]
