@@ -240,6 +240,24 @@ int hci_mlp_train(hci_ctx* ctx, const float* X, const float* m, const float* y, 
 int hci_mlp_collect(hci_ctx* ctx, const hci_csr* h, const double* x_exact, const hci_dav_opts* opts,
                     int64_t per_iter, float* X, float* m, float* y, int64_t cap, int64_t* count);
 
+/* Heat-bath selection (replaces VarWf::find_new_dets / iterate_excites,
+ * src/wf/mod.rs:1418-1631, with the ExciteGenerator of src/excite/init.rs:40-249
+ * and init_eps of src/wf/eps.rs:44-171).
+ * hci_excite_init builds the sorted excitation lists for the current integrals
+ * (core = the n_core lowest orbitals by h_pp) and returns in *eps_start the
+ * first epsilon of the schedule for the Hartree-Fock start (z_sym = 1).
+ * hci_select writes the determinants (4 words each) reached from dets with
+ * |H| >= eps / |coeff| that are not in dets, without duplicates, in the
+ * reference's discovery order; singles (exact element > eps / |coeff|) only if
+ * singles != 0.  *count = the number found; if it exceeds cap the call fails
+ * with HCI_ERR_ARG and nothing is written (call again with cap >= *count).
+ * The caller adds each one's spin-flipped partner (src/wf/mod.rs:99-116).
+ * hci_excite_info returns the largest magnitudes of the three lists. */
+int hci_excite_init(hci_ctx* ctx, int n_core, int n_up, int n_dn, double eps_var, double* eps_start);
+int hci_excite_info(const hci_ctx* ctx, double* max_opp, double* max_same, double* max_sing);
+int hci_select(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_t n, double eps, int singles,
+               uint64_t* out, int64_t cap, int64_t* count, int mem);
+
 /* Device-memory and timing plumbing (no reference counterpart: for callers
  * that keep inputs resident in HBM, and for timing on the context's stream).
  * Events are 16 slots per context recorded on the context stream. */

@@ -72,6 +72,7 @@ void hci_destroy(hci_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     mlp_free_state(c);
+    select_free(c);
     hipFree(c->d_two);
     hipFree(c->d_h1);
     hipFree(c->d_J3);
@@ -184,6 +185,9 @@ int hci_set_integrals(hci_ctx* c, int n_orb, double e_core, const double* one_bo
     c->e_core = e_core;
     c->n_two = need2;
     c->doubles_vanish = vanish;
+    c->h_one.assign(one_body, one_body + need1);
+    c->h_two.assign(two_body, two_body + need2);
+    select_free(c);  // lists of the previous integrals
     return HCI_OK;
 }
 

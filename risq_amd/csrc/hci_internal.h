@@ -9,7 +9,8 @@
 #include "../../include/hci_gpu.h"
 #include "hci_device.h"
 
-struct MlpState;  // learned preconditioner (mlp.hip)
+struct MlpState;     // learned preconditioner (mlp.hip)
+struct SelectLists;  // heat-bath excitation lists (select.hip)
 
 // TILED storage.  The build enumerates sub-blocks: the pairs between nr <= 256
 // rows starting at r0 and nc <= 4096 columns starting at c0 (both in the
@@ -72,7 +73,9 @@ struct hci_ctx {
     double* d_D3 = nullptr;
     size_t n_two = 0;
     bool doubles_vanish = false;  // every (pq|rs) with p != q and r != s is zero (Hubbard)
+    std::vector<double> h_one, h_two;  // host copies of the packed tables (selection lists)
     MlpState* mlp = nullptr;
+    SelectLists* sel = nullptr;
     hipEvent_t events[16] = {};
 
     DevInts ints() const {
@@ -154,6 +157,9 @@ int csr_rownorm(hci_csr* h);       // fills d_rownorm[p] = sum_q |H_pq| (off-dia
 // caller order <-> internal order
 void permute_to_internal(const hci_csr* h, const double* x_caller, double* x_int);
 void permute_to_caller(const hci_csr* h, const double* x_int, double* x_caller);
+
+// heat-bath selection (select.hip)
+void select_free(hci_ctx* c);
 
 // launch helpers
 inline unsigned grid_for(int64_t n, int block) {

@@ -503,7 +503,6 @@ void mlp_free_state(hci_ctx* c) {
 int mlp_correction(hci_ctx* c, const hci_csr* h, double lam, double rnorm, double tnorm, const double* r,
                    const double* x, const double* dpr, double* t) {
     MlpState* s = c->mlp;
-    hipStream_t st = c->stream;
     const int64_t n = h->n;
     if (n > s->feat_cap) {
         hipFree(s->feat);

@@ -1,0 +1,577 @@
+// Heat-bath selection of new determinants on the GPU.
+//
+// Lists (reference src/excite/init.rs:40-249), built once per integrals on the
+// host with the reference's stable sorts and summation order:
+//   opposite-spin doubles, for every ordered valence pair (p, q): targets
+//     (r, s), r != p, s != q, sorted by |(pr|qs)| descending;
+//   same-spin doubles, for every valence pair p < q: targets r < s outside
+//     {p, q}, sorted by |(pr|qs) - (ps|qr)| descending;
+//   singles, for every valence p: targets r != p sorted by the bound
+//     max(|h_pr + smallest partial sums|, |h_pr + largest partial sums|) over
+//     the n_up - 1 same-spin terms (pr|qq) - (pq|qr) and the n_dn
+//     opposite-spin terms (pr|qq).
+// Core orbitals are the n_core lowest by h_pp (src/ham/read_ints.rs:161-184).
+//
+// Selection (src/wf/mod.rs:1467-1631): one thread per determinant D with
+// local eps = eps / |c_D| walks, in the reference's order, its opposite-spin
+// pairs (i up, j dn ascending), its same-spin pairs (up then dn) and, when
+// asked, its singles, each list until the stored magnitude drops below local
+// eps; valid targets become candidates, singles only if the exact element
+// (elem_single, the same device function as every other element) exceeds local
+// eps.  A count pass sizes the candidate array; the fill pass writes the
+// candidates in discovery order (determinant-major).  Duplicates and members
+// of the current space are removed by stable radix sorts on the determinant
+// words: the first occurrence of each new determinant survives, and the
+// survivors are returned in discovery order, which is the reference's order
+// of new determinants.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <rocprim/rocprim.hpp>
+
+#include "hci_internal.h"
+
+struct SelectLists {
+    int n_core = 0, n_up = 0, n_dn = 0, nv = 0;
+    int64_t Lo = 0, Ls = 0, Lg = 0;  // list lengths: opposite doubles, same doubles, singles
+    double max_opp = 0.0, max_same = 0.0, max_sing = 0.0, eps_start = 0.0;
+    uint64_t vmask[2] = {0, 0};
+    double *d_oh = nullptr, *d_sh = nullptr, *d_gh = nullptr;
+    uint16_t *d_ot = nullptr, *d_st = nullptr, *d_gt = nullptr;  // targets r | s << 8
+    int32_t *d_vpos = nullptr, *d_spos = nullptr;
+    ~SelectLists() {
+        hipFree(d_oh);
+        hipFree(d_sh);
+        hipFree(d_gh);
+        hipFree(d_ot);
+        hipFree(d_st);
+        hipFree(d_gt);
+        hipFree(d_vpos);
+        hipFree(d_spos);
+    }
+};
+
+void select_free(hci_ctx* c) {
+    if (c && c->sel) {
+        delete c->sel;
+        c->sel = nullptr;
+    }
+}
+
+namespace {
+
+inline size_t sc2(size_t i, size_t j) { return i > j ? (i * (i - 1)) / 2 + j : (j * (j - 1)) / 2 + i; }
+
+struct DevSel {
+    int nv;
+    int64_t Lo, Ls, Lg;
+    double max_opp, max_same, max_sing;
+    const double *oh, *sh, *gh;
+    const uint16_t *ot, *st, *gt;
+    const int32_t *vpos, *spos;
+};
+
+template <class S>
+__device__ __forceinline__ S bit(int p) {
+    return ((S)1) << p;
+}
+
+// Walks one determinant's excitations in the reference's order; emit(up, dn)
+// receives every candidate.
+template <class S, class Emit>
+__device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, double local, bool singles, Emit emit) {
+    if (L.max_opp >= local) {
+        for (S a = vmask & up; a; a &= a - (S)1) {
+            const int i = ctz(a);
+            for (S b = vmask & dn; b; b &= b - (S)1) {
+                const int j = ctz(b);
+                const int64_t base = ((int64_t)L.vpos[i] * L.nv + L.vpos[j]) * L.Lo;
+                for (int64_t e = 0; e < L.Lo; ++e) {
+                    if (L.oh[base + e] < local) break;
+                    const uint16_t t = L.ot[base + e];
+                    const int r = t & 0xFF, s = t >> 8;
+                    if (((up >> r) & (S)1) || ((dn >> s) & (S)1)) continue;
+                    emit((up ^ bit<S>(i)) | bit<S>(r), (dn ^ bit<S>(j)) | bit<S>(s));
+                }
+            }
+        }
+    }
+    if (L.max_same >= local && L.Ls > 0) {
+        for (int spin = 0; spin < 2; ++spin) {
+            const S cfg = spin == 0 ? up : dn;
+            for (S a = vmask & cfg; a; a &= a - (S)1) {
+                const int i = ctz(a);
+                for (S b = a & (a - (S)1); b; b &= b - (S)1) {
+                    const int j = ctz(b);
+                    const int64_t base = (int64_t)L.spos[L.vpos[i] * L.nv + L.vpos[j]] * L.Ls;
+                    for (int64_t e = 0; e < L.Ls; ++e) {
+                        if (L.sh[base + e] < local) break;
+                        const uint16_t t = L.st[base + e];
+                        const int r = t & 0xFF, s = t >> 8;
+                        if (((cfg >> r) & (S)1) || ((cfg >> s) & (S)1)) continue;
+                        const S nw = (cfg ^ bit<S>(i) ^ bit<S>(j)) | bit<S>(r) | bit<S>(s);
+                        if (spin == 0)
+                            emit(nw, dn);
+                        else
+                            emit(up, nw);
+                    }
+                }
+            }
+        }
+    }
+    if (singles && L.max_sing >= local) {
+        for (int spin = 0; spin < 2; ++spin) {
+            const S cfg = spin == 0 ? up : dn, other = spin == 0 ? dn : up;
+            for (S a = vmask & cfg; a; a &= a - (S)1) {
+                const int i = ctz(a);
+                const int64_t base = (int64_t)L.vpos[i] * L.Lg;
+                for (int64_t e = 0; e < L.Lg; ++e) {
+                    if (L.gh[base + e] < local) break;
+                    const int r = L.gt[base + e] & 0xFF;
+                    if ((cfg >> r) & (S)1) continue;
+                    const S nw = (cfg ^ bit<S>(i)) | bit<S>(r);
+                    // ham_sing(D, new) (src/ham/mod.rs:135-168)
+                    if (fabs(elem_single<S>(T, cfg, nw, other)) > local) {
+                        if (spin == 0)
+                            emit(nw, dn);
+                        else
+                            emit(up, nw);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void store_string(uint64_t* w, S x);
+template <>
+__device__ __forceinline__ void store_string<uint64_t>(uint64_t* w, uint64_t x) {
+    w[0] = x;
+    w[1] = 0;
+}
+template <>
+__device__ __forceinline__ void store_string<u128>(uint64_t* w, u128 x) {
+    w[0] = (uint64_t)x;
+    w[1] = (uint64_t)(x >> 64);
+}
+
+template <class S, bool FILL>
+__global__ void k_select(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const uint64_t* __restrict__ dets,
+                         const double* __restrict__ coeffs, int64_t n, double eps, int singles, int64_t* counts,
+                         const int64_t* __restrict__ offs, uint64_t* out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const S up = load_string<S>(dets + 4 * k), dn = load_string<S>(dets + 4 * k + 2);
+    S vmask;
+    if constexpr (sizeof(S) > 8)
+        vmask = ((S)vm1 << 64) | (S)vm0;
+    else
+        vmask = (S)vm0;
+    const double c = fabs(coeffs[k]);
+    const double local = c > 0.0 ? eps / c : INFINITY;
+    int64_t cnt = 0;
+    uint64_t* o = FILL ? out + 4 * offs[k] : nullptr;
+    walk<S>(L, T, vmask, up, dn, local, singles != 0, [&](S nu, S nd) {
+        if (FILL) {
+            store_string<S>(o + 4 * cnt, nu);
+            store_string<S>(o + 4 * cnt + 2, nd);
+        }
+        ++cnt;
+    });
+    if (!FILL) counts[k] = cnt;
+}
+
+__global__ void k_iota64(int64_t* v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
+__global__ void k_word(const uint64_t* w, const int64_t* idx, int64_t n, int word, uint64_t* key) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] = w[4 * idx[i] + word];
+}
+
+// lexicographic on words 3, 2, 1, 0 (the order the LSD passes below produce)
+__device__ __forceinline__ int cmp_det(const uint64_t* a, const uint64_t* b) {
+    for (int w = 3; w >= 0; --w)
+        if (a[w] != b[w]) return a[w] < b[w] ? -1 : 1;
+    return 0;
+}
+
+// keep[i] = 1 iff sorted candidate i is the first of its run and absent from the
+// sorted current space
+__global__ void k_keep(const uint64_t* cand, const int64_t* cidx, int64_t m, const uint64_t* cur, const int64_t* didx,
+                       int64_t n, int64_t* keep) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t* a = cand + 4 * cidx[i];
+    int64_t k = 1;
+    if (i > 0 && cmp_det(a, cand + 4 * cidx[i - 1]) == 0) k = 0;
+    if (k) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (cmp_det(cur + 4 * didx[mid], a) < 0)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo < n && cmp_det(cur + 4 * didx[lo], a) == 0) k = 0;
+    }
+    keep[i] = k;
+}
+
+__global__ void k_compact(const int64_t* keep, const int64_t* pos, const int64_t* cidx, int64_t m, int64_t* rank) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m && keep[i]) rank[pos[i]] = cidx[i];
+}
+
+__global__ void k_gather_dets(const uint64_t* cand, const int64_t* rank, int64_t k, uint64_t* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    for (int w = 0; w < 4; ++w) out[4 * i + w] = cand[4 * rank[i] + w];
+}
+
+struct Tmp {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, need);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    ~Tmp() { hipFree(p); }
+};
+
+struct Bufs {
+    std::vector<void*> v;
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        hipError_t e = hipMalloc((void**)p, count * sizeof(T) + 16);
+        if (e == hipSuccess) v.push_back(*p);
+        return e;
+    }
+    ~Bufs() {
+        for (void* p : v) hipFree(p);
+    }
+};
+
+// indices 0..n-1 of the 4-word determinants w, stably sorted by words 0..3
+// (least significant first), into *idx (device, n entries)
+hipError_t sort_dets(Tmp& tmp, Bufs& b, const uint64_t* w, int64_t n, int n_words, int64_t* idx, hipStream_t s) {
+    int64_t *i2 = nullptr;
+    uint64_t *k1 = nullptr, *k2 = nullptr;
+    hipError_t e;
+    if ((e = b.alloc(&i2, (size_t)n)) != hipSuccess) return e;
+    if ((e = b.alloc(&k1, (size_t)n)) != hipSuccess) return e;
+    if ((e = b.alloc(&k2, (size_t)n)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_iota64, grid_for(n, 256), 256, 0, s, idx, n);
+    // least significant word first: the order of cmp_det (words 3, 2, 1, 0);
+    // the high words are zero when every orbital fits one word
+    for (int wd = 0; wd < 4; ++wd) {
+        if (n_words == 2 && (wd == 1 || wd == 3)) continue;
+        hipLaunchKernelGGL(k_word, grid_for(n, 256), 256, 0, s, w, idx, n, wd, k1);
+        size_t need = 0;
+        if ((e = rocprim::radix_sort_pairs(nullptr, need, k1, k2, idx, i2, (size_t)n, 0, 64, s)) != hipSuccess) return e;
+        if ((e = tmp.ensure(need)) != hipSuccess) return e;
+        if ((e = rocprim::radix_sort_pairs(tmp.p, need, k1, k2, idx, i2, (size_t)n, 0, 64, s)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(idx, i2, (size_t)n * 8, hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// host list construction ------------------------------------------------------
+
+struct Ent {
+    double a;
+    uint16_t t;
+};
+
+}  // namespace
+
+extern "C" {
+
+int hci_excite_init(hci_ctx* c, int n_core, int n_up, int n_dn, double eps_var, double* eps_start) {
+    if (!c) return HCI_ERR_ARG;
+    if (c->h_two.empty()) return hci_fail(c, HCI_ERR_STATE, "hci_excite_init: integrals not set");
+    const int n = c->n_orb;
+    if (n_core < 0 || n_up < 1 || n_dn < 1 || n_up > n || n_dn > n || n_core > std::min(n_up, n_dn))
+        return hci_fail(c, HCI_ERR_ARG, "hci_excite_init: bad electron or core counts");
+    HIP_TRY(c, hipSetDevice(c->device));
+    auto h1 = [&](int p, int q) { return c->h_one[sc2((size_t)p + 1, (size_t)q + 1)]; };
+    auto eri = [&](int p, int q, int r, int s) {
+        return c->h_two[sc2(sc2((size_t)p + 1, (size_t)q + 1), sc2((size_t)r + 1, (size_t)s + 1))];
+    };
+    SelectLists* L = new SelectLists();
+    select_free(c);
+    c->sel = L;
+    std::vector<int> order(n);
+    for (int p = 0; p < n; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h1(a, a) < h1(b, b); });
+    std::vector<int> val(order.begin() + n_core, order.end());
+    const int nv = (int)val.size();
+    std::vector<int32_t> vpos(n, -1), spos((size_t)nv * nv, -1);
+    for (int a = 0; a < nv; ++a) {
+        vpos[val[a]] = a;
+        if (val[a] < 64)
+            L->vmask[0] |= 1ull << val[a];
+        else
+            L->vmask[1] |= 1ull << (val[a] - 64);
+    }
+    L->n_core = n_core;
+    L->n_up = n_up;
+    L->n_dn = n_dn;
+    L->nv = nv;
+    L->Lo = (int64_t)(nv - 1) * (nv - 1);
+    L->Ls = nv >= 4 ? (int64_t)(nv - 2) * (nv - 3) / 2 : 0;
+    L->Lg = nv - 1;
+    // opposite-spin doubles: direct(p, q, r, s) = (pr|qs)
+    std::vector<double> oh((size_t)nv * nv * L->Lo), sh, gh((size_t)nv * L->Lg);
+    std::vector<uint16_t> ot(oh.size()), st, gt(gh.size());
+    std::vector<Ent> v;
+    for (int p : val)
+        for (int q : val) {
+            v.clear();
+            for (int r : val) {
+                if (r == p) continue;
+                for (int s : val) {
+                    if (s == q) continue;
+                    const double a = std::fabs(eri(p, r, q, s));
+                    L->max_opp = std::max(L->max_opp, a);
+                    v.push_back({a, (uint16_t)(r | (s << 8))});
+                }
+            }
+            std::stable_sort(v.begin(), v.end(), [](const Ent& x, const Ent& y) { return x.a > y.a; });
+            const size_t base = ((size_t)vpos[p] * nv + vpos[q]) * L->Lo;
+            for (size_t e = 0; e < v.size(); ++e) {
+                oh[base + e] = v[e].a;
+                ot[base + e] = v[e].t;
+            }
+        }
+    // same-spin doubles: direct_plus_exchange(p, q, r, s) = (pr|qs) - (ps|qr), p < q, r < s
+    int n_same = 0;
+    for (int p : val)
+        for (int q : val)
+            if (p < q) spos[(size_t)vpos[p] * nv + vpos[q]] = n_same++;
+    sh.assign((size_t)n_same * L->Ls, 0.0);
+    st.assign(sh.size(), 0);
+    for (int p : val)
+        for (int q : val) {
+            if (p >= q) continue;
+            v.clear();
+            for (int r : val) {
+                if (r == p || r == q) continue;
+                for (int s : val) {
+                    if (r >= s || s == p || s == q) continue;
+                    const double a = std::fabs(eri(p, r, q, s) - eri(p, s, q, r));
+                    L->max_same = std::max(L->max_same, a);
+                    v.push_back({a, (uint16_t)(r | (s << 8))});
+                }
+            }
+            std::stable_sort(v.begin(), v.end(), [](const Ent& x, const Ent& y) { return x.a > y.a; });
+            const size_t base = (size_t)spos[(size_t)vpos[p] * nv + vpos[q]] * L->Ls;
+            for (size_t e = 0; e < v.size(); ++e) {
+                sh[base + e] = v[e].a;
+                st[base + e] = v[e].t;
+            }
+        }
+    // singles: the bound of src/excite/init.rs:181-231
+    std::vector<double> vs, vo;
+    for (int p : val) {
+        v.clear();
+        for (int r : val) {
+            if (r == p) continue;
+            vs.clear();
+            vo.clear();
+            for (int q = 0; q < n; ++q) {
+                if (q == p || q == r) continue;
+                vs.push_back(eri(p, r, q, q) - eri(p, q, q, r));
+                vo.push_back(eri(p, r, q, q));
+            }
+            std::sort(vs.begin(), vs.end());
+            std::sort(vo.begin(), vo.end());
+            double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+            for (int i = 0; i < n_up - 1; ++i) s1 += vs[i];
+            for (int i = 0; i < n_dn; ++i) s2 += vo[i];
+            for (size_t i = vs.size() - (size_t)(n_up - 1); i < vs.size(); ++i) s3 += vs[i];
+            for (size_t i = vs.size() - (size_t)n_dn; i < vo.size(); ++i) s4 += vo[i];
+            const double lo = h1(p, r) + s1 + s2, hi = h1(p, r) + s3 + s4;
+            v.push_back({std::fabs(lo) > std::fabs(hi) ? std::fabs(lo) : std::fabs(hi), (uint16_t)r});
+        }
+        std::stable_sort(v.begin(), v.end(), [](const Ent& x, const Ent& y) { return x.a > y.a; });
+        const size_t base = (size_t)vpos[p] * L->Lg;
+        for (size_t e = 0; e < v.size(); ++e) {
+            gh[base + e] = v[e].a;
+            gt[base + e] = v[e].t;
+        }
+        if (!v.empty()) L->max_sing = std::max(L->max_sing, v[0].a);
+    }
+    // initial epsilon from Hartree-Fock (src/wf/eps.rs:44-171, z_sym = 1)
+    {
+        auto occ = [](int p, int ne) { return p < ne; };  // Hartree-Fock: the lowest orbitals by index
+        double max_sym = eps_var, max_asym = eps_var;
+        for (int i = 0; i < n_up; ++i) {
+            if (vpos[i] < 0) continue;
+            for (int j = 0; j < n_dn; ++j) {
+                if (vpos[j] < 0) continue;
+                bool fs = false, fa = false;
+                const size_t base = ((size_t)vpos[i] * nv + vpos[j]) * L->Lo;
+                for (int64_t e = 0; e < L->Lo; ++e) {
+                    const int r = ot[base + e] & 0xFF, s = ot[base + e] >> 8;
+                    if (occ(r, n_up) || occ(s, n_dn)) continue;
+                    const double a = oh[base + e];
+                    if (i == j && r == s) {
+                        fs = true;
+                        max_sym = std::max(max_sym, a);
+                    } else {
+                        fa = true;
+                        max_asym = std::max(max_asym, a);
+                    }
+                    if (fs && fa) break;
+                }
+            }
+        }
+        for (int spin = 0; spin < 2; ++spin) {
+            const int ne = spin == 0 ? n_up : n_dn;
+            for (int i = 0; i < ne; ++i) {
+                if (vpos[i] < 0) continue;
+                for (int j = i + 1; j < ne; ++j) {
+                    if (vpos[j] < 0 || L->Ls == 0) continue;
+                    const size_t base = (size_t)spos[(size_t)vpos[i] * nv + vpos[j]] * L->Ls;
+                    for (int64_t e = 0; e < L->Ls; ++e) {
+                        const int r = st[base + e] & 0xFF, s = st[base + e] >> 8;
+                        if (occ(r, ne) || occ(s, ne)) continue;
+                        max_asym = std::max(max_asym, sh[base + e]);
+                        break;
+                    }
+                }
+            }
+        }
+        L->eps_start = (max_sym > max_asym ? max_sym : max_asym) - 1e-9;
+    }
+    auto up = [&](auto** d, const auto& h) -> int {
+        using T = typename std::remove_reference<decltype(h)>::type::value_type;
+        HIP_TRY(c, hipMalloc((void**)d, h.size() * sizeof(T) + 16));
+        if (!h.empty()) HIP_TRY(c, hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return HCI_OK;
+    };
+    int rc;
+    if ((rc = up(&L->d_oh, oh)) || (rc = up(&L->d_ot, ot)) || (rc = up(&L->d_sh, sh)) || (rc = up(&L->d_st, st)) ||
+        (rc = up(&L->d_gh, gh)) || (rc = up(&L->d_gt, gt)) || (rc = up(&L->d_vpos, vpos)) ||
+        (rc = up(&L->d_spos, spos)))
+        return rc;
+    if (eps_start) *eps_start = L->eps_start;
+    return HCI_OK;
+}
+
+int hci_excite_info(const hci_ctx* c, double* max_opp, double* max_same, double* max_sing) {
+    if (!c || !c->sel) return HCI_ERR_STATE;
+    if (max_opp) *max_opp = c->sel->max_opp;
+    if (max_same) *max_same = c->sel->max_same;
+    if (max_sing) *max_sing = c->sel->max_sing;
+    return HCI_OK;
+}
+
+int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n, double eps, int singles,
+               uint64_t* out, int64_t cap, int64_t* count, int mem) {
+    if (!c) return HCI_ERR_ARG;
+    if (!c->sel) return hci_fail(c, HCI_ERR_STATE, "hci_select: call hci_excite_init first");
+    if (n < 1 || !dets || !coeffs || !count || cap < 0 || (cap > 0 && !out) || !(eps > 0.0))
+        return hci_fail(c, HCI_ERR_ARG, "hci_select: bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const SelectLists& L = *c->sel;
+    DevSel D{L.nv, L.Lo, L.Ls, L.Lg, L.max_opp, L.max_same, L.max_sing, L.d_oh, L.d_sh, L.d_gh,
+             L.d_ot, L.d_st, L.d_gt, L.d_vpos, L.d_spos};
+    const DevInts T = c->ints();
+    Bufs b;
+    Tmp tmp;
+    const uint64_t* d_dets = dets;
+    const double* d_coef = coeffs;
+    if (mem == HCI_MEM_HOST) {
+        uint64_t* w = nullptr;
+        double* q = nullptr;
+        HIP_TRY(c, b.alloc(&w, (size_t)n * 4));
+        HIP_TRY(c, b.alloc(&q, (size_t)n));
+        HIP_TRY(c, hipMemcpyAsync(w, dets, (size_t)n * 32, hipMemcpyHostToDevice, s));
+        HIP_TRY(c, hipMemcpyAsync(q, coeffs, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        d_dets = w;
+        d_coef = q;
+    }
+    const bool wide = c->n_orb > 64;
+    int64_t *cnt = nullptr, *off = nullptr;
+    HIP_TRY(c, b.alloc(&cnt, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&off, (size_t)n + 1));
+    if (wide)
+        hipLaunchKernelGGL((k_select<u128, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets,
+                           d_coef, n, eps, singles, cnt, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_select<uint64_t, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                           d_dets, d_coef, n, eps, singles, cnt, nullptr, nullptr);
+    HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
+    size_t need = 0;
+    HIP_TRY(c, rocprim::exclusive_scan(nullptr, need, cnt, off, (int64_t)0, (size_t)n + 1, rocprim::plus<int64_t>(), s));
+    HIP_TRY(c, tmp.ensure(need));
+    HIP_TRY(c, rocprim::exclusive_scan(tmp.p, need, cnt, off, (int64_t)0, (size_t)n + 1, rocprim::plus<int64_t>(), s));
+    int64_t m = 0;
+    HIP_TRY(c, hipMemcpyAsync(&m, off + n, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (m == 0) {
+        *count = 0;
+        return HCI_OK;
+    }
+    uint64_t* cand = nullptr;
+    HIP_TRY(c, b.alloc(&cand, (size_t)m * 4));
+    if (wide)
+        hipLaunchKernelGGL((k_select<u128, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets,
+                           d_coef, n, eps, singles, nullptr, off, cand);
+    else
+        hipLaunchKernelGGL((k_select<uint64_t, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                           d_dets, d_coef, n, eps, singles, nullptr, off, cand);
+    HIP_TRY(c, hipGetLastError());
+    // sort candidates and the current space; keep first occurrences not in the space
+    const int nw = wide ? 4 : 2;
+    int64_t *cidx = nullptr, *didx = nullptr, *keep = nullptr, *pos = nullptr, *rank = nullptr, *rank2 = nullptr;
+    HIP_TRY(c, b.alloc(&cidx, (size_t)m));
+    HIP_TRY(c, b.alloc(&didx, (size_t)n));
+    HIP_TRY(c, sort_dets(tmp, b, cand, m, nw, cidx, s));
+    HIP_TRY(c, sort_dets(tmp, b, d_dets, n, nw, didx, s));
+    HIP_TRY(c, b.alloc(&keep, (size_t)m + 1));
+    HIP_TRY(c, b.alloc(&pos, (size_t)m + 1));
+    hipLaunchKernelGGL(k_keep, grid_for(m, 256), 256, 0, s, cand, cidx, m, d_dets, didx, n, keep);
+    HIP_TRY(c, hipMemsetAsync(keep + m, 0, 8, s));
+    need = 0;
+    HIP_TRY(c, rocprim::exclusive_scan(nullptr, need, keep, pos, (int64_t)0, (size_t)m + 1, rocprim::plus<int64_t>(), s));
+    HIP_TRY(c, tmp.ensure(need));
+    HIP_TRY(c, rocprim::exclusive_scan(tmp.p, need, keep, pos, (int64_t)0, (size_t)m + 1, rocprim::plus<int64_t>(), s));
+    int64_t k = 0;
+    HIP_TRY(c, hipMemcpyAsync(&k, pos + m, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    *count = k;
+    if (k > cap) return hci_fail(c, HCI_ERR_ARG, "hci_select: output capacity too small (count holds the need)");
+    if (k == 0) return HCI_OK;
+    HIP_TRY(c, b.alloc(&rank, (size_t)k));
+    HIP_TRY(c, b.alloc(&rank2, (size_t)k));
+    hipLaunchKernelGGL(k_compact, grid_for(m, 256), 256, 0, s, keep, pos, cidx, m, rank);
+    // discovery order
+    need = 0;
+    HIP_TRY(c, rocprim::radix_sort_keys(nullptr, need, rank, rank2, (size_t)k, 0, 64, s));
+    HIP_TRY(c, tmp.ensure(need));
+    HIP_TRY(c, rocprim::radix_sort_keys(tmp.p, need, rank, rank2, (size_t)k, 0, 64, s));
+    uint64_t* res = nullptr;
+    HIP_TRY(c, b.alloc(&res, (size_t)k * 4));
+    hipLaunchKernelGGL(k_gather_dets, grid_for(k, 256), 256, 0, s, cand, rank2, k, res);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(out, res, (size_t)k * 32,
+                              mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return HCI_OK;
+}
+
+}  // extern "C"

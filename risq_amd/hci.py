@@ -118,6 +118,10 @@ SIGNATURES = {
     "hci_memcpy": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
     "hci_event_record": (C.c_int, [_P, C.c_int]),
     "hci_event_elapsed_ms": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "hci_excite_init": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]),
+    "hci_excite_info": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hci_select": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_int, _P, C.c_int64,
+                             C.POINTER(C.c_int64), C.c_int]),
 }
 
 _lib = None
@@ -234,6 +238,35 @@ class HciContext:
         self._check(self.lib.hci_elements(self.handle, _ptr(det1), _ptr(det2), m, _ptr(val), _ptr(deg),
                                           MEM_HOST))
         return val, deg
+
+    # -- heat-bath selection --------------------------------------------------------
+    def excite_init(self, n_core: int, n_up: int, n_dn: int, eps_var: float) -> float:
+        """Builds the sorted excitation lists; returns the schedule's first epsilon."""
+        e = C.c_double()
+        self._check(self.lib.hci_excite_init(self.handle, n_core, n_up, n_dn, eps_var, C.byref(e)))
+        return e.value
+
+    def excite_info(self) -> tuple[float, float, float]:
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.lib.hci_excite_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def select(self, dets: np.ndarray, coeffs: np.ndarray, eps: float, singles: bool) -> np.ndarray:
+        """New determinants (n_new, 4) uint64 in the reference's discovery order."""
+        dets = np.ascontiguousarray(dets, dtype=np.uint64)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        cnt = C.c_int64()
+        cap = max(1024, 4 * dets.shape[0])
+        while True:
+            out = np.empty((cap, 4), dtype=np.uint64)
+            rc = self.lib.hci_select(self.handle, _ptr(dets), _ptr(coeffs), dets.shape[0], eps, int(singles),
+                                     _ptr(out), cap, C.byref(cnt), MEM_HOST)
+            if rc == HCI_OK:
+                return out[:cnt.value]
+            if cnt.value > cap:
+                cap = cnt.value
+                continue
+            self._check(rc)
 
     # -- timing ------------------------------------------------------------------
     def record(self, slot: int) -> None:

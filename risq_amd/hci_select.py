@@ -175,6 +175,12 @@ def _words(dets) -> np.ndarray:
     return out
 
 
+def _pairs(words: np.ndarray):
+    """(n, 4) uint64 words -> [(up, dn)] Python ints."""
+    w = words.astype(object)
+    return [(int(r[0]) | (int(r[1]) << 64), int(r[2]) | (int(r[3]) << 64)) for r in w]
+
+
 def find_new_dets(dets, coeffs, index: dict, gen: ExciteGenerator, eps: float, singles: bool, elements):
     """src/wf/mod.rs:1467-1631 (iterate_excites without matmul): new determinants in
     the order the reference discovers them.  `elements(words1, words2)` gives the
@@ -247,14 +253,24 @@ def variational(solver, ints: Integrals, n_up: int, n_dn: int, eps_var: float, n
     """The reference's variational stage (src/var/mod.rs:61-134, src/wf/mod.rs:1687-1740)."""
     if z_sym != 1:
         raise NotImplementedError("only z_sym = 1 (the reference's default) is supported")
-    gen = ExciteGenerator(ints, n_core, n_up, n_dn)
     hf = ((1 << n_up) - 1, (1 << n_dn) - 1)
     dets = [hf]
     index = {hf: 0}
     coeffs = np.ones(1)
     diag = solver.ham_diag(_words(dets))
     energy = float(diag[0])
-    sched = init_eps(dets, gen, eps_var, z_sym)
+    if hasattr(solver, "select"):
+        # lists and selection in the library
+        sched = EpsSchedule(solver.select_init(n_core, n_up, n_dn, eps_var), eps_var)
+
+        def finder(eps, singles):
+            return [tuple(d) for d in _pairs(solver.select(_words(dets), coeffs, eps, singles))]
+    else:
+        gen = ExciteGenerator(ints, n_core, n_up, n_dn)
+        sched = init_eps(dets, gen, eps_var, z_sym)
+
+        def finder(eps, singles):
+            return find_new_dets(dets, coeffs, index, gen, eps, singles, solver.elements)
     history = []
     converged = False
     it = 0
@@ -263,7 +279,7 @@ def variational(solver, ints: Integrals, n_up: int, n_dn: int, eps_var: float, n
         if it > max_iters:
             break
         eps = sched.next()
-        new = find_new_dets(dets, coeffs, index, gen, eps, eps == eps_var, solver.elements)
+        new = finder(eps, eps == eps_var)
         n_old = len(dets)
         for d in new:  # push_and_spin_flipped (src/wf/mod.rs:99-116)
             for e in ((d[1], d[0]), d) if d[0] != d[1] else (d,):
@@ -286,14 +302,18 @@ def variational(solver, ints: Integrals, n_up: int, n_dn: int, eps_var: float, n
 
 
 class GpuSolver:
-    """Elements, diagonals, incremental H builds and Davidson on the GPU library
-    (reference semantics: FP64 values, Davidson tol 1e-4 / etol 1e-8, max 50
-    iterations, restart to the Ritz vector)."""
+    """Selection, elements, diagonals, incremental H builds and Davidson on the
+    GPU library (reference semantics: FP64 values, Davidson tol 1e-4 / etol
+    1e-8, max 50 iterations, restart to the Ritz vector).  With
+    library_select=False the selection runs in this module (find_new_dets)."""
 
-    def __init__(self, ctx, ints: Integrals):
+    def __init__(self, ctx, ints: Integrals, library_select: bool = True):
         self.ctx = ctx
         ctx.set_integrals(ints)
         self.h = None
+        if library_select:
+            self.select_init = lambda n_core, n_up, n_dn, eps_var: ctx.excite_init(n_core, n_up, n_dn, eps_var)
+            self.select = lambda words, coeffs, eps, singles: ctx.select(words, coeffs, eps, singles)
 
     def ham_diag(self, words):
         return self.ctx.ham_diag(words)
