@@ -1,12 +1,23 @@
 """Benchmark of the HCI variational hot path on one MI355X.
 
-Workload (one "step" = one pass of the hot path over one determinant list):
-  the n_orb=40, 6 alpha + 6 beta electron system of the synthetic molecular
-  recipe (h_pp = -1 + 2p/15, h_pq ~ N(0, 0.05^2), (pq|rs) = sum_K L^K_pq L^K_rs,
-  K = 80 Cholesky vectors, L ~ N(0, 0.08^2), seed 1), over a 2e6-determinant
-  space from the deterministic energy-truncated generator (risq_amd.dets);
-  a step is the sparse H build from HBM-resident determinants followed by
-  Davidson to E0 (lowest root, residual 1e-4 and energy change 1e-8).
+One "step" is one pass of the hot path over one determinant list: the sparse
+H build from HBM-resident determinants followed by Davidson to E0 (lowest
+root, residual 1e-4 and energy change 1e-8).  Workloads (--config):
+
+  1 (default)  n_orb=40, 6 alpha + 6 beta electrons, synthetic molecular
+               integrals (h_pp = -1 + 2p/15, h_pq ~ N(0, 0.05^2),
+               (pq|rs) = sum_K L^K_pq L^K_rs, K = 80, L ~ N(0, 0.08^2), seed 1)
+               over a 2e6-determinant space from the deterministic
+               energy-truncated generator (risq_amd.dets).
+  2            6x6 periodic Hubbard lattice, t = 1, U = 4, on-site disorder
+               U(-0.1, 0.1) (seed 2), 18 + 18 electrons in the site basis;
+               the space is the one the HCI selection (eps_var = 1e-3, library
+               heat-bath selection) holds after its first 10 iterations
+               (1.08e7 determinants), built before the timed region.
+  4            n_orb=80, 10 alpha + 10 beta electrons, the same molecular
+               recipe with K = 160 (seed 4), energy-truncated space of
+               --n-dets determinants (default 8e6, the largest whose FP32
+               hi+lo H fits one GPU's HBM next to the build's scratch).
 
 Timed path (value): tiled storage with FP32 hi+lo values, Davidson with FP32
 sigmas and FP64 refinement, restarts that keep the previous Ritz vector, and the
@@ -37,13 +48,45 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_inputs(n_dets: int, n_orb: int = 40, n_el: int = 6, n_chol: int = 80, seed: int = 1):
+def molecular_inputs(n_dets: int, n_orb: int, n_el: int, n_chol: int, seed: int):
     from risq_amd.dets import energy_truncated_space
     from risq_amd.fcidump import synthetic_molecular
     ints = synthetic_molecular(n_orb, n_chol, seed)
     eps = np.array([ints.h1(p, p) for p in range(n_orb)])
     dets = energy_truncated_space(eps, n_el, n_el, n_dets)
     return ints, dets
+
+
+def hubbard_inputs(ctx, eps_var: float = 1e-3, iterations: int = 10):
+    """The determinants the HCI loop holds after `iterations` iterations on the
+    6x6 Hubbard lattice (library selection, FP64 builds, reference-flow
+    Davidson between selections)."""
+    from risq_amd import hci_select as hs
+    from risq_amd.fcidump import hubbard_2d
+    ints = hubbard_2d(6, 6, 1.0, 4.0, 0.1, 2)
+    solver = hs.GpuSolver(ctx, ints)
+    r = hs.variational(solver, ints, 18, 18, eps_var, max_iters=iterations, log=log)
+    solver.close()
+    return ints, hs._words(r.dets)
+
+
+def make_inputs(config: int, n_dets: int | None, ctx):
+    """(integrals, (n, 4) uint64 determinant words, workload description)."""
+    if config == 1:
+        n = n_dets or 2_000_000
+        ints, dets = molecular_inputs(n, 40, 6, 80, 1)
+        return ints, dets, ("molecular n_orb=40, 6a6b, K=80, seed 1, energy-truncated space, "
+                            f"{dets.shape[0]} dets")
+    if config == 2:
+        ints, dets = hubbard_inputs(ctx)
+        return ints, dets, ("Hubbard 6x6 periodic, t=1, U=4, disorder 0.1 (seed 2), 18a18b, "
+                            f"HCI space after 10 iterations at eps_var=1e-3, {dets.shape[0]} dets")
+    if config == 4:
+        n = n_dets or 8_000_000
+        ints, dets = molecular_inputs(n, 80, 10, 160, 4)
+        return ints, dets, ("molecular n_orb=80, 10a10b, K=160, seed 4, energy-truncated space, "
+                            f"{dets.shape[0]} dets")
+    raise ValueError(f"unknown workload {config}")
 
 
 def spmv_bytes(nnz_u: int, n: int, value_bytes: int) -> int:
@@ -89,12 +132,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n-dets", type=int, default=2_000_000)
+    ap.add_argument("--config", type=int, choices=[1, 2, 4], default=1,
+                    help="workload: 1 molecular n_orb=40 2e6 dets, 2 Hubbard 6x6 1e7 dets, 4 molecular n_orb=80")
+    ap.add_argument("--n-dets", type=int, default=None,
+                    help="determinants of the energy-truncated space (workloads 1 and 4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the untimed comparison runs (FP64 reference path, other preconditioner)")
-    ap.add_argument("--precond", choices=["mlp", "dpr"], default="mlp",
-                    help="correction of the timed path (the other one is reported beside it)")
+    ap.add_argument("--precond", choices=["mlp", "dpr"], default=None,
+                    help="correction of the timed path (the other one is reported beside it); "
+                         "default mlp, dpr for the Hubbard workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has one MI355X; "
@@ -107,12 +154,14 @@ def main() -> int:
                               HciContext)
 
     t0 = time.perf_counter()
-    ints, dets = make_inputs(args.n_dets)
-    n = dets.shape[0]
-    log(f"inputs ready: n={n} ({time.perf_counter() - t0:.1f}s)")
     ctx = HciContext(0)
+    ints, dets, workload = make_inputs(args.config, args.n_dets, ctx)
+    n = dets.shape[0]
+    log(f"inputs ready: {workload} ({time.perf_counter() - t0:.1f}s)")
     ctx.set_integrals(ints)
     precond_train.load(ctx)
+    if args.precond is None:
+        args.precond = "dpr" if args.config == 2 else "mlp"
     precond = PRECOND_MLP if args.precond == "mlp" else PRECOND_DPR
     diag = ctx.ham_diag(dets)
     d_dets = DeviceBuffer.from_array(ctx, dets)
@@ -138,7 +187,7 @@ def main() -> int:
     for w in range(args.warmup):
         h, res, tb, tt = step()
         log(f"warmup {w}: build {tb:.3f}s total {tt:.3f}s E0={res.energy:.10f} iters={res.stats['iters']}")
-        h.free()
+        h.free(recycle=True)
     ctx.sync()
     times, builds, results = [], [], []
     t_start = time.perf_counter()
@@ -149,11 +198,11 @@ def main() -> int:
         results.append(res)
         log(f"step {k}: build {tb:.3f}s total {tt:.3f}s E0={res.energy:.10f} {res.stats}")
         if k + 1 < args.steps:
-            h.free()
+            h.free(recycle=True)
     ctx.sync()
     wall = time.perf_counter() - t_start
     nnz = h.nnz
-    h.free()
+    h.free(recycle=True)
     last = results[-1].stats
     n32, nfull = last["n_sigma_f32"], last["n_sigma_full"]
     # FP32 sigmas read col + hi (8 B/entry), refinement sigmas col + hi + lo (12 B)
@@ -171,9 +220,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 values + f64 accumulation, f64 refinement",
-        "data": "synthetic (seeded integrals; deterministic energy-truncated determinant space)",
-        "config": {"workload": "config-1 recipe: n_orb=40, 6a6b, K=80, seed 1, "
-                               f"{n} dets, H build + Davidson to E0 (tol 1e-4, etol 1e-8)",
+        "data": "synthetic (seeded integrals; deterministic determinant space)",
+        "config": {"workload": f"{workload}; H build + Davidson to E0 (tol 1e-4, etol 1e-8)",
                    "n_dets": n, "nnz_u": nnz, "storage": "tiled, fp32 hi+lo",
                    "davidson": f"mixed precision, restart keeps 2 vectors, correction={args.precond}",
                    "parallelism": "single GPU"},
@@ -194,7 +242,7 @@ def main() -> int:
         st = ref.stats
         r_ms = st["sigma_ms"] / max(1, st["n_sigma_full"])
         r_ach = spmv_bytes(h.nnz, n, 8) / (r_ms * 1e-3) / 1e9
-        h.free()
+        h.free(recycle=True)
         out["fp64_reference_path"] = {
             "s_to_E0": round(tr, 4), "build_s": round(tb, 4), "E0": ref.energy, "iters": st["iters"],
             "converged": bool(st["converged"]), "sigma_ms": r_ms, "sigma_gbps": r_ach,

@@ -180,6 +180,13 @@ int hci_csr_row(const hci_csr* h, int64_t row, int32_t* cols, double* vals, int6
 int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* val,
                      double* diag);
 void hci_csr_free(hci_csr* h);
+/* Frees h like hci_csr_free but keeps an HCI_STORE_TILED matrix's entry arrays
+ * in its context for the next build there (device memory freed and allocated
+ * again is cleared by the driver first: seconds at the 100-GB scale).  The next
+ * TILED build takes what fits and frees the rest; any other build, or
+ * hci_destroy, frees them.  No counterpart in the reference, whose sparse
+ * matrix is a Rust Vec freed on drop (src/var/sparse.rs:24-36). */
+int hci_csr_recycle(hci_csr* h);
 
 /* y[:,c] = H x[:,c] for ncol column-major vectors of length n
  * (matrix_vector_prod / matrix_matrix_prod, src/var/sparse.rs:64-88). */

@@ -554,7 +554,10 @@ int davidson_core(int64_t N, const double* d, spmv_fn spmv, const void* ctx, con
     std::vector<std::vector<double>> W;
     for (int c = 0; c < max_sub; ++c) V[c][c] = 1.0;  // identity(N, max_sub)
     if (init) std::copy(init, init + N, V[0].begin());
-    int dim_sub = init_dim;
+    // The reference takes basis.columns(0, init_dim) of an N x max_sub basis
+    // and panics when N = 1 without a start vector; the subspace is clamped
+    // to N here (a one-determinant space returns H_00 and c = [1]).
+    int dim_sub = init_dim < N ? init_dim : (int)N;
     auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
         double s = 0.0;
         for (int64_t i = 0; i < N; ++i) s += a[i] * b[i];

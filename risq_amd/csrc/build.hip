@@ -64,6 +64,11 @@ struct BuildView {
     const S* bkey;
     const int32_t* bkq;
     int n_beta;
+    // the same for alpha strings at beta-sorted positions, sorted within each
+    // beta group (built only when every double vanishes)
+    const S* akey;
+    const int32_t* akq;
+    int n_alpha;
 };
 
 __device__ __forceinline__ bool keep_single(double v, int rule) {
@@ -317,19 +322,30 @@ __global__ void __launch_bounds__(kBlock)
             pos = atomicAdd(&wcur[slice], 1);
         }
         if (FILL && keep)
-            put16(out, woff[slice] + pos, ((uint32_t)rl << 8) | (uint32_t)(cl % kWinCols), v);
+            put16(out, woff[slice] + pos, ((uint32_t)(rl + sb.rb) << 8) | (uint32_t)(cl % kWinCols), v);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
-    if (sb.kind == 3) {
-        // work item g = key * nr + row: the row's beta string minus its key-th
-        // electron; every determinant of the column group whose beta string
-        // contains that key has an equal or single-connected beta string
-        const int nbk = B.n_beta;
-        const int32_t G = sb.nr * nbk;
-        const int64_t seg_lo = (int64_t)B.ga_off[sb.g_col] * nbk, seg_hi = (int64_t)B.ga_off[sb.g_col + 1] * nbk;
-        const S ua_row = B.ua[sb.g_row], ua_col = B.ua[sb.g_col];
+    if (sb.kind >= 3) {
+        // kind 3: alpha-major rows against a single-connected alpha group;
+        // kind 4: alpha-major rows against their own alpha group (beta
+        // singles); kind 5: beta-major rows against their own beta group
+        // (alpha singles).  Work item g = key * nr + row: the row's string
+        // (beta for kinds 3 and 4, alpha for 5) minus its key-th electron;
+        // every determinant of the column group whose string contains that
+        // key has an equal or single-connected string
+        const bool bm = sb.kind == 5;
+        const int nk = bm ? B.n_alpha : B.n_beta;
+        const S* keys = bm ? B.akey : B.bkey;
+        const int32_t* kq = bm ? B.akq : B.bkq;
+        const int32_t* goff = bm ? B.gb_off : B.ga_off;
+        const S* str = bm ? B.sb_up : B.sa_dn;
+        const int32_t* cidx = bm ? B.sb_idx : B.sa_idx;
+        const int32_t G = sb.nr * nk;
+        const int64_t seg_lo = (int64_t)goff[sb.g_col] * nk, seg_hi = (int64_t)goff[sb.g_col + 1] * nk;
+        const S other_row = bm ? ub[sb.g_row] : B.ua[sb.g_row];
+        const S ua_col = bm ? other_row : B.ua[sb.g_col];
         for (int32_t gb = 0; gb < G; gb += 64) {
             const int32_t g = gb + lane;
             int32_t rl = 0, kk = 0, i = 0;
@@ -339,15 +355,15 @@ __global__ void __launch_bounds__(kBlock)
                 kk = g / sb.nr;
                 rl = g - kk * sb.nr;
                 const int32_t p = sb.r0 + rl;
-                i = B.sa_idx[p];
-                b = B.sa_dn[p];
+                i = cidx[p];
+                b = str[p];
                 S m = b;
                 for (int t = 0; t < kk; ++t) m &= m - (S)1;
                 key = b & ~(((S)1) << ctz(m));
                 int64_t lo = seg_lo, hi = seg_hi;
                 while (lo < hi) {
                     const int64_t mid = (lo + hi) >> 1;
-                    if (B.bkey[mid] < key)
+                    if (keys[mid] < key)
                         lo = mid + 1;
                     else
                         hi = mid;
@@ -355,26 +371,36 @@ __global__ void __launch_bounds__(kBlock)
                 cur = lo;
             }
             while (true) {
-                const bool pending = cur < seg_hi && B.bkey[cur] == key;
+                const bool pending = cur < seg_hi && keys[cur] == key;
                 if (!__ballot(pending)) break;
                 bool keep = false;
                 double v = 0.0;
                 int32_t cl = 0;
                 if (pending) {
-                    const int32_t q = B.bkq[cur];
+                    const int32_t q = kq[cur];
                     ++cur;
                     if (q >= sb.c0 && q < sb.c0 + sb.nc) {
                         cl = q - sb.c0;
-                        const int32_t j = B.sa_idx[q];
-                        S da = b, db = B.sa_dn[q], u1 = ua_row, u2 = ua_col;
+                        const int32_t j = cidx[q];
+                        S da = b, db = str[q];
                         const bool same = (da == db);
-                        // an equal beta string matches every key: count it once
-                        keep = same ? kk == 0 : dbl != 0;
-                        if (FILL && keep) {
-                            swap_if(j < i, u1, u2);
-                            swap_if(j < i, da, db);
-                            v = same ? single_or_zero(elem_single<S>(T, u1, u2, da), single_rule)
-                                     : elem_opp_double<S>(T, u1, u2, da, db);
+                        if (sb.kind == 3) {
+                            // an equal beta string matches every key: count it once
+                            keep = same ? kk == 0 : dbl != 0;
+                            if (FILL && keep) {
+                                S u1 = other_row, u2 = ua_col;
+                                swap_if(j < i, u1, u2);
+                                swap_if(j < i, da, db);
+                                v = same ? single_or_zero(elem_single<S>(T, u1, u2, da), single_rule)
+                                         : elem_opp_double<S>(T, u1, u2, da, db);
+                            }
+                        } else {
+                            // strings are unique within a group: equal only on the diagonal
+                            keep = !same;
+                            if (FILL && keep) {
+                                swap_if(j < i, da, db);
+                                v = single_or_zero(elem_single<S>(T, da, db, other_row), single_rule);
+                            }
                         }
                     }
                 }
@@ -396,12 +422,15 @@ __global__ void __launch_bounds__(kBlock)
             rl = f - cl * sb.nr;
             const int32_t p = sb.r0 + rl, q = sb.c0 + cl;
             if (sb.kind == 2) {
+                // same beta string: alpha doubles, or alpha singles when every
+                // double vanishes (kinds 1 and 3 are then not built)
                 const int32_t i = B.sb_idx[p], j = B.sb_idx[q];
                 S a = B.sb_up[p], b = B.sb_up[q];
-                keep = popc(a ^ b) == 4;
+                keep = popc(a ^ b) == (dbl ? 4 : 2);
                 if (FILL && keep) {
                     swap_if(j < i, a, b);
-                    v = elem_same_double<S>(T, a, b);
+                    v = dbl ? elem_same_double<S>(T, a, b)
+                            : single_or_zero(elem_single<S>(T, a, b, ub[sb.g_row]), single_rule);
                 }
             } else {
                 const int32_t i = B.sa_idx[p], j = B.sa_idx[q];
@@ -523,6 +552,34 @@ __global__ void k_task_remap(TileTask* t, int64_t n, const int32_t* nidx) {
     if (k >= n) return;
     t[k].w_begin = nidx[t[k].w_begin];
     t[k].w_end = nidx[t[k].w_end];
+}
+
+// Stable partition of each tile's windows into those of more than kSmallWin
+// entries, then the rest; sets w_split.  One wavefront per tile.
+constexpr int kPartWaves = 4;
+__global__ void __launch_bounds__(64 * kPartWaves)
+    k_win_partition(TileTask* tasks, int64_t n_task, const Window* __restrict__ in, Window* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t k = (int64_t)blockIdx.x * kPartWaves + (threadIdx.x >> 6);
+    if (k >= n_task) return;
+    const int32_t b = tasks[k].w_begin, e = tasks[k].w_end;
+    int32_t n_big = 0;
+    for (int32_t w = b; w < e; w += 64)
+        n_big += __popcll(__ballot(w + lane < e && in[w + lane].cnt() > kSmallWin));
+    const uint64_t below = (1ull << lane) - 1;
+    int32_t pb = b, ps = b + n_big;
+    for (int32_t w = b; w < e; w += 64) {
+        const bool valid = w + lane < e;
+        Window W{};
+        if (valid) W = in[w + lane];
+        const bool big = valid && W.cnt() > kSmallWin;
+        const uint64_t mb = __ballot(big), ms = __ballot(valid && !big);
+        if (big) out[pb + __popcll(mb & below)] = W;
+        if (valid && !big) out[ps + __popcll(ms & below)] = W;
+        pb += __popcll(mb);
+        ps += __popcll(ms);
+    }
+    if (lane == 0) tasks[k].w_split = b + n_big;
 }
 
 // ---- sorting / bucketing helpers ------------------------------------------------
@@ -869,6 +926,38 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     B.bkey = k_out;
     B.bkq = q_out;
     B.n_beta = nbk;
+    const int dbl = c->doubles_vanish ? 0 : 1;  // no same-beta alpha doubles to place otherwise
+    if (!dbl) {
+        // alpha-minus-one-electron keys, sorted within each beta group: same-beta
+        // alpha singles are then found inside beta groups (kind 5)
+        int nak = 0;
+        {
+            uint64_t w[2];
+            HIP_TRY(c, hipMemcpy(w, B.sb_up, sizeof(S) <= 8 ? 8 : 16, hipMemcpyDeviceToHost));
+            nak = __builtin_popcountll(w[0]) + (sizeof(S) > 8 ? __builtin_popcountll(w[1]) : 0);
+        }
+        const int64_t NA = N * (int64_t)std::max(nak, 1);
+        S *a_in = nullptr, *a_out = nullptr;
+        int32_t *aq_in = nullptr, *aq_out = nullptr, *aseg = nullptr;
+        HIP_TRY(c, kb.alloc(&a_in, (size_t)NA));
+        HIP_TRY(c, kb.alloc(&a_out, (size_t)NA));
+        HIP_TRY(c, kb.alloc(&aq_in, (size_t)NA));
+        HIP_TRY(c, kb.alloc(&aq_out, (size_t)NA));
+        HIP_TRY(c, kb.alloc(&aseg, (size_t)Ub + 1));
+        if (nak > 0) {
+            hipLaunchKernelGGL(k_beta_r1<S>, grid_for(N, 256), 256, 0, s, B.sb_up, N, nak, a_in, aq_in);
+            hipLaunchKernelGGL(k_scale_i32, grid_for(Ub + 1, 256), 256, 0, s, B.gb_off, (int64_t)Ub + 1, nak, aseg);
+            size_t need = 0;
+            HIP_TRY(c, rocprim::segmented_radix_sort_pairs(nullptr, need, a_in, a_out, aq_in, aq_out, (size_t)NA,
+                                                           (unsigned)Ub, aseg, aseg + 1, 0, c->n_orb, s));
+            HIP_TRY(c, tmp.ensure(need));
+            HIP_TRY(c, rocprim::segmented_radix_sort_pairs(tmp.p, need, a_in, a_out, aq_in, aq_out, (size_t)NA,
+                                                           (unsigned)Ub, aseg, aseg + 1, 0, c->n_orb, s));
+        }
+        B.akey = a_out;
+        B.akq = aq_out;
+        B.n_alpha = nak;
+    }
     std::vector<int32_t> ga((size_t)Ua + 1), gb((size_t)Ub + 1), no((size_t)Ua + 1), nb((size_t)n_pairs + 1);
     HIP_TRY(c, hipMemcpy(ga.data(), B.ga_off, ga.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(gb.data(), B.gb_off, gb.size() * 4, hipMemcpyDeviceToHost));
@@ -877,13 +966,14 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     std::vector<SubBlock> sbs;
     std::vector<TileTask> ta, tb;
     int64_t n_wall = 0;
-    const int dbl = c->doubles_vanish ? 0 : 1;  // no same-beta alpha doubles to place otherwise
     sbs.reserve((size_t)n_pairs + (size_t)Ua + (size_t)Ub + 16);
-    auto add_cols = [&](int32_t r0, int32_t nr, int32_t g_row, int32_t g_col, int kind, int32_t c0, int32_t c1) {
+    auto add_cols = [&](int32_t r0, int32_t nr, int32_t rb, int32_t g_row, int32_t g_col, int kind, int32_t c0,
+                        int32_t c1) {
         for (int32_t cs = c0; cs < c1; cs += kTileCols) {
             SubBlock b{};
             b.r0 = r0;
             b.nr = nr;
+            b.rb = rb;
             b.c0 = cs;
             b.nc = std::min(kTileCols, c1 - cs);
             b.g_row = g_row;
@@ -894,27 +984,88 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
             sbs.push_back(b);
         }
     };
+    // A group larger than a tile is split into tiles of its own; smaller
+    // consecutive groups share a tile of at most kTileRows rows and about
+    // kPackWindows windows (their windows are consecutive, and entries carry
+    // rows relative to the tile).
+    constexpr int64_t kPackWindows = 256;
+    auto windows_of = [](int32_t cols) { return (int64_t)(cols + kWinCols - 1) / kWinCols; };
+    auto tile_groups = [&](const std::vector<int32_t>& off, int32_t U, std::vector<TileTask>& tasks, auto&& n_windows,
+                           auto&& add_group) {
+        TileTask cur{0, 0, 0, 0, 0};
+        bool open = false;
+        int64_t cur_w = 0;
+        auto close = [&]() {
+            if (!open) return;
+            cur.w_end = (int32_t)n_wall;
+            tasks.push_back(cur);
+            open = false;
+        };
+        for (int32_t g = 0; g < U; ++g) {
+            const int32_t size = off[g + 1] - off[g];
+            if (size > kTileRows) {
+                close();
+                for (int32_t r0 = off[g]; r0 < off[g + 1]; r0 += kTileRows) {
+                    const int32_t nr = std::min(kTileRows, off[g + 1] - r0);
+                    TileTask t{r0, nr, (int32_t)n_wall, 0, 0};
+                    add_group(g, r0, nr, 0);
+                    t.w_end = (int32_t)n_wall;
+                    tasks.push_back(t);
+                }
+                continue;
+            }
+            const int64_t w = n_windows(g);
+            if (open && (cur.nr + size > kTileRows || cur_w + w > kPackWindows)) close();
+            if (!open) {
+                cur = TileTask{off[g], 0, (int32_t)n_wall, 0, 0};
+                open = true;
+                cur_w = 0;
+            }
+            add_group(g, off[g], size, cur.nr);
+            cur.nr += size;
+            cur_w += w;
+        }
+        close();
+    };
     std::vector<int32_t> cols;
-    for (int32_t g = 0; g < Ua; ++g) {
-        cols.assign(nb.begin() + no[g], nb.begin() + no[g + 1]);
-        cols.insert(std::lower_bound(cols.begin(), cols.end(), g), g);
-        for (int32_t r0 = ga[g]; r0 < ga[g + 1]; r0 += kTileRows) {
-            const int32_t nr = std::min(kTileRows, ga[g + 1] - r0);
-            TileTask t{r0, nr, (int32_t)n_wall, 0};
-            for (int32_t cg : cols)
-                add_cols(r0, nr, g, cg, cg == g ? 0 : (ga[cg + 1] - ga[cg] > kKeyedCols ? 3 : 1), ga[cg], ga[cg + 1]);
-            t.w_end = (int32_t)n_wall;
-            ta.push_back(t);
-        }
-    }
-    for (int32_t g = 0; g < Ub; ++g) {
-        for (int32_t r0 = gb[g]; r0 < gb[g + 1]; r0 += kTileRows) {
-            const int32_t nr = std::min(kTileRows, gb[g + 1] - r0);
-            TileTask t{r0, nr, (int32_t)n_wall, 0};
-            if (gb[g + 1] - gb[g] > 1 && dbl) add_cols(r0, nr, g, g, 2, gb[g], gb[g + 1]);
-            t.w_end = (int32_t)n_wall;
-            tb.push_back(t);
-        }
+    if (dbl) {
+        // alpha-major: the group itself (beta singles and doubles) and its
+        // single-connected groups (same-beta alpha singles, opposite-spin
+        // doubles); beta-major: same-beta alpha doubles
+        tile_groups(
+            ga, Ua, ta,
+            [&](int32_t g) {
+                int64_t w = windows_of(ga[g + 1] - ga[g]);
+                for (int32_t k = no[g]; k < no[g + 1]; ++k) w += windows_of(ga[nb[k] + 1] - ga[nb[k]]);
+                return w;
+            },
+            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
+                cols.assign(nb.begin() + no[g], nb.begin() + no[g + 1]);
+                cols.insert(std::lower_bound(cols.begin(), cols.end(), g), g);
+                for (int32_t cg : cols)
+                    add_cols(r0, nr, rb, g, cg, cg == g ? 0 : (ga[cg + 1] - ga[cg] > kKeyedCols ? 3 : 1), ga[cg],
+                             ga[cg + 1]);
+            });
+        tile_groups(
+            gb, Ub, tb, [&](int32_t g) { return gb[g + 1] - gb[g] > 1 ? windows_of(gb[g + 1] - gb[g]) : 0; },
+            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
+                if (gb[g + 1] - gb[g] > 1) add_cols(r0, nr, rb, g, g, 2, gb[g], gb[g + 1]);
+            });
+    } else {
+        // every double vanishes: beta singles inside alpha groups and alpha
+        // singles inside beta groups are all the pairs
+        tile_groups(
+            ga, Ua, ta, [&](int32_t g) { return ga[g + 1] - ga[g] > 1 ? windows_of(ga[g + 1] - ga[g]) : 0; },
+            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
+                const int32_t size = ga[g + 1] - ga[g];
+                if (size > 1) add_cols(r0, nr, rb, g, g, size > kKeyedCols ? 4 : 0, ga[g], ga[g + 1]);
+            });
+        tile_groups(
+            gb, Ub, tb, [&](int32_t g) { return gb[g + 1] - gb[g] > 1 ? windows_of(gb[g + 1] - gb[g]) : 0; },
+            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
+                const int32_t size = gb[g + 1] - gb[g];
+                if (size > 1) add_cols(r0, nr, rb, g, g, size > kKeyedCols ? 5 : 2, gb[g], gb[g + 1]);
+            });
     }
     const int64_t n_sb = (int64_t)sbs.size();
     if (n_wall >= (int64_t)INT32_MAX) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many tiles");
@@ -964,17 +1115,18 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     H->nnz_alloc = end;
     H->n_win = n_win;
     const size_t cap = (size_t)end + kWinAlign;
-    HIP_TRY(c, hipMalloc(&H->d_idx16, cap * 2));
-    TileOut fo{H->d_idx16, nullptr, nullptr, nullptr, d_woff};
+    H->entry_cap = cap;
+    // one 8-byte-per-entry value block for either precision (FP32 hi in its
+    // first half, lo in its second), so a recycled block serves both
     if (o.value_prec == HCI_PREC_F64) {
-        HIP_TRY(c, hipMalloc(&H->d_val, cap * 8));
-        fo.val = H->d_val;
+        HIP_TRY(c, recycled_take(c, (void**)&H->d_val, cap * 8));
     } else {
-        HIP_TRY(c, hipMalloc(&H->d_hi, cap * 4));
-        HIP_TRY(c, hipMalloc(&H->d_lo, cap * 4));
-        fo.hi = H->d_hi;
-        fo.lo = H->d_lo;
+        HIP_TRY(c, recycled_take(c, (void**)&H->d_hi, cap * 8));
+        H->d_lo = H->d_hi + cap;
     }
+    HIP_TRY(c, recycled_take(c, (void**)&H->d_idx16, cap * 2));
+    recycled_flush(c);
+    TileOut fo{H->d_idx16, H->d_val, H->d_hi, H->d_lo, d_woff};
     HIP_TRY(c, hipMalloc(&H->d_win, (size_t)n_win * sizeof(Window) + 16));
     if (n_sb > 0)
         hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
@@ -991,6 +1143,20 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         hipLaunchKernelGGL(k_task_remap, grid_for((int64_t)tb.size(), 256), 256, 0, s, H->d_task_b,
                            (int64_t)tb.size(), d_nidx);
     HIP_TRY(c, hipGetLastError());
+    {
+        // every window lies in the range of exactly one tile of one part
+        Window* parted = nullptr;
+        HIP_TRY(c, wb.alloc(&parted, (size_t)n_win + 1));
+        if (!ta.empty())
+            hipLaunchKernelGGL(k_win_partition, grid_for((int64_t)ta.size(), kPartWaves), 64 * kPartWaves, 0, s,
+                               H->d_task_a, (int64_t)ta.size(), H->d_win, parted);
+        if (!tb.empty())
+            hipLaunchKernelGGL(k_win_partition, grid_for((int64_t)tb.size(), kPartWaves), 64 * kPartWaves, 0, s,
+                               H->d_task_b, (int64_t)tb.size(), H->d_win, parted);
+        HIP_TRY(c, hipGetLastError());
+        if (n_win > 0)
+            HIP_TRY(c, hipMemcpyAsync(H->d_win, parted, (size_t)n_win * sizeof(Window), hipMemcpyDeviceToDevice, s));
+    }
     HIP_TRY(c, hipMalloc(&H->d_bpos, (size_t)N * 4 + 16));
     HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
     HIP_TRY(c, hipMalloc(&H->d_xb, (size_t)N * 8 + 16));
@@ -1176,6 +1342,9 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     B.bkey = nullptr;
     B.bkq = nullptr;
     B.n_beta = 0;
+    B.akey = nullptr;
+    B.akq = nullptr;
+    B.n_alpha = 0;
     const DevInts T = c->ints();
 
     if (o.storage == HCI_STORE_TILED) {
@@ -1350,6 +1519,7 @@ extern "C" int hci_build_h(hci_ctx* c, const uint64_t* dets, const double* diag,
     }
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(c, hipSetDevice(c->device));
+    if (opts->storage != HCI_STORE_TILED) recycled_flush(c);
     const uint64_t* d_dets = dets;
     const double* d_diag = diag;
     uint64_t* tmp_dets = nullptr;

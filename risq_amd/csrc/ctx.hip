@@ -50,6 +50,24 @@ bool words_fit(const uint64_t* w, int64_t n, int n_orb) {
 
 }  // namespace
 
+hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes) {
+    int best = -1;
+    for (int i = 0; i < (int)c->recycled.size(); ++i)
+        if (c->recycled[i].bytes >= bytes && (best < 0 || c->recycled[i].bytes < c->recycled[best].bytes)) best = i;
+    if (best >= 0) {
+        *p = c->recycled[best].p;
+        c->recycled.erase(c->recycled.begin() + best);
+        return hipSuccess;
+    }
+    recycled_flush(c);  // nothing fits: give the memory back before allocating
+    return hipMalloc(p, bytes);
+}
+
+void recycled_flush(hci_ctx* c) {
+    for (auto& r : c->recycled) hipFree(r.p);
+    c->recycled.clear();
+}
+
 extern "C" {
 
 int hci_create(int device, hci_ctx** out) {
@@ -71,6 +89,7 @@ int hci_create(int device, hci_ctx** out) {
 void hci_destroy(hci_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
+    recycled_flush(c);
     mlp_free_state(c);
     select_free(c);
     hipFree(c->d_two);
@@ -457,6 +476,25 @@ int hci_csr_row(const hci_csr* h, int64_t row, int32_t* cols, double* vals, int6
     return HCI_OK;
 }
 
+int hci_csr_recycle(hci_csr* h) {
+    if (!h) return HCI_ERR_ARG;
+    hci_ctx* c = h->ctx;
+    if (h->storage == HCI_STORE_TILED && h->entry_cap > 0) {
+        HIP_TRY(c, hipSetDevice(c->device));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        recycled_flush(c);
+        const size_t cap = h->entry_cap;
+        c->recycled.push_back({h->d_idx16, cap * 2});
+        c->recycled.push_back({h->d_val ? (void*)h->d_val : (void*)h->d_hi, cap * 8});
+        h->d_idx16 = nullptr;
+        h->d_val = nullptr;
+        h->d_hi = nullptr;
+        h->d_lo = nullptr;
+    }
+    hci_csr_free(h);
+    return HCI_OK;
+}
+
 void hci_csr_free(hci_csr* h) {
     if (!h) return;
     hipSetDevice(h->ctx->device);
@@ -465,7 +503,7 @@ void hci_csr_free(hci_csr* h) {
     hipFree(h->d_col);
     hipFree(h->d_val);
     hipFree(h->d_hi);
-    hipFree(h->d_lo);
+    if (h->storage != HCI_STORE_TILED) hipFree(h->d_lo);  // TILED: lo lives in hi's block
     hipFree(h->d_diag);
     hipFree(h->d_chunk_row);
     hipFree(h->d_item_row);

@@ -40,6 +40,7 @@ struct SubBlock {
     int32_t r0, nr, c0, nc;
     int32_t g_row, g_col;
     int32_t kind, win_begin;  // first of its ceil(nc / 256) windows
+    int32_t rb;               // r0 - the first row of its tile (tiles pack small groups)
 };
 struct Window {
     int64_t off;      // first slot (multiple of kWinAlign)
@@ -55,9 +56,12 @@ __host__ __device__ inline int64_t win_slot(int64_t j, int64_t cnt) {
     const int64_t t = j - full, R = (cnt - full + kWinAlign - 1) / kWinAlign;
     return full + (t % R) * kWinAlign + t / R;
 }
-// one workgroup: rows [r0, r0+nr) and windows [w_begin, w_end)
+// one workgroup: rows [r0, r0+nr) and windows [w_begin, w_end); windows of
+// more than kSmallWin entries come first and are taken one per wavefront,
+// those from w_split on are taken one per lane
+constexpr int kSmallWin = 32;
 struct TileTask {
-    int32_t r0, nr, w_begin, w_end;
+    int32_t r0, nr, w_begin, w_end, w_split;
 };
 
 struct hci_ctx {
@@ -77,6 +81,14 @@ struct hci_ctx {
     MlpState* mlp = nullptr;
     SelectLists* sel = nullptr;
     hipEvent_t events[16] = {};
+    // entry arrays of a recycled matrix, offered to the next TILED build
+    // (device memory handed back and allocated again is cleared by the driver
+    // before reuse: about 4 s for 200 GB)
+    struct Recycled {
+        void* p;
+        size_t bytes;
+    };
+    std::vector<Recycled> recycled;
 
     DevInts ints() const {
         DevInts t;
@@ -123,6 +135,7 @@ struct hci_csr {
     // per-row absolute sums (learned-preconditioner feature), lazily built
     double* d_rownorm = nullptr;
     // TILED storage
+    size_t entry_cap = 0;          // allocated length of d_idx16 / d_val / d_hi / d_lo
     uint16_t* d_idx16 = nullptr;   // nnz_alloc packed (local_row << 8 | local_col)
     Window* d_win = nullptr;       // non-empty windows, tile by tile
     int64_t n_win = 0;
@@ -134,6 +147,11 @@ struct hci_csr {
     double* d_xb = nullptr;        // work vectors in beta-major order
     double* d_yb = nullptr;
 };
+
+// recycled entry arrays (ctx.hip): recycled_take hands out a kept buffer of at
+// least `bytes` or allocates; recycled_flush frees what is left
+hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes);
+void recycled_flush(hci_ctx* c);
 
 // status helpers ------------------------------------------------------------
 int hci_fail(hci_ctx* ctx, int code, const std::string& msg);

@@ -261,14 +261,17 @@ __global__ void __launch_bounds__(kTileThreads)
                double* __restrict__ y) {
     __shared__ double acc[kTileRows];
     __shared__ double xs[kTileWaves][kWinCols];
-    __shared__ int next;
+    __shared__ int next, next_small;
     const TileTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63;
     double* xw = xs[tid >> 6];
     for (int r = tid; r < kTileRows; r += kTileThreads) acc[r] = 0.0;
-    if (tid == 0) next = 0;
+    if (tid == 0) {
+        next = 0;
+        next_small = 0;
+    }
     __syncthreads();
-    const int n_w = t.w_end - t.w_begin;
+    const int n_w = t.w_split - t.w_begin;
     auto grab = [&]() {
         int k = 0;
         if (lane == 0) k = atomicAdd(&next, 1);
@@ -338,6 +341,34 @@ __global__ void __launch_bounds__(kTileThreads)
         __builtin_amdgcn_wave_barrier();
         W = Wn;
         k = kn;
+    }
+    // windows of at most kSmallWin entries, one per lane: x is read where an
+    // entry needs it instead of staging the window's slice
+    const int n_s = t.w_end - t.w_split;
+    for (;;) {
+        int kb = 0;
+        if (lane == 0) kb = atomicAdd(&next_small, 64);
+        kb = __shfl(kb, 0);
+        if (kb >= n_s) break;
+        if (kb + lane < n_s) {
+            const Window Ws = wins[t.w_split + kb + lane];
+            const int m = Ws.cnt();
+            const int R = (m + kWinAlign - 1) / kWinAlign;
+            for (int g = 0; g < R; ++g) {
+                Oct<MODE> o;
+                o.load(idx, val, hi, lo, Ws.off + (int64_t)kWinAlign * g);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (g + q * R < m) {
+                        const uint32_t u = o.u[q];
+                        const double v = o.value(q);
+                        if (ABS)
+                            atomicAdd(&acc[u >> 8], fabs(v));
+                        else
+                            atomicAdd(&acc[u >> 8], v * x[Ws.c0 + (int)(u & 0xFFu)]);
+                    }
+            }
+        }
     }
     __syncthreads();
     for (int r = tid; r < t.nr; r += kTileThreads) {
@@ -420,8 +451,9 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
     if (h->storage == HCI_STORE_TILED) {
         // beta-major part first (x gathered to beta-major order), then the
         // alpha-major part writes y, then the beta-major rows are added
-        if (!absval) hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, s, h->d_bpos, x, h->n, h->d_xb);
         if (h->n_task_b > 0) {
+            if (!absval)
+                hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, s, h->d_bpos, x, h->n, h->d_xb);
             if (absval)
                 hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
                                    h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);

@@ -100,6 +100,7 @@ SIGNATURES = {
     "hci_csr_download": (C.c_int, [_P, _P, _P, _P, _P]),
     "hci_csr_row": (C.c_int, [_P, C.c_int64, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "hci_csr_free": (None, [_P]),
+    "hci_csr_recycle": (C.c_int, [_P]),
     "hci_sigma": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.c_int]),
     "hci_dav_default_opts": (None, [C.POINTER(DavOpts)]),
     "hci_davidson": (C.c_int, [_P, _P, _P, C.POINTER(DavOpts), C.POINTER(C.c_double), _P,
@@ -409,9 +410,14 @@ class SparseMatUpperTri:
         self.build_stats = {k: getattr(stats, k) for k, _ in stats._fields_}
         ctx._owned.add(self)
 
-    def free(self):
+    def free(self, recycle: bool = False):
+        """Release the matrix; with recycle=True a tiled matrix's entry arrays
+        stay with the context for its next tiled build (hci_csr_recycle)."""
         if self.handle and self.ctx.handle:
-            self.ctx.lib.hci_csr_free(self.handle)
+            if recycle:
+                self.ctx._check(self.ctx.lib.hci_csr_recycle(self.handle))
+            else:
+                self.ctx.lib.hci_csr_free(self.handle)
         self.handle = None
 
     def __del__(self):

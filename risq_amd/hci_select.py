@@ -291,7 +291,14 @@ def variational(solver, ints: Integrals, n_up: int, n_dn: int, eps_var: float, n
             break
         coeffs = np.concatenate([coeffs, np.zeros(len(dets) - n_old)])
         diag = np.concatenate([diag, solver.ham_diag(_words(dets[n_old:]))])
-        e, c, ok = solver.optimize(_words(dets), diag, coeffs if it > 1 else None)
+        # The reference warm-starts from the previous coefficients from the
+        # second iteration on (src/var/mod.rs:104).  When the previous space
+        # was the single start determinant (no double reaches eps, e.g. the
+        # site-basis Hubbard model), that vector is an exact eigenvector of
+        # the old space, its DPR correction is 0/0 and the reference's
+        # Davidson::new panics for N = 1; such a solve starts cold here.
+        warm = it > 1 and n_old > 1
+        e, c, ok = solver.optimize(_words(dets), diag, coeffs if warm else None)
         if ok:
             energy, coeffs = e, c
         history.append((eps, len(dets), energy))

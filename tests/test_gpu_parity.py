@@ -340,3 +340,54 @@ def test_tiled_rejects_incremental(gpu_ctx):
     h1 = gpu_ctx.build_h(dets[:100], diag[:100], storage=STORE_TILED)
     with pytest.raises(HciError, match="not incremental"):
         gpu_ctx.build_h(dets, diag, prev=h1, n_stored=100, storage=STORE_TILED)
+
+
+def test_recycled_entry_arrays(gpu_ctx, oracle_lib):
+    """A recycled tiled matrix's arrays serve the next tiled builds (FP32 after
+    FP32, FP64 after FP32, a smaller space after a larger one): downloads equal
+    and sigma products agree with those of builds on fresh memory."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 20000)
+    gpu_ctx.set_integrals(ints)
+    diag = gpu_ctx.ham_diag(dets)
+    x = np.random.default_rng(9).standard_normal(dets.shape[0])
+    fresh = {}
+    for prec, n in ((32, 20000), (64, 20000), (32, 12000)):
+        h = gpu_ctx.build_h(dets[:n], diag[:n], value_prec=prec, storage=STORE_TILED)
+        fresh[(prec, n)] = (h.download(), h.matrix_vector_prod(x[:n]))
+        h.free()
+    h = gpu_ctx.build_h(dets, diag, value_prec=32, storage=STORE_TILED)
+    for prec, n in ((32, 20000), (64, 20000), (32, 12000)):
+        h.free(recycle=True)
+        h = gpu_ctx.build_h(dets[:n], diag[:n], value_prec=prec, storage=STORE_TILED)
+        d, y = fresh[(prec, n)]
+        for u, v in zip(d, h.download()):
+            assert np.array_equal(u, v)
+        # products accumulate in LDS atomics, whose order varies from run to run
+        assert np.max(np.abs(y - h.matrix_vector_prod(x[:n]))) < 1e-12 * np.max(np.abs(y))
+    h.free(recycle=True)
+    # a full-storage build releases what a recycled matrix left behind
+    f = gpu_ctx.build_h(dets, diag)
+    assert f.nnz == fresh[(32, 20000)][0][1].size
+    f.free()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_tiled_hubbard_large_groups(gpu_ctx, oracle_lib, prec):
+    """Every double vanishes: the tiled build finds beta singles inside alpha
+    groups and alpha singles inside beta groups, by key lookup in groups wider
+    than 48 and in groups split over several tiles (> 256 rows) and column
+    blocks (> 4096); pattern, values and products equal full storage's."""
+    ints, dets = hubbard_space(5, 4, 6, 6, 300000)
+    _, ca = np.unique(dets[:, 0], return_counts=True)
+    assert ca.max() > 4096 and np.sum((ca > 1) & (ca <= 48)) > 0
+    gpu_ctx.set_integrals(ints)
+    diag = gpu_ctx.ham_diag(dets)
+    f = gpu_ctx.build_h(dets, diag, value_prec=prec)
+    t = gpu_ctx.build_h(dets, diag, value_prec=prec, storage=STORE_TILED)
+    assert t.nnz == f.nnz
+    for u, v in zip(f.download(), t.download()):
+        assert np.array_equal(u, v)
+    x = np.random.default_rng(10).standard_normal(dets.shape[0])
+    for mode in ([0, 1] if prec == 32 else [0]):
+        a, b = f.matrix_vector_prod(x, mode=mode), t.matrix_vector_prod(x, mode=mode)
+        assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
