@@ -112,3 +112,30 @@ def test_eps_schedule():
     assert seq[0] == 0.5 - 1e-9
     assert seq[1] == seq[0] * 0.1 and seq[2] == seq[1] * 0.1
     assert seq[3:] == [5e-4, 5e-4, 5e-4]
+
+
+def test_site_basis_hubbard_starts_from_one_determinant():
+    """Site-basis Hubbard lattice: no double reaches eps (they all vanish), so
+    the first iteration keeps the start determinant alone (where the reference's
+    Davidson::new would take two columns of a one-column basis) and the next
+    solve starts cold; the loop then converges to the lowest eigenvalue of its
+    final space."""
+    from risq_amd.fcidump import hubbard_2d
+    from helpers import csr_to_dense
+    ints = hubbard_2d(3, 3, 1.0, 4.0, 0.1, 2)
+    solver = OracleSolver(ints)
+    r = hs.variational(solver, ints, 4, 4, 2e-2)
+    assert r.history[0][1] == 1 and r.history[1][1] > 1
+    assert r.converged
+    H = pyoracle.OracleHam(ints)
+    S = pyoracle.OracleSparse()
+    w = hs._words(r.dets)
+    S.build(H, w, H.ham_diag(w))
+    e0 = np.linalg.eigvalsh(csr_to_dense(*S.export()))[0]
+    assert abs(r.energy - e0) < 1e-6
+    # the one-determinant solve itself: E = H_00 and c = [1]
+    one = pyoracle.OracleSparse()
+    w1 = w[:1]
+    one.build(H, w1, H.ham_diag(w1))
+    rc, e, c, _, _ = one.davidson()
+    assert rc == 0 and e == H.ham_diag(w1)[0] and c[0] == 1.0
