@@ -265,6 +265,20 @@ int hci_excite_info(const hci_ctx* ctx, double* max_opp, double* max_same, doubl
 int hci_select(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_t n, double eps, int singles,
                uint64_t* out, int64_t cap, int64_t* count, int mem);
 
+/* Deterministic second-order (Epstein-Nesbet) energy of the space (dets,
+ * coeffs) with variational energy e0 (dtm_pt, src/pt/mod.rs:81-122): every
+ * determinant a outside the space reached from some i by an excitation with
+ * |stored magnitude| * |c_i| >= eps (the lists of hci_excite_init; singles also
+ * need |H_ai c_i| >= eps, src/pt/mod.rs:97-102) contributes
+ * (sum_i H_ai c_i)^2 / (e0 - H_aa).  *e2 receives the sum and *n_ext (nullable)
+ * the number of such determinants.  H_aa is computed from the integrals (the
+ * reference updates the parent's diagonal, src/wf/det.rs:249-260; the two agree
+ * to rounding).  Source determinants are taken in batches of at most `batch`
+ * candidate excitations (0: 2^26, about 10 GB of scratch; a determinant with
+ * more is a batch of its own).  Needs hci_excite_init on the same integrals. */
+int hci_pt2_dtm(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_t n, double e0, double eps,
+                int64_t batch, double* e2, int64_t* n_ext, int mem);
+
 /* Device-memory and timing plumbing (no reference counterpart: for callers
  * that keep inputs resident in HBM, and for timing on the context's stream).
  * Events are 16 slots per context recorded on the context stream. */

@@ -24,6 +24,16 @@
 // words: the first occurrence of each new determinant survives, and the
 // survivors are returned in discovery order, which is the reference's order
 // of new determinants.
+//
+// Deterministic PT2 (dtm_pt, src/pt/mod.rs:81-122, candidates from
+// dets_excites_and_excited_dets, src/excite/iterator.rs:33-73): the same
+// lists walked with the reference's test |stored magnitude| * |c_i| >= eps
+// (singles then need |H_ai c_i| >= eps), targets inside the space dropped,
+// contributions H_ai c_i summed per external determinant in discovery order
+// (stable sort by determinant words, then one pass per run), and
+// e2 = sum_a (sum_i H_ai c_i)^2 / (e0 - H_aa).  Source determinants are
+// processed in batches whose candidates fit a fixed budget; the per-batch sums
+// of one external determinant are added in batch order.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -144,6 +154,79 @@ __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, dou
     }
 }
 
+// dtm_pt's candidates of one determinant with coefficient c, in the order of
+// valence_elecs_and_epairs (src/utils/bits.rs:76-85): opposite-spin pairs,
+// same-spin pairs (up, then dn), single electrons (up, then dn); each list
+// taken while |stored magnitude| * |c| >= eps (src/excite/iterator.rs:44-46).
+// emit(up', dn', H_ai * c) for valid targets (doubles always, singles when
+// |H_ai * c| >= eps, src/pt/mod.rs:97-102).
+template <class S, class Emit>
+__device__ void walk_pt(const DevSel& L, const DevInts& T, S vmask, S up, S dn, double c, double eps, Emit emit) {
+    const double ac = fabs(c);
+    if (L.max_opp * ac >= eps) {
+        for (S a = vmask & up; a; a &= a - (S)1) {
+            const int i = ctz(a);
+            for (S b = vmask & dn; b; b &= b - (S)1) {
+                const int j = ctz(b);
+                const int64_t base = ((int64_t)L.vpos[i] * L.nv + L.vpos[j]) * L.Lo;
+                for (int64_t e = 0; e < L.Lo; ++e) {
+                    if (!(L.oh[base + e] * ac >= eps)) break;
+                    const uint16_t t = L.ot[base + e];
+                    const int r = t & 0xFF, s = t >> 8;
+                    if (((up >> r) & (S)1) || ((dn >> s) & (S)1)) continue;
+                    const S nu = (up ^ bit<S>(i)) | bit<S>(r), nd = (dn ^ bit<S>(j)) | bit<S>(s);
+                    emit(nu, nd, elem_opp_double<S>(T, up, nu, dn, nd) * c);
+                }
+            }
+        }
+    }
+    if (L.max_same * ac >= eps && L.Ls > 0) {
+        for (int spin = 0; spin < 2; ++spin) {
+            const S cfg = spin == 0 ? up : dn;
+            for (S a = vmask & cfg; a; a &= a - (S)1) {
+                const int i = ctz(a);
+                for (S b = a & (a - (S)1); b; b &= b - (S)1) {
+                    const int j = ctz(b);
+                    const int64_t base = (int64_t)L.spos[L.vpos[i] * L.nv + L.vpos[j]] * L.Ls;
+                    for (int64_t e = 0; e < L.Ls; ++e) {
+                        if (!(L.sh[base + e] * ac >= eps)) break;
+                        const uint16_t t = L.st[base + e];
+                        const int r = t & 0xFF, s = t >> 8;
+                        if (((cfg >> r) & (S)1) || ((cfg >> s) & (S)1)) continue;
+                        const S nw = (cfg ^ bit<S>(i) ^ bit<S>(j)) | bit<S>(r) | bit<S>(s);
+                        const double h = elem_same_double<S>(T, cfg, nw) * c;
+                        if (spin == 0)
+                            emit(nw, dn, h);
+                        else
+                            emit(up, nw, h);
+                    }
+                }
+            }
+        }
+    }
+    if (L.max_sing * ac >= eps) {
+        for (int spin = 0; spin < 2; ++spin) {
+            const S cfg = spin == 0 ? up : dn, other = spin == 0 ? dn : up;
+            for (S a = vmask & cfg; a; a &= a - (S)1) {
+                const int i = ctz(a);
+                const int64_t base = (int64_t)L.vpos[i] * L.Lg;
+                for (int64_t e = 0; e < L.Lg; ++e) {
+                    if (!(L.gh[base + e] * ac >= eps)) break;
+                    const int r = L.gt[base + e] & 0xFF;
+                    if ((cfg >> r) & (S)1) continue;
+                    const S nw = (cfg ^ bit<S>(i)) | bit<S>(r);
+                    const double h = elem_single<S>(T, cfg, nw, other) * c;
+                    if (!(fabs(h) >= eps)) continue;
+                    if (spin == 0)
+                        emit(nw, dn, h);
+                    else
+                        emit(up, nw, h);
+                }
+            }
+        }
+    }
+}
+
 template <class S>
 __device__ __forceinline__ void store_string(uint64_t* w, S x);
 template <>
@@ -181,6 +264,32 @@ __global__ void k_select(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const 
         ++cnt;
     });
     if (!FILL) counts[k] = cnt;
+}
+
+template <class S, bool FILL>
+__global__ void k_pt_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const uint64_t* __restrict__ dets,
+                          const double* __restrict__ coeffs, int64_t k0, int64_t n, double eps, int64_t* counts,
+                          const int64_t* __restrict__ offs, uint64_t* words, double* vals) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t k = k0 + t;
+    const S up = load_string<S>(dets + 4 * k), dn = load_string<S>(dets + 4 * k + 2);
+    S vmask;
+    if constexpr (sizeof(S) > 8)
+        vmask = ((S)vm1 << 64) | (S)vm0;
+    else
+        vmask = (S)vm0;
+    int64_t cnt = 0;
+    const int64_t o = FILL ? offs[k] - offs[k0] : 0;
+    walk_pt<S>(L, T, vmask, up, dn, coeffs[k], eps, [&](S nu, S nd, double h) {
+        if (FILL) {
+            store_string<S>(words + 4 * (o + cnt), nu);
+            store_string<S>(words + 4 * (o + cnt) + 2, nd);
+            vals[o + cnt] = h;
+        }
+        ++cnt;
+    });
+    if (!FILL) counts[t] = cnt;
 }
 
 __global__ void k_iota64(int64_t* v, int64_t n) {
@@ -232,6 +341,71 @@ __global__ void k_gather_dets(const uint64_t* cand, const int64_t* rank, int64_t
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k) return;
     for (int w = 0; w < 4; ++w) out[4 * i + w] = cand[4 * rank[i] + w];
+}
+
+// head[i] = 1 iff sorted entry i starts a run of equal determinants
+__global__ void k_run_heads(const uint64_t* w, const int64_t* idx, int64_t m, int64_t* head) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) head[i] = (i == 0 || cmp_det(w + 4 * idx[i], w + 4 * idx[i - 1]) != 0) ? 1 : 0;
+}
+
+__global__ void k_run_starts(const int64_t* head, const int64_t* pos, int64_t m, int64_t* start) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m && head[i]) start[pos[i]] = i;
+    if (i == m) start[pos[m]] = m;
+}
+
+// One thread per run: the run's contributions summed in sorted (= discovery)
+// order; a determinant of the space (sorted by sidx) gets keep = 0.
+__global__ void k_run_sum(const uint64_t* w, const double* v, const int64_t* idx, const int64_t* start, int64_t runs,
+                          const uint64_t* space, const int64_t* sidx, int64_t n, uint64_t* out_w, double* out_v,
+                          int64_t* keep) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= runs) return;
+    const int64_t a = start[u], b = start[u + 1];
+    double sum = 0.0;
+    for (int64_t i = a; i < b; ++i) sum += v[idx[i]];
+    const uint64_t* d = w + 4 * idx[a];
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cmp_det(space + 4 * sidx[mid], d) < 0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    keep[u] = (lo < n && cmp_det(space + 4 * sidx[lo], d) == 0) ? 0 : 1;
+    for (int q = 0; q < 4; ++q) out_w[4 * u + q] = d[q];
+    out_v[u] = sum;
+}
+
+__global__ void k_pt_keep(const uint64_t* w, const double* v, const int64_t* keep, const int64_t* pos, int64_t runs,
+                          uint64_t* out_w, double* out_v) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= runs || !keep[u]) return;
+    const int64_t p = pos[u];
+    for (int q = 0; q < 4; ++q) out_w[4 * p + q] = w[4 * u + q];
+    out_v[p] = v[u];
+}
+
+// term_a = num_a^2 / (e0 - H_aa); fixed-order partial sums, one per block
+constexpr int kPtBlocks = 1024;
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_pt_terms(DevInts T, const uint64_t* w, const double* num, int64_t m, double e0, double* partial) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const S up = load_string<S>(w + 4 * i), dn = load_string<S>(w + 4 * i + 2);
+        acc += num[i] * num[i] / (e0 - elem_diag<S>(T, up, dn));
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
 struct Tmp {
@@ -571,6 +745,210 @@ int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n
     HIP_TRY(c, hipMemcpyAsync(out, res, (size_t)k * 32,
                               mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
+    return HCI_OK;
+}
+
+int hci_pt2_dtm(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n, double e0, double eps,
+                int64_t batch, double* e2, int64_t* n_ext, int mem) {
+    if (!c) return HCI_ERR_ARG;
+    if (!c->sel) return hci_fail(c, HCI_ERR_STATE, "hci_pt2_dtm: call hci_excite_init first");
+    if (n < 1 || !dets || !coeffs || !e2 || !(eps > 0.0) || batch < 0)
+        return hci_fail(c, HCI_ERR_ARG, "hci_pt2_dtm: bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const SelectLists& L = *c->sel;
+    DevSel D{L.nv, L.Lo, L.Ls, L.Lg, L.max_opp, L.max_same, L.max_sing, L.d_oh, L.d_sh, L.d_gh,
+             L.d_ot, L.d_st, L.d_gt, L.d_vpos, L.d_spos};
+    const DevInts T = c->ints();
+    const bool wide = c->n_orb > 64;
+    const int nw = wide ? 4 : 2;
+    Bufs b;
+    Tmp tmp;
+    const uint64_t* d_dets = dets;
+    const double* d_coef = coeffs;
+    if (mem == HCI_MEM_HOST) {
+        uint64_t* w = nullptr;
+        double* q = nullptr;
+        HIP_TRY(c, b.alloc(&w, (size_t)n * 4));
+        HIP_TRY(c, b.alloc(&q, (size_t)n));
+        HIP_TRY(c, hipMemcpyAsync(w, dets, (size_t)n * 32, hipMemcpyHostToDevice, s));
+        HIP_TRY(c, hipMemcpyAsync(q, coeffs, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        d_dets = w;
+        d_coef = q;
+    }
+    auto scan = [&](const int64_t* in, int64_t* out, int64_t len) -> hipError_t {
+        size_t need = 0;
+        hipError_t e = rocprim::exclusive_scan(nullptr, need, in, out, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
+        if (e != hipSuccess) return e;
+        if ((e = tmp.ensure(need)) != hipSuccess) return e;
+        return rocprim::exclusive_scan(tmp.p, need, in, out, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
+    };
+    // candidates per source determinant, and their offsets
+    int64_t *cnt = nullptr, *off = nullptr, *didx = nullptr;
+    HIP_TRY(c, b.alloc(&cnt, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&off, (size_t)n + 1));
+    if (wide)
+        hipLaunchKernelGGL((k_pt_walk<u128, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets,
+                           d_coef, (int64_t)0, n, eps, cnt, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_pt_walk<uint64_t, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                           d_dets, d_coef, (int64_t)0, n, eps, cnt, nullptr, nullptr, nullptr);
+    HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
+    HIP_TRY(c, scan(cnt, off, n + 1));
+    std::vector<int64_t> h_off((size_t)n + 1);
+    HIP_TRY(c, hipMemcpyAsync(h_off.data(), off, h_off.size() * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, b.alloc(&didx, (size_t)n));
+    HIP_TRY(c, sort_dets(tmp, b, d_dets, n, nw, didx, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    // external determinants with their summed contributions, batch by batch
+    const int64_t kBatch = batch > 0 ? batch : (int64_t)1 << 26;
+    uint64_t* g_w = nullptr;
+    double* g_v = nullptr;
+    int64_t g_n = 0, g_cap = 0;
+    int n_batches = 0;
+    int rc = HCI_OK;
+    auto release = [&]() {
+        hipFree(g_w);
+        hipFree(g_v);
+    };
+    for (int64_t b0 = 0; b0 < n && rc == HCI_OK;) {
+        int64_t b1 = b0 + 1;
+        while (b1 < n && h_off[b1 + 1] - h_off[b0] <= kBatch) ++b1;
+        const int64_t m = h_off[b1] - h_off[b0];
+        const int64_t k0 = b0;
+        b0 = b1;
+        if (m == 0) continue;
+        ++n_batches;
+        Bufs bb;
+        uint64_t *cw = nullptr, *rw = nullptr;
+        double *cv = nullptr, *rv = nullptr;
+        int64_t *cidx = nullptr, *head = nullptr, *hpos = nullptr, *start = nullptr, *keep = nullptr, *kpos = nullptr;
+        auto step = [&]() -> hipError_t {
+            hipError_t e;
+            if ((e = bb.alloc(&cw, (size_t)m * 4)) != hipSuccess) return e;
+            if ((e = bb.alloc(&cv, (size_t)m)) != hipSuccess) return e;
+            if (wide)
+                hipLaunchKernelGGL((k_pt_walk<u128, true>), grid_for(b1 - k0, 256), 256, 0, s, D, T, L.vmask[0],
+                                   L.vmask[1], d_dets, d_coef, k0, b1 - k0, eps, nullptr, off, cw, cv);
+            else
+                hipLaunchKernelGGL((k_pt_walk<uint64_t, true>), grid_for(b1 - k0, 256), 256, 0, s, D, T, L.vmask[0],
+                                   L.vmask[1], d_dets, d_coef, k0, b1 - k0, eps, nullptr, off, cw, cv);
+            if ((e = bb.alloc(&cidx, (size_t)m)) != hipSuccess) return e;
+            if ((e = sort_dets(tmp, bb, cw, m, nw, cidx, s)) != hipSuccess) return e;
+            if ((e = bb.alloc(&head, (size_t)m + 1)) != hipSuccess) return e;
+            if ((e = bb.alloc(&hpos, (size_t)m + 1)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_run_heads, grid_for(m, 256), 256, 0, s, cw, cidx, m, head);
+            if ((e = hipMemsetAsync(head + m, 0, 8, s)) != hipSuccess) return e;
+            if ((e = scan(head, hpos, m + 1)) != hipSuccess) return e;
+            int64_t runs = 0;
+            if ((e = hipMemcpyAsync(&runs, hpos + m, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            if ((e = bb.alloc(&start, (size_t)runs + 1)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_run_starts, grid_for(m + 1, 256), 256, 0, s, head, hpos, m, start);
+            if ((e = bb.alloc(&rw, (size_t)runs * 4)) != hipSuccess) return e;
+            if ((e = bb.alloc(&rv, (size_t)runs)) != hipSuccess) return e;
+            if ((e = bb.alloc(&keep, (size_t)runs + 1)) != hipSuccess) return e;
+            if ((e = bb.alloc(&kpos, (size_t)runs + 1)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_run_sum, grid_for(runs, 256), 256, 0, s, cw, cv, cidx, start, runs, d_dets, didx, n, rw,
+                               rv, keep);
+            if ((e = hipMemsetAsync(keep + runs, 0, 8, s)) != hipSuccess) return e;
+            if ((e = scan(keep, kpos, runs + 1)) != hipSuccess) return e;
+            int64_t kept = 0;
+            if ((e = hipMemcpyAsync(&kept, kpos + runs, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+            if (g_n + kept > g_cap) {
+                const int64_t cap = std::max<int64_t>(2 * (g_n + kept), 1 << 20);
+                uint64_t* nw_ = nullptr;
+                double* nv_ = nullptr;
+                if ((e = hipMalloc(&nw_, (size_t)cap * 32)) != hipSuccess) return e;
+                if ((e = hipMalloc(&nv_, (size_t)cap * 8)) != hipSuccess) {
+                    hipFree(nw_);
+                    return e;
+                }
+                if (g_n > 0) {
+                    hipMemcpyAsync(nw_, g_w, (size_t)g_n * 32, hipMemcpyDeviceToDevice, s);
+                    hipMemcpyAsync(nv_, g_v, (size_t)g_n * 8, hipMemcpyDeviceToDevice, s);
+                }
+                if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+                release();
+                g_w = nw_;
+                g_v = nv_;
+                g_cap = cap;
+            }
+            hipLaunchKernelGGL(k_pt_keep, grid_for(runs, 256), 256, 0, s, rw, rv, keep, kpos, runs, g_w + 4 * g_n,
+                               g_v + g_n);
+            g_n += kept;
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            return hipStreamSynchronize(s);
+        };
+        hipError_t e = step();
+        if (e != hipSuccess) rc = hci_fail(c, HCI_ERR_HIP, std::string("hci_pt2_dtm: ") + hipGetErrorString(e));
+    }
+    if (rc != HCI_OK) {
+        release();
+        return rc;
+    }
+    // one external determinant may come from several batches: merge in batch order
+    uint64_t* uw = g_w;
+    double* uv = g_v;
+    int64_t un = g_n;
+    if (n_batches > 1 && g_n > 0) {
+        int64_t *gidx = nullptr, *head = nullptr, *hpos = nullptr, *start = nullptr, *keep = nullptr;
+        uint64_t* rw = nullptr;
+        double* rv = nullptr;
+        int64_t runs = 0;
+        hipError_t e = b.alloc(&gidx, (size_t)g_n);
+        if (e == hipSuccess) e = sort_dets(tmp, b, g_w, g_n, nw, gidx, s);
+        if (e == hipSuccess) e = b.alloc(&head, (size_t)g_n + 1);
+        if (e == hipSuccess) e = b.alloc(&hpos, (size_t)g_n + 1);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_run_heads, grid_for(g_n, 256), 256, 0, s, g_w, gidx, g_n, head);
+            e = hipMemsetAsync(head + g_n, 0, 8, s);
+        }
+        if (e == hipSuccess) e = scan(head, hpos, g_n + 1);
+        if (e == hipSuccess) e = hipMemcpyAsync(&runs, hpos + g_n, 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = b.alloc(&start, (size_t)runs + 1);
+        if (e == hipSuccess) e = b.alloc(&rw, (size_t)runs * 4);
+        if (e == hipSuccess) e = b.alloc(&rv, (size_t)runs);
+        if (e == hipSuccess) e = b.alloc(&keep, (size_t)runs);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_run_starts, grid_for(g_n + 1, 256), 256, 0, s, head, hpos, g_n, start);
+            hipLaunchKernelGGL(k_run_sum, grid_for(runs, 256), 256, 0, s, g_w, g_v, gidx, start, runs, d_dets, didx,
+                               (int64_t)0, rw, rv, keep);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) {
+            release();
+            return hci_fail(c, HCI_ERR_HIP, std::string("hci_pt2_dtm: ") + hipGetErrorString(e));
+        }
+        uw = rw;
+        uv = rv;
+        un = runs;
+    }
+    double sum = 0.0;
+    if (un > 0) {
+        double* part = nullptr;
+        hipError_t e = b.alloc(&part, kPtBlocks);
+        if (e == hipSuccess) {
+            if (wide)
+                hipLaunchKernelGGL(k_pt_terms<u128>, kPtBlocks, 256, 0, s, T, uw, uv, un, e0, part);
+            else
+                hipLaunchKernelGGL(k_pt_terms<uint64_t>, kPtBlocks, 256, 0, s, T, uw, uv, un, e0, part);
+            e = hipGetLastError();
+        }
+        std::vector<double> hp(kPtBlocks);
+        if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, kPtBlocks * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            release();
+            return hci_fail(c, HCI_ERR_HIP, std::string("hci_pt2_dtm: ") + hipGetErrorString(e));
+        }
+        for (double v : hp) sum += v;
+    }
+    release();
+    *e2 = sum;
+    if (n_ext) *n_ext = un;
     return HCI_OK;
 }
 

@@ -123,6 +123,8 @@ SIGNATURES = {
     "hci_excite_info": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hci_select": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_int, _P, C.c_int64,
                              C.POINTER(C.c_int64), C.c_int]),
+    "hci_pt2_dtm": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_double, C.c_int64, C.POINTER(C.c_double),
+                              C.POINTER(C.c_int64), C.c_int]),
 }
 
 _lib = None
@@ -268,6 +270,20 @@ class HciContext:
                 cap = cnt.value
                 continue
             self._check(rc)
+
+    def pt2_dtm(self, dets: np.ndarray, coeffs: np.ndarray, e0: float, eps: float,
+                batch: int = 0) -> tuple[float, int]:
+        """Deterministic PT2 energy of the space and the number of external
+        determinants that contribute (dtm_pt, src/pt/mod.rs:81-122); needs
+        excite_init on the same integrals.  batch: candidate excitations per
+        pass (0: the library's default)."""
+        dets = np.ascontiguousarray(dets, dtype=np.uint64)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        e2 = C.c_double()
+        n_ext = C.c_int64()
+        self._check(self.lib.hci_pt2_dtm(self.handle, _ptr(dets), _ptr(coeffs), dets.shape[0], e0, eps, batch,
+                                         C.byref(e2), C.byref(n_ext), MEM_HOST))
+        return e2.value, n_ext.value
 
     # -- timing ------------------------------------------------------------------
     def record(self, slot: int) -> None:

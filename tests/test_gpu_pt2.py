@@ -1,0 +1,96 @@
+"""Deterministic PT2 on the GPU (hci_pt2_dtm, through the C ABI) against the
+enumeration oracle (oracle/pt2_oracle.py, itself pinned by dense
+Epstein-Nesbet sums in tests/test_pt2.py).  Bar: the same number of external
+determinants and e2 within 1e-11 relative (the numerators are summed in a
+different order: per batch, then batch by batch; the oracle per source
+determinant)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import mol_space
+from oracle import pt2_oracle
+from risq_amd import hci_select as hs
+from risq_amd.fcidump import read_fcidump
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CASES = json.load(open(os.path.join(GOLD, "hci_cases.json")))
+
+
+def _solve(ctx, ints, dets):
+    ctx.set_integrals(ints)
+    h = ctx.build_h(dets, ctx.ham_diag(dets))
+    r = ctx.davidson(h, tol=1e-8, etol=1e-12, max_iters=500)
+    h.free()
+    return r.energy, r.coeffs
+
+
+@pytest.mark.parametrize("eps", [1e-5, 1e-7])
+@pytest.mark.parametrize("n_core", [0, 1])
+def test_pt2_matches_oracle(gpu_ctx, eps, n_core):
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    if n_core:  # orbital 0 is the core: keep it doubly occupied
+        dets = dets[(dets[:, 0] & dets[:, 2] & np.uint64(1)) == 1][:2000]
+    else:
+        dets = dets[:2000]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    gpu_ctx.excite_init(n_core, 4, 4, 1e-3)
+    e2, n_ext = gpu_ctx.pt2_dtm(dets, c, e0, eps)
+    o2, on = pt2_oracle.dtm_pt(ints, dets, c, e0, eps, n_core=n_core)
+    print(f"eps={eps:g} n_core={n_core}: e2={e2:.12f} (oracle {o2:.12f}) external={n_ext}")
+    assert n_ext == on
+    assert abs(e2 - o2) < 1e-11 * abs(o2)
+
+
+def test_pt2_batches_agree(gpu_ctx):
+    """Small batches (several passes, merged in batch order) give the one-pass
+    result."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    e0, c = _solve(gpu_ctx, ints, dets)
+    gpu_ctx.excite_init(0, 4, 4, 1e-3)
+    e2, n = gpu_ctx.pt2_dtm(dets, c, e0, 1e-6)
+    for batch in (200000, 5000, 1):
+        e2b, nb = gpu_ctx.pt2_dtm(dets, c, e0, 1e-6, batch=batch)
+        assert nb == n
+        assert abs(e2b - e2) < 1e-13 * abs(e2)
+
+
+def test_pt2_two_word_strings(gpu_ctx):
+    """n_orb > 64: two-word strings through the same kernels."""
+    ints, dets = mol_space(70, 8, 5, 3, 3, 1500)
+    dets = dets[:120]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    gpu_ctx.excite_init(0, 3, 3, 1e-3)
+    e2, n_ext = gpu_ctx.pt2_dtm(dets, c, e0, 1e-5)
+    o2, on = pt2_oracle.dtm_pt(ints, dets, c, e0, 1e-5)
+    assert n_ext == on and n_ext > 0
+    assert abs(e2 - o2) < 1e-11 * abs(o2)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("name", ["be", "li2", "be2"])
+def test_pt2_after_hci_on_reference_examples(gpu_ctx, name):
+    """The reference examples' variational wavefunctions (HCI loop on the GPU)
+    and their deterministic PT2 at eps 1e-6 (the examples' eps_pt_dtm), with
+    the examples' frozen core and with every orbital active.  With the core
+    frozen these small spaces already hold every external determinant that
+    symmetry lets H reach (e2 = 0 from both sides); with it active they do
+    not."""
+    cs = CASES[name]
+    ints = read_fcidump(os.path.join(GOLD, cs["fcidump"]), cs["norb"])
+    solver = hs.GpuSolver(gpu_ctx, ints)
+    r = hs.variational(solver, ints, cs["nup"], cs["ndn"], cs["eps_var"], n_core=cs["norb_core"])
+    solver.close()
+    words = hs._words(r.dets)
+    for n_core in (cs["norb_core"], 0):
+        gpu_ctx.excite_init(n_core, cs["nup"], cs["ndn"], cs["eps_var"])
+        e2, n_ext = gpu_ctx.pt2_dtm(words, r.coeffs, r.energy, 1e-6)
+        o2, on = pt2_oracle.dtm_pt(ints, words, r.coeffs, r.energy, 1e-6, n_core=n_core)
+        print(f"{name} core={n_core}: E_var={r.energy:.10f} e2={e2:.12f} (oracle {o2:.12f}) external={n_ext}")
+        assert n_ext == on
+        assert abs(e2 - o2) <= 1e-11 * abs(o2)
+    assert n_ext > 0 and e2 < 0
