@@ -273,9 +273,11 @@ int hci_select(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_t
  * (sum_i H_ai c_i)^2 / (e0 - H_aa).  *e2 receives the sum and *n_ext (nullable)
  * the number of such determinants.  H_aa is computed from the integrals (the
  * reference updates the parent's diagonal, src/wf/det.rs:249-260; the two agree
- * to rounding).  Source determinants are taken in batches of at most `batch`
- * candidate excitations (0: 2^26, about 10 GB of scratch; a determinant with
- * more is a batch of its own).  Needs hci_excite_init on the same integrals. */
+ * to rounding).  When the candidate excitations outnumber `batch` (0: 2^28,
+ * about 27 GB of scratch), the external determinants are split into
+ * ceil(candidates / batch) parts by a hash of their strings (at most 2^20), and
+ * each part walks every source determinant again, keeping its own.  Needs
+ * hci_excite_init on the same integrals. */
 int hci_pt2_dtm(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_t n, double e0, double eps,
                 int64_t batch, double* e2, int64_t* n_ext, int mem);
 

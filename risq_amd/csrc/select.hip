@@ -31,9 +31,10 @@
 // (singles then need |H_ai c_i| >= eps), targets inside the space dropped,
 // contributions H_ai c_i summed per external determinant in discovery order
 // (stable sort by determinant words, then one pass per run), and
-// e2 = sum_a (sum_i H_ai c_i)^2 / (e0 - H_aa).  Source determinants are
-// processed in batches whose candidates fit a fixed budget; the per-batch sums
-// of one external determinant are added in batch order.
+// e2 = sum_a (sum_i H_ai c_i)^2 / (e0 - H_aa).  When the candidates exceed a
+// budget, external determinants are split into parts by a hash of their words
+// and every part walks all source determinants keeping only its own, so each
+// external determinant is summed whole, in discovery order, within one part.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -266,13 +267,23 @@ __global__ void k_select(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const 
     if (!FILL) counts[k] = cnt;
 }
 
+// part of an external determinant: a 64-bit mix of its strings, mod n_parts
+template <class S>
+__device__ __forceinline__ uint32_t part_of(S up, S dn, uint32_t n_parts) {
+    uint64_t h = (uint64_t)up * 0x9E3779B97F4A7C15ull ^ (uint64_t)dn;
+    if constexpr (sizeof(S) > 8) h ^= (uint64_t)(up >> 64) * 0xC2B2AE3D27D4EB4Full ^ (uint64_t)(dn >> 64) * 0x165667B19E3779F9ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    return (uint32_t)(h % n_parts);
+}
+
 template <class S, bool FILL>
 __global__ void k_pt_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const uint64_t* __restrict__ dets,
-                          const double* __restrict__ coeffs, int64_t k0, int64_t n, double eps, int64_t* counts,
-                          const int64_t* __restrict__ offs, uint64_t* words, double* vals) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int64_t k = k0 + t;
+                          const double* __restrict__ coeffs, int64_t n, double eps, uint32_t n_parts, uint32_t part,
+                          int64_t* counts, const int64_t* __restrict__ offs, uint64_t* words, double* vals) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
     const S up = load_string<S>(dets + 4 * k), dn = load_string<S>(dets + 4 * k + 2);
     S vmask;
     if constexpr (sizeof(S) > 8)
@@ -280,8 +291,9 @@ __global__ void k_pt_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const
     else
         vmask = (S)vm0;
     int64_t cnt = 0;
-    const int64_t o = FILL ? offs[k] - offs[k0] : 0;
+    const int64_t o = FILL ? offs[k] : 0;
     walk_pt<S>(L, T, vmask, up, dn, coeffs[k], eps, [&](S nu, S nd, double h) {
+        if (n_parts > 1 && part_of<S>(nu, nd, n_parts) != part) return;
         if (FILL) {
             store_string<S>(words + 4 * (o + cnt), nu);
             store_string<S>(words + 4 * (o + cnt) + 2, nd);
@@ -289,7 +301,7 @@ __global__ void k_pt_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const
         }
         ++cnt;
     });
-    if (!FILL) counts[t] = cnt;
+    if (!FILL) counts[k] = cnt;
 }
 
 __global__ void k_iota64(int64_t* v, int64_t n) {
@@ -379,23 +391,16 @@ __global__ void k_run_sum(const uint64_t* w, const double* v, const int64_t* idx
     out_v[u] = sum;
 }
 
-__global__ void k_pt_keep(const uint64_t* w, const double* v, const int64_t* keep, const int64_t* pos, int64_t runs,
-                          uint64_t* out_w, double* out_v) {
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= runs || !keep[u]) return;
-    const int64_t p = pos[u];
-    for (int q = 0; q < 4; ++q) out_w[4 * p + q] = w[4 * u + q];
-    out_v[p] = v[u];
-}
-
-// term_a = num_a^2 / (e0 - H_aa); fixed-order partial sums, one per block
+// term_a = num_a^2 / (e0 - H_aa) for kept runs; fixed-order partial sums, one per block
 constexpr int kPtBlocks = 1024;
 template <class S>
 __global__ void __launch_bounds__(256)
-    k_pt_terms(DevInts T, const uint64_t* w, const double* num, int64_t m, double e0, double* partial) {
+    k_pt_terms(DevInts T, const uint64_t* w, const double* num, const int64_t* keep, int64_t m, double e0,
+               double* partial) {
     __shared__ double sh[256];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!keep[i]) continue;
         const S up = load_string<S>(w + 4 * i), dn = load_string<S>(w + 4 * i + 2);
         acc += num[i] * num[i] / (e0 - elem_diag<S>(T, up, dn));
     }
@@ -783,172 +788,98 @@ int hci_pt2_dtm(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t 
         if ((e = tmp.ensure(need)) != hipSuccess) return e;
         return rocprim::exclusive_scan(tmp.p, need, in, out, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
     };
-    // candidates per source determinant, and their offsets
+    auto walk = [&](bool fill, uint32_t n_parts, uint32_t part, int64_t* cnt, const int64_t* off, uint64_t* cw,
+                    double* cv) {
+        if (wide) {
+            if (fill)
+                hipLaunchKernelGGL((k_pt_walk<u128, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                                   d_dets, d_coef, n, eps, n_parts, part, cnt, off, cw, cv);
+            else
+                hipLaunchKernelGGL((k_pt_walk<u128, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                                   d_dets, d_coef, n, eps, n_parts, part, cnt, off, cw, cv);
+        } else {
+            if (fill)
+                hipLaunchKernelGGL((k_pt_walk<uint64_t, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0],
+                                   L.vmask[1], d_dets, d_coef, n, eps, n_parts, part, cnt, off, cw, cv);
+            else
+                hipLaunchKernelGGL((k_pt_walk<uint64_t, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0],
+                                   L.vmask[1], d_dets, d_coef, n, eps, n_parts, part, cnt, off, cw, cv);
+        }
+    };
     int64_t *cnt = nullptr, *off = nullptr, *didx = nullptr;
+    double* partial = nullptr;
     HIP_TRY(c, b.alloc(&cnt, (size_t)n + 1));
     HIP_TRY(c, b.alloc(&off, (size_t)n + 1));
-    if (wide)
-        hipLaunchKernelGGL((k_pt_walk<u128, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets,
-                           d_coef, (int64_t)0, n, eps, cnt, nullptr, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((k_pt_walk<uint64_t, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
-                           d_dets, d_coef, (int64_t)0, n, eps, cnt, nullptr, nullptr, nullptr);
+    HIP_TRY(c, b.alloc(&didx, (size_t)n));
+    HIP_TRY(c, b.alloc(&partial, kPtBlocks));
+    HIP_TRY(c, sort_dets(tmp, b, d_dets, n, nw, didx, s));
+    // all candidates first: their number sets the number of parts
+    walk(false, 1, 0, cnt, nullptr, nullptr, nullptr);
     HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
     HIP_TRY(c, scan(cnt, off, n + 1));
-    std::vector<int64_t> h_off((size_t)n + 1);
-    HIP_TRY(c, hipMemcpyAsync(h_off.data(), off, h_off.size() * 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, b.alloc(&didx, (size_t)n));
-    HIP_TRY(c, sort_dets(tmp, b, d_dets, n, nw, didx, s));
+    int64_t total = 0;
+    HIP_TRY(c, hipMemcpyAsync(&total, off + n, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    // external determinants with their summed contributions, batch by batch
-    const int64_t kBatch = batch > 0 ? batch : (int64_t)1 << 26;
-    uint64_t* g_w = nullptr;
-    double* g_v = nullptr;
-    int64_t g_n = 0, g_cap = 0;
-    int n_batches = 0;
-    int rc = HCI_OK;
-    auto release = [&]() {
-        hipFree(g_w);
-        hipFree(g_v);
-    };
-    for (int64_t b0 = 0; b0 < n && rc == HCI_OK;) {
-        int64_t b1 = b0 + 1;
-        while (b1 < n && h_off[b1 + 1] - h_off[b0] <= kBatch) ++b1;
-        const int64_t m = h_off[b1] - h_off[b0];
-        const int64_t k0 = b0;
-        b0 = b1;
+    const int64_t budget = batch > 0 ? batch : (int64_t)1 << 28;
+    const int64_t parts64 = (total + budget - 1) / budget;
+    if (parts64 > (int64_t)1 << 20) return hci_fail(c, HCI_ERR_ARG, "hci_pt2_dtm: budget too small for the candidates");
+    const uint32_t n_parts = (uint32_t)std::max<int64_t>(parts64, 1);
+    double sum = 0.0;
+    int64_t n_out = 0;
+    std::vector<double> hp(kPtBlocks);
+    for (uint32_t part = 0; part < n_parts; ++part) {
+        int64_t m = total;
+        if (n_parts > 1) {
+            walk(false, n_parts, part, cnt, nullptr, nullptr, nullptr);
+            HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
+            HIP_TRY(c, scan(cnt, off, n + 1));
+            HIP_TRY(c, hipMemcpyAsync(&m, off + n, 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+        }
         if (m == 0) continue;
-        ++n_batches;
         Bufs bb;
         uint64_t *cw = nullptr, *rw = nullptr;
         double *cv = nullptr, *rv = nullptr;
         int64_t *cidx = nullptr, *head = nullptr, *hpos = nullptr, *start = nullptr, *keep = nullptr, *kpos = nullptr;
-        auto step = [&]() -> hipError_t {
-            hipError_t e;
-            if ((e = bb.alloc(&cw, (size_t)m * 4)) != hipSuccess) return e;
-            if ((e = bb.alloc(&cv, (size_t)m)) != hipSuccess) return e;
-            if (wide)
-                hipLaunchKernelGGL((k_pt_walk<u128, true>), grid_for(b1 - k0, 256), 256, 0, s, D, T, L.vmask[0],
-                                   L.vmask[1], d_dets, d_coef, k0, b1 - k0, eps, nullptr, off, cw, cv);
-            else
-                hipLaunchKernelGGL((k_pt_walk<uint64_t, true>), grid_for(b1 - k0, 256), 256, 0, s, D, T, L.vmask[0],
-                                   L.vmask[1], d_dets, d_coef, k0, b1 - k0, eps, nullptr, off, cw, cv);
-            if ((e = bb.alloc(&cidx, (size_t)m)) != hipSuccess) return e;
-            if ((e = sort_dets(tmp, bb, cw, m, nw, cidx, s)) != hipSuccess) return e;
-            if ((e = bb.alloc(&head, (size_t)m + 1)) != hipSuccess) return e;
-            if ((e = bb.alloc(&hpos, (size_t)m + 1)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_run_heads, grid_for(m, 256), 256, 0, s, cw, cidx, m, head);
-            if ((e = hipMemsetAsync(head + m, 0, 8, s)) != hipSuccess) return e;
-            if ((e = scan(head, hpos, m + 1)) != hipSuccess) return e;
-            int64_t runs = 0;
-            if ((e = hipMemcpyAsync(&runs, hpos + m, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-            if ((e = bb.alloc(&start, (size_t)runs + 1)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_run_starts, grid_for(m + 1, 256), 256, 0, s, head, hpos, m, start);
-            if ((e = bb.alloc(&rw, (size_t)runs * 4)) != hipSuccess) return e;
-            if ((e = bb.alloc(&rv, (size_t)runs)) != hipSuccess) return e;
-            if ((e = bb.alloc(&keep, (size_t)runs + 1)) != hipSuccess) return e;
-            if ((e = bb.alloc(&kpos, (size_t)runs + 1)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_run_sum, grid_for(runs, 256), 256, 0, s, cw, cv, cidx, start, runs, d_dets, didx, n, rw,
-                               rv, keep);
-            if ((e = hipMemsetAsync(keep + runs, 0, 8, s)) != hipSuccess) return e;
-            if ((e = scan(keep, kpos, runs + 1)) != hipSuccess) return e;
-            int64_t kept = 0;
-            if ((e = hipMemcpyAsync(&kept, kpos + runs, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-            if (g_n + kept > g_cap) {
-                const int64_t cap = std::max<int64_t>(2 * (g_n + kept), 1 << 20);
-                uint64_t* nw_ = nullptr;
-                double* nv_ = nullptr;
-                if ((e = hipMalloc(&nw_, (size_t)cap * 32)) != hipSuccess) return e;
-                if ((e = hipMalloc(&nv_, (size_t)cap * 8)) != hipSuccess) {
-                    hipFree(nw_);
-                    return e;
-                }
-                if (g_n > 0) {
-                    hipMemcpyAsync(nw_, g_w, (size_t)g_n * 32, hipMemcpyDeviceToDevice, s);
-                    hipMemcpyAsync(nv_, g_v, (size_t)g_n * 8, hipMemcpyDeviceToDevice, s);
-                }
-                if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-                release();
-                g_w = nw_;
-                g_v = nv_;
-                g_cap = cap;
-            }
-            hipLaunchKernelGGL(k_pt_keep, grid_for(runs, 256), 256, 0, s, rw, rv, keep, kpos, runs, g_w + 4 * g_n,
-                               g_v + g_n);
-            g_n += kept;
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-            return hipStreamSynchronize(s);
-        };
-        hipError_t e = step();
-        if (e != hipSuccess) rc = hci_fail(c, HCI_ERR_HIP, std::string("hci_pt2_dtm: ") + hipGetErrorString(e));
-    }
-    if (rc != HCI_OK) {
-        release();
-        return rc;
-    }
-    // one external determinant may come from several batches: merge in batch order
-    uint64_t* uw = g_w;
-    double* uv = g_v;
-    int64_t un = g_n;
-    if (n_batches > 1 && g_n > 0) {
-        int64_t *gidx = nullptr, *head = nullptr, *hpos = nullptr, *start = nullptr, *keep = nullptr;
-        uint64_t* rw = nullptr;
-        double* rv = nullptr;
+        HIP_TRY(c, bb.alloc(&cw, (size_t)m * 4));
+        HIP_TRY(c, bb.alloc(&cv, (size_t)m));
+        walk(true, n_parts, part, nullptr, off, cw, cv);
+        // stable sort by words: each external determinant's contributions stay
+        // in discovery order
+        HIP_TRY(c, bb.alloc(&cidx, (size_t)m));
+        HIP_TRY(c, sort_dets(tmp, bb, cw, m, nw, cidx, s));
+        HIP_TRY(c, bb.alloc(&head, (size_t)m + 1));
+        HIP_TRY(c, bb.alloc(&hpos, (size_t)m + 1));
+        hipLaunchKernelGGL(k_run_heads, grid_for(m, 256), 256, 0, s, cw, cidx, m, head);
+        HIP_TRY(c, hipMemsetAsync(head + m, 0, 8, s));
+        HIP_TRY(c, scan(head, hpos, m + 1));
         int64_t runs = 0;
-        hipError_t e = b.alloc(&gidx, (size_t)g_n);
-        if (e == hipSuccess) e = sort_dets(tmp, b, g_w, g_n, nw, gidx, s);
-        if (e == hipSuccess) e = b.alloc(&head, (size_t)g_n + 1);
-        if (e == hipSuccess) e = b.alloc(&hpos, (size_t)g_n + 1);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_run_heads, grid_for(g_n, 256), 256, 0, s, g_w, gidx, g_n, head);
-            e = hipMemsetAsync(head + g_n, 0, 8, s);
-        }
-        if (e == hipSuccess) e = scan(head, hpos, g_n + 1);
-        if (e == hipSuccess) e = hipMemcpyAsync(&runs, hpos + g_n, 8, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e == hipSuccess) e = b.alloc(&start, (size_t)runs + 1);
-        if (e == hipSuccess) e = b.alloc(&rw, (size_t)runs * 4);
-        if (e == hipSuccess) e = b.alloc(&rv, (size_t)runs);
-        if (e == hipSuccess) e = b.alloc(&keep, (size_t)runs);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_run_starts, grid_for(g_n + 1, 256), 256, 0, s, head, hpos, g_n, start);
-            hipLaunchKernelGGL(k_run_sum, grid_for(runs, 256), 256, 0, s, g_w, g_v, gidx, start, runs, d_dets, didx,
-                               (int64_t)0, rw, rv, keep);
-            e = hipGetLastError();
-        }
-        if (e != hipSuccess) {
-            release();
-            return hci_fail(c, HCI_ERR_HIP, std::string("hci_pt2_dtm: ") + hipGetErrorString(e));
-        }
-        uw = rw;
-        uv = rv;
-        un = runs;
-    }
-    double sum = 0.0;
-    if (un > 0) {
-        double* part = nullptr;
-        hipError_t e = b.alloc(&part, kPtBlocks);
-        if (e == hipSuccess) {
-            if (wide)
-                hipLaunchKernelGGL(k_pt_terms<u128>, kPtBlocks, 256, 0, s, T, uw, uv, un, e0, part);
-            else
-                hipLaunchKernelGGL(k_pt_terms<uint64_t>, kPtBlocks, 256, 0, s, T, uw, uv, un, e0, part);
-            e = hipGetLastError();
-        }
-        std::vector<double> hp(kPtBlocks);
-        if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, kPtBlocks * 8, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            release();
-            return hci_fail(c, HCI_ERR_HIP, std::string("hci_pt2_dtm: ") + hipGetErrorString(e));
-        }
+        HIP_TRY(c, hipMemcpyAsync(&runs, hpos + m, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        HIP_TRY(c, bb.alloc(&start, (size_t)runs + 1));
+        HIP_TRY(c, bb.alloc(&rw, (size_t)runs * 4));
+        HIP_TRY(c, bb.alloc(&rv, (size_t)runs));
+        HIP_TRY(c, bb.alloc(&keep, (size_t)runs + 1));
+        HIP_TRY(c, bb.alloc(&kpos, (size_t)runs + 1));
+        hipLaunchKernelGGL(k_run_starts, grid_for(m + 1, 256), 256, 0, s, head, hpos, m, start);
+        hipLaunchKernelGGL(k_run_sum, grid_for(runs, 256), 256, 0, s, cw, cv, cidx, start, runs, d_dets, didx, n, rw, rv,
+                           keep);
+        HIP_TRY(c, hipMemsetAsync(keep + runs, 0, 8, s));
+        HIP_TRY(c, scan(keep, kpos, runs + 1));
+        if (wide)
+            hipLaunchKernelGGL(k_pt_terms<u128>, kPtBlocks, 256, 0, s, T, rw, rv, keep, runs, e0, partial);
+        else
+            hipLaunchKernelGGL(k_pt_terms<uint64_t>, kPtBlocks, 256, 0, s, T, rw, rv, keep, runs, e0, partial);
+        HIP_TRY(c, hipGetLastError());
+        int64_t kept = 0;
+        HIP_TRY(c, hipMemcpyAsync(&kept, kpos + runs, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipMemcpyAsync(hp.data(), partial, kPtBlocks * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
         for (double v : hp) sum += v;
+        n_out += kept;
     }
-    release();
     *e2 = sum;
-    if (n_ext) *n_ext = un;
+    if (n_ext) *n_ext = n_out;
     return HCI_OK;
 }
 

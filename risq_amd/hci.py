@@ -276,7 +276,8 @@ class HciContext:
         """Deterministic PT2 energy of the space and the number of external
         determinants that contribute (dtm_pt, src/pt/mod.rs:81-122); needs
         excite_init on the same integrals.  batch: candidate excitations per
-        pass (0: the library's default)."""
+        pass over the source determinants (0: the library's default); more
+        candidates split the external determinants into hash parts."""
         dets = np.ascontiguousarray(dets, dtype=np.uint64)
         coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
         e2 = C.c_double()
