@@ -259,11 +259,15 @@ int hci_mlp_collect(hci_ctx* ctx, const hci_csr* h, const double* x_exact, const
  * singles != 0.  *count = the number found; if it exceeds cap the call fails
  * with HCI_ERR_ARG and nothing is written (call again with cap >= *count).
  * The caller adds each one's spin-flipped partner (src/wf/mod.rs:99-116).
+ * With diag (the space's diagonal) and out_diag non-null, out_diag receives
+ * each new determinant's diagonal updated from its first discoverer's in
+ * O(N) (Det::new_diag_opp/same/sing, src/wf/det.rs:271-420; the reference
+ * stores these in Det.diag, src/wf/mod.rs:1525,1569,1618); both may be NULL.
  * hci_excite_info returns the largest magnitudes of the three lists. */
 int hci_excite_init(hci_ctx* ctx, int n_core, int n_up, int n_dn, double eps_var, double* eps_start);
 int hci_excite_info(const hci_ctx* ctx, double* max_opp, double* max_same, double* max_sing);
-int hci_select(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_t n, double eps, int singles,
-               uint64_t* out, int64_t cap, int64_t* count, int mem);
+int hci_select(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
+               double eps, int singles, uint64_t* out, double* out_diag, int64_t cap, int64_t* count, int mem);
 
 /* Deterministic second-order (Epstein-Nesbet) energy of the space (dets,
  * coeffs) with variational energy e0 (dtm_pt, src/pt/mod.rs:81-122): every

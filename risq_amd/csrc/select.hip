@@ -88,8 +88,16 @@ __device__ __forceinline__ S bit(int p) {
     return ((S)1) << p;
 }
 
-// Walks one determinant's excitations in the reference's order; emit(up, dn)
-// receives every candidate.
+// Excitation code of a candidate: bits 0-1 kind (0 opposite-spin double p up,
+// q dn -> r up, s dn; 1 same-spin double p < q -> r < s; 2 single p -> r),
+// bit 2 alpha (kinds 1, 2), then p, q, r, s in 7 bits each from bit 3.
+__device__ __forceinline__ uint32_t exc_code(int kind, int alpha, int p, int q, int r, int s) {
+    return (uint32_t)kind | ((uint32_t)alpha << 2) | ((uint32_t)p << 3) | ((uint32_t)q << 10) | ((uint32_t)r << 17) |
+           ((uint32_t)s << 24);
+}
+
+// Walks one determinant's excitations in the reference's order; emit(up, dn,
+// code) receives every candidate.
 template <class S, class Emit>
 __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, double local, bool singles, Emit emit) {
     if (L.max_opp >= local) {
@@ -103,7 +111,7 @@ __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, dou
                     const uint16_t t = L.ot[base + e];
                     const int r = t & 0xFF, s = t >> 8;
                     if (((up >> r) & (S)1) || ((dn >> s) & (S)1)) continue;
-                    emit((up ^ bit<S>(i)) | bit<S>(r), (dn ^ bit<S>(j)) | bit<S>(s));
+                    emit((up ^ bit<S>(i)) | bit<S>(r), (dn ^ bit<S>(j)) | bit<S>(s), exc_code(0, 0, i, j, r, s));
                 }
             }
         }
@@ -122,10 +130,11 @@ __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, dou
                         const int r = t & 0xFF, s = t >> 8;
                         if (((cfg >> r) & (S)1) || ((cfg >> s) & (S)1)) continue;
                         const S nw = (cfg ^ bit<S>(i) ^ bit<S>(j)) | bit<S>(r) | bit<S>(s);
+                        const uint32_t code = exc_code(1, spin == 0, i, j, r, s);
                         if (spin == 0)
-                            emit(nw, dn);
+                            emit(nw, dn, code);
                         else
-                            emit(up, nw);
+                            emit(up, nw, code);
                     }
                 }
             }
@@ -144,10 +153,11 @@ __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, dou
                     const S nw = (cfg ^ bit<S>(i)) | bit<S>(r);
                     // ham_sing(D, new) (src/ham/mod.rs:135-168)
                     if (fabs(elem_single<S>(T, cfg, nw, other)) > local) {
+                        const uint32_t code = exc_code(2, spin == 0, i, 0, r, 0);
                         if (spin == 0)
-                            emit(nw, dn);
+                            emit(nw, dn, code);
                         else
-                            emit(up, nw);
+                            emit(up, nw, code);
                     }
                 }
             }
@@ -244,7 +254,7 @@ __device__ __forceinline__ void store_string<u128>(uint64_t* w, u128 x) {
 template <class S, bool FILL>
 __global__ void k_select(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const uint64_t* __restrict__ dets,
                          const double* __restrict__ coeffs, int64_t n, double eps, int singles, int64_t* counts,
-                         const int64_t* __restrict__ offs, uint64_t* out) {
+                         const int64_t* __restrict__ offs, uint64_t* out, int64_t* src, uint32_t* codes) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const S up = load_string<S>(dets + 4 * k), dn = load_string<S>(dets + 4 * k + 2);
@@ -257,10 +267,14 @@ __global__ void k_select(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const 
     const double local = c > 0.0 ? eps / c : INFINITY;
     int64_t cnt = 0;
     uint64_t* o = FILL ? out + 4 * offs[k] : nullptr;
-    walk<S>(L, T, vmask, up, dn, local, singles != 0, [&](S nu, S nd) {
+    walk<S>(L, T, vmask, up, dn, local, singles != 0, [&](S nu, S nd, uint32_t code) {
         if (FILL) {
             store_string<S>(o + 4 * cnt, nu);
             store_string<S>(o + 4 * cnt + 2, nd);
+            if (src) {
+                src[offs[k] + cnt] = k;
+                codes[offs[k] + cnt] = code;
+            }
         }
         ++cnt;
     });
@@ -276,6 +290,74 @@ __device__ __forceinline__ uint32_t part_of(S up, S dn, uint32_t n_parts) {
     h *= 0xBF58476D1CE4E5B9ull;
     h ^= h >> 29;
     return (uint32_t)(h % n_parts);
+}
+
+// Det::new_diag_opp / new_diag_same / new_diag_sing (src/wf/det.rs:271-420):
+// the candidate's diagonal from its parent's, in the reference's order of
+// operations (direct(i,r,i,r) = (ii|rr) = J3, direct_plus_exchange(i,r,i,r) =
+// (ii|rr) - (ir|ri) = D3, the same IEEE values the reference reads).
+template <class S>
+__device__ double new_diag(const DevInts& T, S up, S dn, double diag, uint32_t code) {
+    const int n = T.n;
+    const int kind = code & 3, alpha = (code >> 2) & 1;
+    const int p = (code >> 3) & 127, q = (code >> 10) & 127, r = (code >> 17) & 127, s = (code >> 24) & 127;
+    auto J = [&](int i, int k) { return T.J3[(size_t)(i * n + i) * n + k]; };
+    auto Dx = [&](int i, int k) { return T.D3[(size_t)(i * n + i) * n + k]; };
+    auto h = [&](int i) { return T.h1[i * n + i]; };
+    if (kind == 2) {
+        double out = diag + h(r) - h(p);
+        const S same = alpha ? up : dn, other = alpha ? dn : up;
+        for (S m = same; m; m &= m - (S)1) {
+            const int i = ctz(m);
+            if (i == p) continue;
+            out += Dx(i, r) - Dx(i, p);
+        }
+        for (S m = other; m; m &= m - (S)1) {
+            const int i = ctz(m);
+            out += J(i, r) - J(i, p);
+        }
+        return out;
+    }
+    double out = diag + h(r) + h(s) - h(p) - h(q);
+    if (kind == 0) {
+        out += J(r, s) - J(p, q);
+        for (S m = up; m; m &= m - (S)1) {
+            const int i = ctz(m);
+            if (i == p) continue;
+            out += Dx(i, r) - Dx(i, p);
+            out += J(i, s) - J(i, q);
+        }
+        for (S m = dn; m; m &= m - (S)1) {
+            const int i = ctz(m);
+            if (i == q) continue;
+            out += Dx(i, s) - Dx(i, q);
+            out += J(i, r) - J(i, p);
+        }
+        return out;
+    }
+    out += Dx(r, s) - Dx(p, q);
+    const S same = alpha ? up : dn, other = alpha ? dn : up;
+    for (S m = same; m; m &= m - (S)1) {
+        const int i = ctz(m);
+        if (i == p || i == q) continue;
+        out += Dx(i, r) + Dx(i, s) - Dx(i, p) - Dx(i, q);
+    }
+    for (S m = other; m; m &= m - (S)1) {
+        const int i = ctz(m);
+        out += J(i, r) + J(i, s) - J(i, p) - J(i, q);
+    }
+    return out;
+}
+
+// diagonals of the surviving candidates (rank: candidate index per survivor)
+template <class S>
+__global__ void k_new_diag(DevInts T, const uint64_t* __restrict__ dets, const double* __restrict__ diag,
+                           const int64_t* __restrict__ src, const uint32_t* __restrict__ codes,
+                           const int64_t* __restrict__ rank, int64_t k, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const int64_t c = rank[i], j = src[c];
+    out[i] = new_diag<S>(T, load_string<S>(dets + 4 * j), load_string<S>(dets + 4 * j + 2), diag[j], codes[c]);
 }
 
 template <class S, bool FILL>
@@ -657,11 +739,11 @@ int hci_excite_info(const hci_ctx* c, double* max_opp, double* max_same, double*
     return HCI_OK;
 }
 
-int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n, double eps, int singles,
-               uint64_t* out, int64_t cap, int64_t* count, int mem) {
+int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n, double eps,
+               int singles, uint64_t* out, double* out_diag, int64_t cap, int64_t* count, int mem) {
     if (!c) return HCI_ERR_ARG;
     if (!c->sel) return hci_fail(c, HCI_ERR_STATE, "hci_select: call hci_excite_init first");
-    if (n < 1 || !dets || !coeffs || !count || cap < 0 || (cap > 0 && !out) || !(eps > 0.0))
+    if (n < 1 || !dets || !coeffs || !count || cap < 0 || (cap > 0 && !out) || !(eps > 0.0) || (out_diag && !diag))
         return hci_fail(c, HCI_ERR_ARG, "hci_select: bad arguments");
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -673,6 +755,7 @@ int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n
     Tmp tmp;
     const uint64_t* d_dets = dets;
     const double* d_coef = coeffs;
+    const double* d_diag = diag;
     if (mem == HCI_MEM_HOST) {
         uint64_t* w = nullptr;
         double* q = nullptr;
@@ -682,6 +765,12 @@ int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n
         HIP_TRY(c, hipMemcpyAsync(q, coeffs, (size_t)n * 8, hipMemcpyHostToDevice, s));
         d_dets = w;
         d_coef = q;
+        if (out_diag) {
+            double* g = nullptr;
+            HIP_TRY(c, b.alloc(&g, (size_t)n));
+            HIP_TRY(c, hipMemcpyAsync(g, diag, (size_t)n * 8, hipMemcpyHostToDevice, s));
+            d_diag = g;
+        }
     }
     const bool wide = c->n_orb > 64;
     int64_t *cnt = nullptr, *off = nullptr;
@@ -689,10 +778,10 @@ int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n
     HIP_TRY(c, b.alloc(&off, (size_t)n + 1));
     if (wide)
         hipLaunchKernelGGL((k_select<u128, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets,
-                           d_coef, n, eps, singles, cnt, nullptr, nullptr);
+                           d_coef, n, eps, singles, cnt, nullptr, nullptr, nullptr, nullptr);
     else
         hipLaunchKernelGGL((k_select<uint64_t, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
-                           d_dets, d_coef, n, eps, singles, cnt, nullptr, nullptr);
+                           d_dets, d_coef, n, eps, singles, cnt, nullptr, nullptr, nullptr, nullptr);
     HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
     size_t need = 0;
     HIP_TRY(c, rocprim::exclusive_scan(nullptr, need, cnt, off, (int64_t)0, (size_t)n + 1, rocprim::plus<int64_t>(), s));
@@ -706,13 +795,19 @@ int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n
         return HCI_OK;
     }
     uint64_t* cand = nullptr;
+    int64_t* csrc = nullptr;
+    uint32_t* ccode = nullptr;
     HIP_TRY(c, b.alloc(&cand, (size_t)m * 4));
+    if (out_diag) {
+        HIP_TRY(c, b.alloc(&csrc, (size_t)m));
+        HIP_TRY(c, b.alloc(&ccode, (size_t)m));
+    }
     if (wide)
         hipLaunchKernelGGL((k_select<u128, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets,
-                           d_coef, n, eps, singles, nullptr, off, cand);
+                           d_coef, n, eps, singles, nullptr, off, cand, csrc, ccode);
     else
         hipLaunchKernelGGL((k_select<uint64_t, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
-                           d_dets, d_coef, n, eps, singles, nullptr, off, cand);
+                           d_dets, d_coef, n, eps, singles, nullptr, off, cand, csrc, ccode);
     HIP_TRY(c, hipGetLastError());
     // sort candidates and the current space; keep first occurrences not in the space
     const int nw = wide ? 4 : 2;
@@ -746,6 +841,18 @@ int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t n
     uint64_t* res = nullptr;
     HIP_TRY(c, b.alloc(&res, (size_t)k * 4));
     hipLaunchKernelGGL(k_gather_dets, grid_for(k, 256), 256, 0, s, cand, rank2, k, res);
+    if (out_diag) {
+        double* dg = nullptr;
+        HIP_TRY(c, b.alloc(&dg, (size_t)k));
+        if (wide)
+            hipLaunchKernelGGL(k_new_diag<u128>, grid_for(k, 256), 256, 0, s, T, d_dets, d_diag, csrc, ccode, rank2, k,
+                               dg);
+        else
+            hipLaunchKernelGGL(k_new_diag<uint64_t>, grid_for(k, 256), 256, 0, s, T, d_dets, d_diag, csrc, ccode,
+                               rank2, k, dg);
+        HIP_TRY(c, hipMemcpyAsync(out_diag, dg, (size_t)k * 8,
+                                  mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMemcpyAsync(out, res, (size_t)k * 32,
                               mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));

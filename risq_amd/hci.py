@@ -121,7 +121,7 @@ SIGNATURES = {
     "hci_event_elapsed_ms": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "hci_excite_init": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "hci_excite_info": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-    "hci_select": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_int, _P, C.c_int64,
+    "hci_select": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_int, _P, _P, C.c_int64,
                              C.POINTER(C.c_int64), C.c_int]),
     "hci_pt2_dtm": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_double, C.c_int64, C.POINTER(C.c_double),
                               C.POINTER(C.c_int64), C.c_int]),
@@ -254,18 +254,25 @@ class HciContext:
         self._check(self.lib.hci_excite_info(self.handle, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
-    def select(self, dets: np.ndarray, coeffs: np.ndarray, eps: float, singles: bool) -> np.ndarray:
-        """New determinants (n_new, 4) uint64 in the reference's discovery order."""
+    def select(self, dets: np.ndarray, coeffs: np.ndarray, eps: float, singles: bool, diag=None):
+        """New determinants (n_new, 4) uint64 in the reference's discovery order;
+        with the space's diagonal, also the new determinants' diagonals
+        (new_diag_* from their first discoverer): (dets, diags)."""
         dets = np.ascontiguousarray(dets, dtype=np.uint64)
         coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        d_in = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
         cnt = C.c_int64()
         cap = max(1024, 4 * dets.shape[0])
         while True:
             out = np.empty((cap, 4), dtype=np.uint64)
-            rc = self.lib.hci_select(self.handle, _ptr(dets), _ptr(coeffs), dets.shape[0], eps, int(singles),
-                                     _ptr(out), cap, C.byref(cnt), MEM_HOST)
+            d_out = np.empty(cap) if d_in is not None else None
+            rc = self.lib.hci_select(self.handle, _ptr(dets), _ptr(coeffs), None if d_in is None else _ptr(d_in),
+                                     dets.shape[0], eps, int(singles), _ptr(out),
+                                     None if d_out is None else _ptr(d_out), cap, C.byref(cnt), MEM_HOST)
             if rc == HCI_OK:
-                return out[:cnt.value]
+                if d_in is None:
+                    return out[:cnt.value]
+                return out[:cnt.value], d_out[:cnt.value]
             if cnt.value > cap:
                 cap = cnt.value
                 continue

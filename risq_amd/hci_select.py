@@ -181,11 +181,65 @@ def _pairs(words: np.ndarray):
     return [(int(r[0]) | (int(r[1]) << 64), int(r[2]) | (int(r[3]) << 64)) for r in w]
 
 
-def find_new_dets(dets, coeffs, index: dict, gen: ExciteGenerator, eps: float, singles: bool, elements):
+def new_diag(ints: Integrals, parent: tuple[int, int], diag: float, exc: tuple) -> float:
+    """Det::new_diag_opp / new_diag_same / new_diag_sing (src/wf/det.rs:271-420):
+    the diagonal of the determinant `exc` leads to, updated from its parent's in
+    O(N) with the reference's order of operations.  exc is ("opp", (p, q), (r, s))
+    for p up, q dn -> r up, s dn; ("same", (p, q), (r, s), alpha); ("sing", p, r,
+    alpha)."""
+    def direct(p, q, r, s):  # src/ham/mod.rs:69-73: (pr|qs)
+        return ints.eri(p, r, q, s)
+
+    def dpe(p, q, r, s):  # src/ham/mod.rs:75-82: (pr|qs) - (ps|qr)
+        return ints.eri(p, r, q, s) - ints.eri(p, s, q, r)
+
+    up, dn = parent
+    if exc[0] == "sing":
+        _, p, r, alpha = exc
+        out = diag + ints.h1(r, r) - ints.h1(p, p)
+        same, other = (up, dn) if alpha else (dn, up)
+        for i in _bits(same):
+            if i == p:
+                continue
+            out += dpe(i, r, i, r) - dpe(i, p, i, p)
+        for i in _bits(other):
+            out += direct(i, r, i, r) - direct(i, p, i, p)
+        return out
+    (p, q), (r, s) = exc[1], exc[2]
+    out = diag + ints.h1(r, r) + ints.h1(s, s) - ints.h1(p, p) - ints.h1(q, q)
+    if exc[0] == "opp":
+        out += direct(r, s, r, s) - direct(p, q, p, q)
+        for i in _bits(up):
+            if i == p:
+                continue
+            out += dpe(i, r, i, r) - dpe(i, p, i, p)
+            out += direct(i, s, i, s) - direct(i, q, i, q)
+        for i in _bits(dn):
+            if i == q:
+                continue
+            out += dpe(i, s, i, s) - dpe(i, q, i, q)
+            out += direct(i, r, i, r) - direct(i, p, i, p)
+        return out
+    alpha = exc[3]
+    out += dpe(r, s, r, s) - dpe(p, q, p, q)
+    same, other = (up, dn) if alpha else (dn, up)
+    for i in _bits(same):
+        if i == p or i == q:
+            continue
+        out += dpe(i, r, i, r) + dpe(i, s, i, s) - dpe(i, p, i, p) - dpe(i, q, i, q)
+    for i in _bits(other):
+        out += direct(i, r, i, r) + direct(i, s, i, s) - direct(i, p, i, p) - direct(i, q, i, q)
+    return out
+
+
+def find_new_dets(dets, coeffs, index: dict, gen: ExciteGenerator, eps: float, singles: bool, elements,
+                  diags=None, ints: Integrals | None = None):
     """src/wf/mod.rs:1467-1631 (iterate_excites without matmul): new determinants in
     the order the reference discovers them.  `elements(words1, words2)` gives the
-    exact singles' matrix elements (one batched call)."""
-    cands = []  # (config, parent index or -1 for a double) in discovery order
+    exact singles' matrix elements (one batched call).  With the space's `diags`
+    (and `ints`) it returns (new determinants, their diagonals from the first
+    discoverer, src/wf/mod.rs:1525,1569,1618)."""
+    cands = []  # (config, single slot or -1 for a double, parent index, excitation) in discovery order
     single_pairs = []
     for k, (up, dn) in enumerate(dets):
         c = abs(coeffs[k])
@@ -198,7 +252,8 @@ def find_new_dets(dets, coeffs, index: dict, gen: ExciteGenerator, eps: float, s
                             break
                         if (up >> r) & 1 or (dn >> s) & 1:
                             continue
-                        cands.append((((up ^ (1 << i)) | (1 << r), (dn ^ (1 << j)) | (1 << s)), -1))
+                        cands.append((((up ^ (1 << i)) | (1 << r), (dn ^ (1 << j)) | (1 << s)), -1, k,
+                                      ("opp", (i, j), (r, s))))
         if gen.max_same_doub >= local:
             for cfg, alpha in ((up, True), (dn, False)):
                 for i, j in _bit_pairs(gen.valence & cfg):
@@ -208,7 +263,7 @@ def find_new_dets(dets, coeffs, index: dict, gen: ExciteGenerator, eps: float, s
                         if (cfg >> r) & 1 or (cfg >> s) & 1:
                             continue
                         new = (cfg ^ (1 << i) ^ (1 << j)) | (1 << r) | (1 << s)
-                        cands.append(((new, dn) if alpha else (up, new), -1))
+                        cands.append(((new, dn) if alpha else (up, new), -1, k, ("same", (i, j), (r, s), alpha)))
         if singles and gen.max_sing >= local:
             for cfg, alpha in ((up, True), (dn, False)):
                 for i in _bits(gen.valence & cfg):
@@ -221,21 +276,23 @@ def find_new_dets(dets, coeffs, index: dict, gen: ExciteGenerator, eps: float, s
                         d = (new, dn) if alpha else (up, new)
                         if d in index:
                             continue
-                        cands.append((d, len(single_pairs)))
+                        cands.append((d, len(single_pairs), k, ("sing", i, r, alpha)))
                         single_pairs.append((k, d, local))
     ok = np.ones(len(single_pairs), dtype=bool)
     if single_pairs:
         vals = elements(_words([dets[k] for k, _, _ in single_pairs]), _words([d for _, d, _ in single_pairs]))
         ok = np.abs(vals) > np.array([loc for _, _, loc in single_pairs])
-    out, seen = [], set()
-    for d, si in cands:
+    out, out_diag, seen = [], [], set()
+    for d, si, k, exc in cands:
         if d in index or d in seen:
             continue
         if si >= 0 and not ok[si]:
             continue
         seen.add(d)
         out.append(d)
-    return out
+        if diags is not None:
+            out_diag.append(new_diag(ints, dets[k], float(diags[k]), exc))
+    return (out, out_diag) if diags is not None else out
 
 
 @dataclass
@@ -259,18 +316,21 @@ def variational(solver, ints: Integrals, n_up: int, n_dn: int, eps_var: float, n
     coeffs = np.ones(1)
     diag = solver.ham_diag(_words(dets))
     energy = float(diag[0])
+    # new determinants carry the diagonal their first discoverer computes
+    # incrementally (new_diag_*), as the reference's Det.diag does
     if hasattr(solver, "select"):
         # lists and selection in the library
         sched = EpsSchedule(solver.select_init(n_core, n_up, n_dn, eps_var), eps_var)
 
         def finder(eps, singles):
-            return [tuple(d) for d in _pairs(solver.select(_words(dets), coeffs, eps, singles))]
+            w, dg = solver.select(_words(dets), coeffs, eps, singles, diag)
+            return [tuple(d) for d in _pairs(w)], list(dg)
     else:
         gen = ExciteGenerator(ints, n_core, n_up, n_dn)
         sched = init_eps(dets, gen, eps_var, z_sym)
 
         def finder(eps, singles):
-            return find_new_dets(dets, coeffs, index, gen, eps, singles, solver.elements)
+            return find_new_dets(dets, coeffs, index, gen, eps, singles, solver.elements, diags=diag, ints=ints)
     history = []
     converged = False
     it = 0
@@ -279,18 +339,20 @@ def variational(solver, ints: Integrals, n_up: int, n_dn: int, eps_var: float, n
         if it > max_iters:
             break
         eps = sched.next()
-        new = finder(eps, eps == eps_var)
+        new, new_diag_vals = finder(eps, eps == eps_var)
         n_old = len(dets)
-        for d in new:  # push_and_spin_flipped (src/wf/mod.rs:99-116)
+        added = []
+        for d, dg in zip(new, new_diag_vals):  # push_and_spin_flipped (src/wf/mod.rs:99-116)
             for e in ((d[1], d[0]), d) if d[0] != d[1] else (d,):
                 if e not in index:
                     index[e] = len(dets)
                     dets.append(e)
+                    added.append(dg)  # the flipped copy is a clone, diagonal included
         if eps == eps_var and not new:
             converged = True
             break
         coeffs = np.concatenate([coeffs, np.zeros(len(dets) - n_old)])
-        diag = np.concatenate([diag, solver.ham_diag(_words(dets[n_old:]))])
+        diag = np.concatenate([diag, np.array(added, dtype=np.float64)])
         # The reference warm-starts from the previous coefficients from the
         # second iteration on (src/var/mod.rs:104).  When the previous space
         # was the single start determinant (no double reaches eps, e.g. the
@@ -320,7 +382,8 @@ class GpuSolver:
         self.h = None
         if library_select:
             self.select_init = lambda n_core, n_up, n_dn, eps_var: ctx.excite_init(n_core, n_up, n_dn, eps_var)
-            self.select = lambda words, coeffs, eps, singles: ctx.select(words, coeffs, eps, singles)
+            self.select = lambda words, coeffs, eps, singles, diag: ctx.select(words, coeffs, eps, singles,
+                                                                               diag=diag)
 
     def ham_diag(self, words):
         return self.ctx.ham_diag(words)

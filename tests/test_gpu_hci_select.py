@@ -62,7 +62,7 @@ def test_library_selection_matches_restatement(gpu_ctx, name):
     """hci_excite_init / hci_select against the host restatement
     (ExciteGenerator, init_eps, find_new_dets): the same first epsilon and list
     maxima, and at every iteration of a run the same new determinants in the
-    same order."""
+    same order with bit-identical incremental diagonals (new_diag_*)."""
     import numpy as np
     c = CASES[name]
     ints = read_fcidump(os.path.join(GOLD, c["fcidump"]), c["norb"])
@@ -77,12 +77,13 @@ def test_library_selection_matches_restatement(gpu_ctx, name):
     calls = []
     orig = hs.find_new_dets
 
-    def spy(dets, coeffs, index, g, eps, singles, elements):
-        out = orig(dets, coeffs, index, g, eps, singles, elements)
-        lib = hs._pairs(gpu_ctx.select(hs._words(dets), np.asarray(coeffs), eps, singles))
+    def spy(dets, coeffs, index, g, eps, singles, elements, diags=None, ints=None):
+        out, out_diag = orig(dets, coeffs, index, g, eps, singles, elements, diags=diags, ints=ints)
+        w, dg = gpu_ctx.select(hs._words(dets), np.asarray(coeffs), eps, singles, diag=diags)
         calls.append((len(dets), eps, singles, len(out)))
-        assert lib == out, (len(dets), eps, singles)
-        return out
+        assert hs._pairs(w) == out, (len(dets), eps, singles)
+        assert np.array_equal(np.asarray(out_diag, dtype=np.float64).view(np.uint64), dg.view(np.uint64))
+        return out, out_diag
 
     hs.find_new_dets = spy
     try:

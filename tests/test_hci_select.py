@@ -139,3 +139,50 @@ def test_site_basis_hubbard_starts_from_one_determinant():
     one.build(H, w1, H.ham_diag(w1))
     rc, e, c, _, _ = one.davidson()
     assert rc == 0 and e == H.ham_diag(w1)[0] and c[0] == 1.0
+
+
+def test_incremental_diagonal_equals_ham_diag():
+    """new_diag (Det::new_diag_opp/same/sing, src/wf/det.rs:271-420) of random
+    valid excitations against the full O(N^2) Ham::ham_diag of the child, for
+    every kind of excitation and both spins (the update is exact algebra; only
+    rounding differs)."""
+    import random
+    from risq_amd.fcidump import synthetic_molecular
+    ints = synthetic_molecular(12, 24, 5)
+    H = pyoracle.OracleHam(ints)
+    rng = random.Random(3)
+    n, n_up, n_dn = 12, 4, 3
+    kinds = {"opp": 0, "same_a": 0, "same_b": 0, "sing_a": 0, "sing_b": 0}
+    for _ in range(400):
+        up = sum(1 << p for p in rng.sample(range(n), n_up))
+        dn = sum(1 << p for p in rng.sample(range(n), n_dn))
+        d0 = float(H.ham_diag(hs._words([(up, dn)]))[0])
+        occ_u = [p for p in range(n) if up >> p & 1]
+        occ_d = [p for p in range(n) if dn >> p & 1]
+        vir_u = [p for p in range(n) if not up >> p & 1]
+        vir_d = [p for p in range(n) if not dn >> p & 1]
+        kind = rng.choice(list(kinds))
+        if kind == "opp":
+            p, q, r, s = rng.choice(occ_u), rng.choice(occ_d), rng.choice(vir_u), rng.choice(vir_d)
+            child = (up ^ (1 << p) | (1 << r), dn ^ (1 << q) | (1 << s))
+            exc = ("opp", (p, q), (r, s))
+        elif kind.startswith("same"):
+            alpha = kind == "same_a"
+            occ, vir, cfg = (occ_u, vir_u, up) if alpha else (occ_d, vir_d, dn)
+            p, q = sorted(rng.sample(occ, 2))
+            r, s = sorted(rng.sample(vir, 2))
+            nw = cfg ^ (1 << p) ^ (1 << q) | (1 << r) | (1 << s)
+            child = (nw, dn) if alpha else (up, nw)
+            exc = ("same", (p, q), (r, s), alpha)
+        else:
+            alpha = kind == "sing_a"
+            occ, vir, cfg = (occ_u, vir_u, up) if alpha else (occ_d, vir_d, dn)
+            p, r = rng.choice(occ), rng.choice(vir)
+            nw = cfg ^ (1 << p) | (1 << r)
+            child = (nw, dn) if alpha else (up, nw)
+            exc = ("sing", p, r, alpha)
+        kinds[kind] += 1
+        got = hs.new_diag(ints, (up, dn), d0, exc)
+        want = float(H.ham_diag(hs._words([child]))[0])
+        assert abs(got - want) < 1e-12 * max(1.0, abs(want)), (kind, got, want)
+    assert min(kinds.values()) > 40
