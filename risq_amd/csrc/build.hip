@@ -892,6 +892,102 @@ int rank_strings(hci_ctx* c, Scratch& tmp, DevBufs& db, const S* str, int64_t N,
 // tiles of every beta group (columns: the group itself).
 constexpr int32_t kKeyedCols = 48;  // column groups wider than this use key lookups
 
+// Column groups of a tile of group g, ascending: in the alpha-major part (0)
+// the group itself and its single-connected groups, or (every double
+// vanishing) the group itself alone; in the beta-major part (1) the group
+// itself.  A group of one determinant has no pairs of its own.
+struct ColWalk {
+    int part, dbl;
+    int32_t g;
+    const int32_t* off;
+    const int32_t* nbr;
+    int32_t k, k_end;  // neighbour cursor
+    bool self_done;
+    __device__ ColWalk(int part_, int dbl_, int32_t g_, const int32_t* off_, const int32_t* nbr_off,
+                       const int32_t* nbr_)
+        : part(part_), dbl(dbl_), g(g_), off(off_), nbr(nbr_), k(0), k_end(0), self_done(false) {
+        if (part == 0 && dbl) {
+            k = nbr_off[g];
+            k_end = nbr_off[g + 1];
+        }
+        if (off[g + 1] - off[g] < 2 && !(part == 0 && dbl)) self_done = true;  // nothing inside a 1-det group
+    }
+    // next column group, or -1; *kind receives its sub-block kind
+    __device__ int32_t next(int* kind) {
+        const int32_t nb = k < k_end ? nbr[k] : INT32_MAX;
+        int32_t cg;
+        if (!self_done && g < nb) {
+            cg = g;
+            self_done = true;
+        } else if (k < k_end) {
+            cg = nb;
+            ++k;
+        } else {
+            return -1;
+        }
+        const int32_t size = off[cg + 1] - off[cg];
+        if (part == 0)
+            *kind = dbl ? (cg == g ? 0 : (size > kKeyedCols ? 3 : 1)) : (size > kKeyedCols ? 4 : 0);
+        else
+            *kind = (!dbl && size > kKeyedCols) ? 5 : 2;
+        return cg;
+    }
+};
+
+// windows and sub-blocks of one tile of each group
+__global__ void k_group_counts(int part, int32_t U, const int32_t* __restrict__ off, const int32_t* __restrict__ nbr_off,
+                               const int32_t* __restrict__ nbr, int dbl, int32_t* wt, int32_t* sbt) {
+    const int32_t g = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= U) return;
+    ColWalk cw(part, dbl, g, off, nbr_off, nbr);
+    int32_t w = 0, b = 0;
+    int kind;
+    for (int32_t cg = cw.next(&kind); cg >= 0; cg = cw.next(&kind)) {
+        const int32_t size = off[cg + 1] - off[cg];
+        w += (size + kWinCols - 1) / kWinCols;
+        b += (size + kTileCols - 1) / kTileCols;
+    }
+    wt[g] = w;
+    sbt[g] = b;
+}
+
+// every sub-block of group g's tiles at sbbase[g], windows from wbase[g]:
+// tile by tile (256 rows; a group of at most 256 rows is one tile, shared with
+// its neighbours at row offset rb[g]), column group by column group, 4096
+// columns at a time
+__global__ void k_gen_subblocks(int part, int32_t U, const int32_t* __restrict__ off,
+                                const int32_t* __restrict__ nbr_off, const int32_t* __restrict__ nbr, int dbl,
+                                const int64_t* __restrict__ wbase, const int64_t* __restrict__ sbbase,
+                                const int32_t* __restrict__ rb, SubBlock* out) {
+    const int32_t g = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= U) return;
+    const int32_t g0 = off[g], g1 = off[g + 1];
+    int64_t w = wbase[g], k = sbbase[g];
+    for (int32_t r0 = g0; r0 < g1; r0 += kTileRows) {
+        const int32_t nr = min(kTileRows, g1 - r0);
+        const int32_t row_base = (g1 - g0 > kTileRows) ? 0 : rb[g];
+        ColWalk cw(part, dbl, g, off, nbr_off, nbr);
+        int kind;
+        for (int32_t cg = cw.next(&kind); cg >= 0; cg = cw.next(&kind)) {
+            const int32_t c1 = off[cg + 1];
+            for (int32_t cs = off[cg]; cs < c1; cs += kTileCols) {
+                SubBlock b;
+                b.r0 = r0;
+                b.nr = nr;
+                b.c0 = cs;
+                b.nc = min(kTileCols, c1 - cs);
+                b.g_row = g;
+                b.g_col = cg;
+                b.kind = kind;
+                b.win_begin = (int32_t)w;
+                b.rb = row_base;
+                out[k++] = b;
+                w += (b.nc + kWinCols - 1) / kWinCols;
+            }
+        }
+    }
+}
+
 template <class S>
 int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t Ua, int32_t Ub,
                 int32_t n_pairs, const hci_build_opts& o, hci_csr* H) {
@@ -958,40 +1054,51 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         B.akq = aq_out;
         B.n_alpha = nak;
     }
-    std::vector<int32_t> ga((size_t)Ua + 1), gb((size_t)Ub + 1), no((size_t)Ua + 1), nb((size_t)n_pairs + 1);
+    // Sub-blocks are generated on the device, group by group: per group, the
+    // windows and sub-blocks one of its tiles needs; the host packs small
+    // groups into tiles (greedy, at group granularity) and places every
+    // group's sub-blocks and windows by prefix sums.
+    std::vector<int32_t> ga((size_t)Ua + 1), gb((size_t)Ub + 1);
     HIP_TRY(c, hipMemcpy(ga.data(), B.ga_off, ga.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(gb.data(), B.gb_off, gb.size() * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(no.data(), B.nbr_off, no.size() * 4, hipMemcpyDeviceToHost));
-    if (n_pairs > 0) HIP_TRY(c, hipMemcpy(nb.data(), B.nbr, (size_t)n_pairs * 4, hipMemcpyDeviceToHost));
-    std::vector<SubBlock> sbs;
     std::vector<TileTask> ta, tb;
-    int64_t n_wall = 0;
-    sbs.reserve((size_t)n_pairs + (size_t)Ua + (size_t)Ub + 16);
-    auto add_cols = [&](int32_t r0, int32_t nr, int32_t rb, int32_t g_row, int32_t g_col, int kind, int32_t c0,
-                        int32_t c1) {
-        for (int32_t cs = c0; cs < c1; cs += kTileCols) {
-            SubBlock b{};
-            b.r0 = r0;
-            b.nr = nr;
-            b.rb = rb;
-            b.c0 = cs;
-            b.nc = std::min(kTileCols, c1 - cs);
-            b.g_row = g_row;
-            b.g_col = g_col;
-            b.kind = kind;
-            b.win_begin = (int32_t)n_wall;
-            n_wall += (b.nc + kWinCols - 1) / kWinCols;
-            sbs.push_back(b);
-        }
+    int64_t n_wall = 0, n_sb = 0;
+    DevBufs gbufs;
+    struct PartGen {
+        int part;
+        int32_t U;
+        const int32_t* d_off;
+        int32_t* d_wt = nullptr;   // windows of one tile of group g
+        int32_t* d_sbt = nullptr;  // sub-blocks of one tile of group g
+        int64_t* d_wbase = nullptr;
+        int64_t* d_sbbase = nullptr;
+        int32_t* d_rb = nullptr;
     };
-    // A group larger than a tile is split into tiles of its own; smaller
-    // consecutive groups share a tile of at most kTileRows rows and about
-    // kPackWindows windows (their windows are consecutive, and entries carry
-    // rows relative to the tile).
-    constexpr int64_t kPackWindows = 256;
-    auto windows_of = [](int32_t cols) { return (int64_t)(cols + kWinCols - 1) / kWinCols; };
-    auto tile_groups = [&](const std::vector<int32_t>& off, int32_t U, std::vector<TileTask>& tasks, auto&& n_windows,
-                           auto&& add_group) {
+    PartGen parts[2] = {{0, Ua, B.ga_off}, {1, Ub, B.gb_off}};
+    for (PartGen& P : parts) {
+        if (P.part == 1 && P.U == 0) continue;
+        HIP_TRY(c, gbufs.alloc(&P.d_wt, (size_t)P.U + 1));
+        HIP_TRY(c, gbufs.alloc(&P.d_sbt, (size_t)P.U + 1));
+        HIP_TRY(c, gbufs.alloc(&P.d_wbase, (size_t)P.U + 1));
+        HIP_TRY(c, gbufs.alloc(&P.d_sbbase, (size_t)P.U + 1));
+        HIP_TRY(c, gbufs.alloc(&P.d_rb, (size_t)P.U + 1));
+        hipLaunchKernelGGL(k_group_counts, grid_for(P.U, 256), 256, 0, s, P.part, P.U, P.d_off, B.nbr_off, B.nbr, dbl,
+                           P.d_wt, P.d_sbt);
+    }
+    HIP_TRY(c, hipGetLastError());
+    for (PartGen& P : parts) {
+        if (P.part == 1 && P.U == 0) continue;
+        const std::vector<int32_t>& off = P.part == 0 ? ga : gb;
+        std::vector<int32_t> wt((size_t)P.U), sbt((size_t)P.U), rb((size_t)P.U, 0);
+        std::vector<int64_t> wbase((size_t)P.U), sbbase((size_t)P.U);
+        HIP_TRY(c, hipMemcpy(wt.data(), P.d_wt, wt.size() * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(sbt.data(), P.d_sbt, sbt.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<TileTask>& tasks = P.part == 0 ? ta : tb;
+        // A group larger than a tile is split into tiles of its own; smaller
+        // consecutive groups share a tile of at most kTileRows rows and about
+        // kPackWindows windows (their windows are consecutive, and entries
+        // carry rows relative to the tile).
+        constexpr int64_t kPackWindows = 256;
         TileTask cur{0, 0, 0, 0, 0};
         bool open = false;
         int64_t cur_w = 0;
@@ -1001,73 +1108,40 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
             tasks.push_back(cur);
             open = false;
         };
-        for (int32_t g = 0; g < U; ++g) {
+        for (int32_t g = 0; g < P.U; ++g) {
             const int32_t size = off[g + 1] - off[g];
+            wbase[g] = n_wall;
+            sbbase[g] = n_sb;
             if (size > kTileRows) {
                 close();
                 for (int32_t r0 = off[g]; r0 < off[g + 1]; r0 += kTileRows) {
-                    const int32_t nr = std::min(kTileRows, off[g + 1] - r0);
-                    TileTask t{r0, nr, (int32_t)n_wall, 0, 0};
-                    add_group(g, r0, nr, 0);
+                    TileTask t{r0, std::min(kTileRows, off[g + 1] - r0), (int32_t)n_wall, 0, 0};
+                    n_wall += wt[g];
+                    n_sb += sbt[g];
                     t.w_end = (int32_t)n_wall;
                     tasks.push_back(t);
                 }
                 continue;
             }
-            const int64_t w = n_windows(g);
-            if (open && (cur.nr + size > kTileRows || cur_w + w > kPackWindows)) close();
+            if (open && (cur.nr + size > kTileRows || cur_w + wt[g] > kPackWindows)) close();
             if (!open) {
                 cur = TileTask{off[g], 0, (int32_t)n_wall, 0, 0};
                 open = true;
                 cur_w = 0;
             }
-            add_group(g, off[g], size, cur.nr);
+            rb[g] = cur.nr;
             cur.nr += size;
-            cur_w += w;
+            cur_w += wt[g];
+            n_wall += wt[g];
+            n_sb += sbt[g];
+            if (n_wall >= (int64_t)INT32_MAX) break;
         }
         close();
-    };
-    std::vector<int32_t> cols;
-    if (dbl) {
-        // alpha-major: the group itself (beta singles and doubles) and its
-        // single-connected groups (same-beta alpha singles, opposite-spin
-        // doubles); beta-major: same-beta alpha doubles
-        tile_groups(
-            ga, Ua, ta,
-            [&](int32_t g) {
-                int64_t w = windows_of(ga[g + 1] - ga[g]);
-                for (int32_t k = no[g]; k < no[g + 1]; ++k) w += windows_of(ga[nb[k] + 1] - ga[nb[k]]);
-                return w;
-            },
-            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
-                cols.assign(nb.begin() + no[g], nb.begin() + no[g + 1]);
-                cols.insert(std::lower_bound(cols.begin(), cols.end(), g), g);
-                for (int32_t cg : cols)
-                    add_cols(r0, nr, rb, g, cg, cg == g ? 0 : (ga[cg + 1] - ga[cg] > kKeyedCols ? 3 : 1), ga[cg],
-                             ga[cg + 1]);
-            });
-        tile_groups(
-            gb, Ub, tb, [&](int32_t g) { return gb[g + 1] - gb[g] > 1 ? windows_of(gb[g + 1] - gb[g]) : 0; },
-            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
-                if (gb[g + 1] - gb[g] > 1) add_cols(r0, nr, rb, g, g, 2, gb[g], gb[g + 1]);
-            });
-    } else {
-        // every double vanishes: beta singles inside alpha groups and alpha
-        // singles inside beta groups are all the pairs
-        tile_groups(
-            ga, Ua, ta, [&](int32_t g) { return ga[g + 1] - ga[g] > 1 ? windows_of(ga[g + 1] - ga[g]) : 0; },
-            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
-                const int32_t size = ga[g + 1] - ga[g];
-                if (size > 1) add_cols(r0, nr, rb, g, g, size > kKeyedCols ? 4 : 0, ga[g], ga[g + 1]);
-            });
-        tile_groups(
-            gb, Ub, tb, [&](int32_t g) { return gb[g + 1] - gb[g] > 1 ? windows_of(gb[g + 1] - gb[g]) : 0; },
-            [&](int32_t g, int32_t r0, int32_t nr, int32_t rb) {
-                const int32_t size = gb[g + 1] - gb[g];
-                if (size > 1) add_cols(r0, nr, rb, g, g, size > kKeyedCols ? 5 : 2, gb[g], gb[g + 1]);
-            });
+        if (n_wall >= (int64_t)INT32_MAX) break;
+        HIP_TRY(c, hipMemcpy(P.d_wbase, wbase.data(), wbase.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(P.d_sbbase, sbbase.data(), sbbase.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(P.d_rb, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
     }
-    const int64_t n_sb = (int64_t)sbs.size();
     if (n_wall >= (int64_t)INT32_MAX) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many tiles");
     DevBufs wb;
     SubBlock* d_sb = nullptr;
@@ -1084,7 +1158,12 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, wb.alloc(&d_wnc, (size_t)n_wall + 1));
     HIP_TRY(c, wb.alloc(&d_nnz, 1));
     HIP_TRY(c, hipMemsetAsync(d_nnz, 0, 8, s));
-    if (n_sb > 0) HIP_TRY(c, hipMemcpyAsync(d_sb, sbs.data(), (size_t)n_sb * sizeof(SubBlock), hipMemcpyHostToDevice, s));
+    for (PartGen& P : parts) {
+        if (P.part == 1 && P.U == 0) continue;
+        hipLaunchKernelGGL(k_gen_subblocks, grid_for(P.U, 256), 256, 0, s, P.part, P.U, P.d_off, B.nbr_off, B.nbr, dbl,
+                           P.d_wbase, P.d_sbbase, P.d_rb, d_sb);
+    }
+    HIP_TRY(c, hipGetLastError());
     H->n_task_a = (int64_t)ta.size();
     H->n_task_b = (int64_t)tb.size();
     HIP_TRY(c, hipMalloc(&H->d_task_a, ta.size() * sizeof(TileTask) + 16));
