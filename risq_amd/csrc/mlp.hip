@@ -14,7 +14,7 @@
 // accumulate in FP32.  Master weights, Adam moments and activations are FP32.
 // Training runs layer by layer (k_gemm_bf16); inference inside Davidson runs
 // all three layers in one kernel (k_mlp_infer) with the bfloat16 weights in
-// LDS and each wave's 16-row activations never leaving the CU.
+// LDS and each wave's activations in registers.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -213,28 +213,29 @@ __global__ void k_prep_weights(const float* P, short* img) {
     if (i < Hd) img[kImgW1 + kImgW2 + i] = to_bf16(P[oW3 + i]);
 }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
-// Fused inference: each wave takes 16 rows at a time: standardised features
-// (bf16, LDS) -> Z1 = X W1 + b1 (2 MFMA K-steps x 8 column tiles) -> SiLU ->
-// H1 (bf16, LDS) -> Z2 = H1 W2 + b2 (8 x 8 MFMAs) -> SiLU -> out = H2 . W3 + b3
-// (bf16 operands, FP32 sums over the 16 lanes of a row group).  Writes out
-// (if given) and t = (1 + kGAmp tanh(out)) * dpr (if given).  The same
-// roundings as the layer-by-layer path: operands to bf16, FP32 accumulation.
+// Fused inference: each wave takes 32 rows at a time: standardised features
+// (bf16, read straight into MFMA fragments) -> Z1 = X W1 + b1 (2 MFMA K-steps
+// x 8 tiles per 16 rows) -> SiLU -> H1 (bf16, registers) -> Z2 = H1 W2 + b2
+// (8 x 8 MFMAs per 16 rows) -> SiLU -> out = H2 . W3 + b3 (bf16 operands, FP32
+// sums over the 16 lanes of a row group).  Writes out (if given) and
+// t = (1 + kGAmp tanh(out)) * dpr (if given).  The same roundings as the
+// layer-by-layer path: operands to bf16, FP32 accumulation.
 constexpr int kInfWaves = 4;
-constexpr int kXRow = kW1Row, kHRow = kW2Row;
+// SiLU with the hardware reciprocal (1 ulp) for inference, whose activations
+// are rounded to bfloat16 next anyway
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+
+// One wave per 32 determinants (two 16-row halves sharing each weight
+// fragment).  Layer 1 is computed transposed (W1^T X^T), which leaves every
+// lane holding H[row][16 ks + 4 (lane / 16) + i] -- exactly its A fragment of
+// layer 2's K-step ks -- so the hidden activations never leave registers.
 __global__ void __launch_bounds__(64 * kInfWaves)
     k_mlp_infer(const float* __restrict__ X, int64_t n, const float* __restrict__ mean,
                 const float* __restrict__ stdv, const short* __restrict__ img, const float* __restrict__ P,
                 const double* __restrict__ dpr, float* out, double* t) {
     __shared__ __align__(16) short s_w[kImgW1 + kImgW2];
-    __shared__ __align__(16) short s_x[kInfWaves][16 * kXRow];
-    __shared__ __align__(16) short s_h[kInfWaves][16 * kHRow];
-    __shared__ float s_b1[Hd], s_b2[Hd], s_w3[Hd], s_mu[F], s_sd[F];
+    __shared__ float s_b1[Hd], s_b2[Hd], s_w3[Hd], s_mu[kK1], s_sd[kK1];
     const int tid = threadIdx.x;
     for (int i = tid; i < (kImgW1 + kImgW2) / 2; i += blockDim.x) ((int*)s_w)[i] = ((const int*)img)[i];
     for (int i = tid; i < Hd; i += blockDim.x) {
@@ -242,9 +243,9 @@ __global__ void __launch_bounds__(64 * kInfWaves)
         s_b2[i] = P[ob2 + i];
         s_w3[i] = from_bf16(img[kImgW1 + kImgW2 + i]);
     }
-    for (int i = tid; i < F; i += blockDim.x) {
-        s_mu[i] = mean[i];
-        s_sd[i] = stdv[i];
+    for (int i = tid; i < kK1; i += blockDim.x) {
+        s_mu[i] = i < F ? mean[i] : 0.f;
+        s_sd[i] = i < F ? stdv[i] : 1.f;
     }
     __syncthreads();
     const float b3 = P[ob3];
@@ -252,76 +253,80 @@ __global__ void __launch_bounds__(64 * kInfWaves)
     const short* s_w2 = s_w + kImgW1;
     const int lane = tid & 63, wv = tid >> 6;
     const int r16 = lane & 15, kq = (lane >> 4) * 4;
-    short* xs = s_x[wv];
-    short* hs = s_h[wv];
-    for (int64_t g0 = ((int64_t)blockIdx.x * kInfWaves + wv) * 16; g0 < n;
-         g0 += (int64_t)gridDim.x * kInfWaves * 16) {
-        for (int j = lane; j < 16 * kK1; j += 64) {
-            const int rr = j / kK1, k = j % kK1;
-            const int64_t row = g0 + rr;
-            float v = 0.f;
-            if (k < F && row < n) v = (X[row * F + k] - s_mu[k]) / s_sd[k];
-            xs[rr * kXRow + k] = to_bf16(v);
-        }
-        wave_sync();
-        f32x4 acc[Hd / 16];
+    for (int64_t g0 = ((int64_t)blockIdx.x * kInfWaves + wv) * 32; g0 < n;
+         g0 += (int64_t)gridDim.x * kInfWaves * 32) {
+        // X fragments (B operand of the transposed layer 1): X[row][ks + kq + j]
+        bf16x4 xf[2][kK1 / 16];
 #pragma unroll
-        for (int nt = 0; nt < Hd / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int h = 0; h < 2; ++h) {
+            const int64_t row = g0 + 16 * h + r16;
 #pragma unroll
-        for (int ks = 0; ks < kK1; ks += 16) {
-            const bf16x4 a = *(const bf16x4*)&xs[r16 * kXRow + ks + kq];
+            for (int ks = 0; ks < kK1; ks += 16) {
+                const int k0 = ks + kq;
+                f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+                if (row < n && k0 < F) v = *(const f32x4*)&X[row * F + k0];
+                bf16x4 b;
 #pragma unroll
-            for (int nt = 0; nt < Hd / 16; ++nt) {
-                const bf16x4 b = *(const bf16x4*)&s_w1[(nt * 16 + r16) * kW1Row + ks + kq];
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+                for (int j = 0; j < 4; ++j) b[j] = k0 + j < F && row < n ? to_bf16((v[j] - s_mu[k0 + j]) / s_sd[k0 + j]) : (short)0;
+                xf[h][ks / 16] = b;
             }
         }
+        // layer 1, transposed: lane holds z1[row 16 h + r16][hidden 16 nt + kq + i]
+        bf16x4 a2[2][Hd / 16];
 #pragma unroll
         for (int nt = 0; nt < Hd / 16; ++nt) {
-            const int col = nt * 16 + r16;
+            f32x4 c0 = f32x4{0.f, 0.f, 0.f, 0.f}, c1 = c0;
+#pragma unroll
+            for (int ks = 0; ks < kK1; ks += 16) {
+                const bf16x4 w = *(const bf16x4*)&s_w1[(nt * 16 + r16) * kW1Row + ks + kq];
+                c0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, xf[0][ks / 16], c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, xf[1][ks / 16], c1, 0, 0, 0);
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float z = acc[nt][i] + s_b1[col];
-                hs[(kq + i) * kHRow + col] = to_bf16(z * sigmoid(z));
+                const float bias = s_b1[nt * 16 + kq + i];
+                a2[0][nt][i] = to_bf16(silu_fast(c0[i] + bias));
+                a2[1][nt][i] = to_bf16(silu_fast(c1[i] + bias));
             }
         }
-        wave_sync();
-#pragma unroll
-        for (int nt = 0; nt < Hd / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < Hd; ks += 16) {
-            const bf16x4 a = *(const bf16x4*)&hs[r16 * kHRow + ks + kq];
-#pragma unroll
-            for (int nt = 0; nt < Hd / 16; ++nt) {
-                const bf16x4 b = *(const bf16x4*)&s_w2[(nt * 16 + r16) * kW2Row + ks + kq];
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-            }
-        }
-        float o[4] = {0.f, 0.f, 0.f, 0.f};
+        // layer 2: lane holds z2[row 16 h + kq + i][hidden 16 nt + r16]
+        float o[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
         for (int nt = 0; nt < Hd / 16; ++nt) {
+            f32x4 c0 = f32x4{0.f, 0.f, 0.f, 0.f}, c1 = c0;
+#pragma unroll
+            for (int ks = 0; ks < Hd / 16; ++ks) {
+                const bf16x4 w = *(const bf16x4*)&s_w2[(nt * 16 + r16) * kW2Row + ks * 16 + kq];
+                c0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[0][ks], w, c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[1][ks], w, c1, 0, 0, 0);
+            }
             const int col = nt * 16 + r16;
+            const float bias = s_b2[col], w3 = s_w3[col];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float z = acc[nt][i] + s_b2[col];
-                o[i] += from_bf16(to_bf16(z * sigmoid(z))) * s_w3[col];
+                o[0][i] += from_bf16(to_bf16(silu_fast(c0[i] + bias))) * w3;
+                o[1][i] += from_bf16(to_bf16(silu_fast(c1[i] + bias))) * w3;
             }
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int d = 1; d < 16; d <<= 1) o[i] += __shfl_xor(o[i], d);
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int d = 1; d < 16; d <<= 1) o[h][i] += __shfl_xor(o[h][i], d);
         if (r16 == 0) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t row = g0 + kq + i;
-                if (row < n) {
-                    const float v = o[i] + b3;
-                    if (out) out[row] = v;
-                    if (t) t[row] = (double)(1.f + kGAmp * tanhf(v)) * dpr[row];
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t row = g0 + 16 * h + kq + i;
+                    if (row < n) {
+                        const float v = o[h][i] + b3;
+                        if (out) out[row] = v;
+                        if (t) t[row] = (double)(1.f + kGAmp * tanhf(v)) * dpr[row];
+                    }
                 }
-            }
         }
-        wave_sync();
     }
 }
 
@@ -422,7 +427,7 @@ void infer(hci_ctx* c, const float* X, int64_t n, float* out, const double* dpr,
     MlpState* s = c->mlp;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_prep_weights, grid_for(kImgW2, 256), 256, 0, st, s->P, s->img);
-    const unsigned grid = (unsigned)std::min<int64_t>(grid_for((n + 15) / 16, kInfWaves), 2048);
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for((n + 31) / 32, kInfWaves), 2048);
     hipLaunchKernelGGL(k_mlp_infer, grid, 64 * kInfWaves, 0, st, X, n, s->mean, s->stdv, s->img, s->P, dpr, out, t);
 }
 
