@@ -554,6 +554,15 @@ __global__ void k_task_remap(TileTask* t, int64_t n, const int32_t* nidx) {
     t[k].w_end = nidx[t[k].w_end];
 }
 
+// entries of each tile (for the order in which tiles are launched)
+__global__ void k_task_work(const TileTask* t, int64_t n, const Window* __restrict__ wins, int64_t* work) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int64_t w = 0;
+    for (int32_t i = t[k].w_begin; i < t[k].w_end; ++i) w += wins[i].cnt();
+    work[k] = w + t[k].nr;  // rows cost their write even without entries
+}
+
 // Stable partition of each tile's windows into those of more than kSmallWin
 // entries, then the rest; sets w_split.  One wavefront per tile.
 constexpr int kPartWaves = 4;
@@ -1235,6 +1244,27 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         HIP_TRY(c, hipGetLastError());
         if (n_win > 0)
             HIP_TRY(c, hipMemcpyAsync(H->d_win, parted, (size_t)n_win * sizeof(Window), hipMemcpyDeviceToDevice, s));
+    }
+    // tiles launch heaviest first (longest-processing-time order), so the
+    // largest alpha groups do not start last and stretch the product's tail
+    for (int part = 0; part < 2; ++part) {
+        std::vector<TileTask>& tv = part == 0 ? ta : tb;
+        TileTask* d_t = part == 0 ? H->d_task_a : H->d_task_b;
+        const int64_t nt = (int64_t)tv.size();
+        if (nt < 2) continue;
+        int64_t* d_work = nullptr;
+        HIP_TRY(c, wb.alloc(&d_work, (size_t)nt));
+        hipLaunchKernelGGL(k_task_work, grid_for(nt, 256), 256, 0, s, d_t, nt, H->d_win, d_work);
+        std::vector<int64_t> work((size_t)nt);
+        HIP_TRY(c, hipMemcpyAsync(work.data(), d_work, (size_t)nt * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipMemcpyAsync(tv.data(), d_t, (size_t)nt * sizeof(TileTask), hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        std::vector<int64_t> order((size_t)nt);
+        for (int64_t k = 0; k < nt; ++k) order[(size_t)k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return work[(size_t)a] > work[(size_t)b]; });
+        std::vector<TileTask> sorted((size_t)nt);
+        for (int64_t k = 0; k < nt; ++k) sorted[(size_t)k] = tv[(size_t)order[(size_t)k]];
+        HIP_TRY(c, hipMemcpy(d_t, sorted.data(), (size_t)nt * sizeof(TileTask), hipMemcpyHostToDevice));
     }
     HIP_TRY(c, hipMalloc(&H->d_bpos, (size_t)N * 4 + 16));
     HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
