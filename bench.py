@@ -19,8 +19,10 @@ root, residual 1e-4 and energy change 1e-8).  Workloads (--config):
                --n-dets determinants (default 8e6, the largest whose FP32
                hi+lo H fits one GPU's HBM next to the build's scratch).
 
-Timed path (value): tiled storage with FP32 hi+lo values, Davidson with FP32
-sigmas and FP64 refinement, restarts that keep the previous Ritz vector, and the
+Timed path (value): tiled storage with FP32 hi+lo values (plus an FP16 copy of
+each value for workload 1: --values), Davidson with FP16 sigmas, then FP32
+sigmas from 3x the tolerance on, then FP64 refinement (FP32 then FP64 for
+workloads 2 and 4), restarts that keep the previous Ritz vector, and the
 correction chosen by --precond.  Reported beside it from the same run: the
 plain FP64 GPU path (FP64 values, the reference's restart-to-one-vector control
 flow, diagonal correction) and its E0, the other correction, the H-build
@@ -139,6 +141,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the untimed comparison runs (FP64 reference path, other preconditioner)")
+    ap.add_argument("--values", choices=["f16", "f32"], default=None,
+                    help="lowest value precision of the timed path: f16 (FP16 copy, FP16 -> FP32 -> FP64 "
+                         "sigmas) or f32 (FP32 -> FP64); default f16 for workload 1, f32 otherwise")
     ap.add_argument("--precond", choices=["mlp", "dpr"], default=None,
                     help="correction of the timed path (the other one is reported beside it); "
                          "default mlp, dpr for the Hubbard workload")
@@ -150,7 +155,7 @@ def main() -> int:
     rank = int(os.environ.get("RANK", "0"))
 
     from risq_amd import precond_train
-    from risq_amd.hci import (PREC_F32, PREC_F64, PRECOND_DPR, PRECOND_MLP, STORE_TILED, DeviceBuffer,
+    from risq_amd.hci import (PREC_F16, PREC_F32, PREC_F64, PRECOND_DPR, PRECOND_MLP, STORE_TILED, DeviceBuffer,
                               HciContext)
 
     t0 = time.perf_counter()
@@ -181,8 +186,14 @@ def main() -> int:
         ctx.sync()
         return h, res, tb, time.perf_counter() - ts
 
+    if args.values is None:
+        # workload 4's matrix leaves no room for the FP16 copy; the lattice's
+        # products are bound by per-row work, which FP16 values do not shrink
+        args.values = "f16" if args.config == 1 else "f32"
+    fast_prec, fast_mixed = (PREC_F16, 2) if args.values == "f16" else (PREC_F32, 1)
+
     def step():
-        return run(PREC_F32, True, 2, precond)
+        return run(fast_prec, fast_mixed, 2, precond)
 
     for w in range(args.warmup):
         h, res, tb, tt = step()
@@ -204,9 +215,10 @@ def main() -> int:
     nnz = h.nnz
     h.free(recycle=True)
     last = results[-1].stats
-    n32, nfull = last["n_sigma_f32"], last["n_sigma_full"]
-    # FP32 sigmas read col + hi (8 B/entry), refinement sigmas col + hi + lo (12 B)
-    alg_bytes = n32 * spmv_bytes(nnz, n, 4) + nfull * spmv_bytes(nnz, n, 8)
+    n16, n32, nfull = last["n_sigma_f16"], last["n_sigma_f32"], last["n_sigma_full"]
+    # upper-triangle algorithmic bytes: FP16 sigmas col + FP16 value (6 B per
+    # pair), FP32 sigmas col + hi (8 B), refinement sigmas col + hi + lo (12 B)
+    alg_bytes = (n16 * spmv_bytes(nnz, n, 2) + n32 * spmv_bytes(nnz, n, 4) + nfull * spmv_bytes(nnz, n, 8))
     achieved = alg_bytes / (last["sigma_ms"] * 1e-3) / 1e9
     out = {
         "metric": "s to E0 within 1µHa (plain FP64 vs FP32+learned precond); H-build nnz/s; SpMV GB/s",
@@ -219,23 +231,27 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 values + f64 accumulation, f64 refinement",
+        "dtype": ("f16 then f32 values" if args.values == "f16" else "f32 values")
+                 + " + f64 accumulation, f64 refinement",
         "data": "synthetic (seeded integrals; deterministic determinant space)",
         "config": {"workload": f"{workload}; H build + Davidson to E0 (tol 1e-4, etol 1e-8)",
-                   "n_dets": n, "nnz_u": nnz, "storage": "tiled, fp32 hi+lo",
+                   "n_dets": n, "nnz_u": nnz,
+                   "storage": "tiled, fp32 hi+lo" + (" + fp16 copy" if args.values == "f16" else ""),
                    "davidson": f"mixed precision, restart keeps 2 vectors, correction={args.precond}",
                    "parallelism": "single GPU"},
         "E0": results[-1].energy,
-        "davidson": {"iters": last["iters"], "n_sigma_f32": n32, "n_sigma_full": nfull,
+        "davidson": {"iters": last["iters"], "n_sigma_f16": n16, "f16_iter": last["f16_iter"],
+                     "n_sigma_f32": n32, "n_sigma_full": nfull,
                      "seconds": last["seconds"], "converged": bool(last["converged"]),
                      "mlp_applied": last["mlp_applied"], "mlp_fallbacks": last["mlp_fallbacks"]},
         "h_build_nnz_per_s": nnz / float(np.mean(builds)),
         "h_build_s": float(np.mean(builds)),
-        "spmv_ms": last["sigma_ms"] / max(1, n32 + nfull),
+        "spmv_ms": last["sigma_ms"] / max(1, n16 + n32 + nfull),
         "spmv_gbps": achieved,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmv_win (sigma product: FP32-value and FP32 hi+lo launches)"},
+                     "kernel": "k_spmv_win (sigma product: all its launches in the timed step: "
+                               "FP16-value, FP32-value and FP32 hi+lo)"},
     }
     if not args.no_compare:
         h, ref, tb, tr = run(PREC_F64, False, 1, PRECOND_DPR)
@@ -252,11 +268,12 @@ def main() -> int:
         out["abs_dE_vs_fp64_reference"] = abs(ref.energy - out["E0"])
         log(f"fp64 reference path: {tr:.3f}s E0={ref.energy:.10f} iters={st['iters']} sigma {r_ms:.2f} ms")
         other = PRECOND_DPR if precond == PRECOND_MLP else PRECOND_MLP
-        h, alt, tb, ta = run(PREC_F32, True, 2, other)
+        h, alt, tb, ta = run(fast_prec, fast_mixed, 2, other)
         h.free()
         out["other_correction"] = {
             "correction": "dpr" if other == PRECOND_DPR else "mlp", "s_to_E0": round(ta, 4),
-            "E0": alt.energy, "iters": alt.stats["iters"], "n_sigma_f32": alt.stats["n_sigma_f32"],
+            "E0": alt.energy, "iters": alt.stats["iters"], "n_sigma_f16": alt.stats["n_sigma_f16"],
+            "n_sigma_f32": alt.stats["n_sigma_f32"],
             "n_sigma_full": alt.stats["n_sigma_full"], "mlp_applied": alt.stats["mlp_applied"],
             "mlp_fallbacks": alt.stats["mlp_fallbacks"]}
         log(f"other correction: {ta:.3f}s E0={alt.energy:.10f} {alt.stats}")

@@ -58,6 +58,9 @@ extern "C" {
 #define HCI_PREC_F64 64 /* int32 col + float64 value (12 B / nonzero) */
 #define HCI_PREC_F32 32 /* int32 col + float32 hi + float32 lo (hi read alone in
                            mixed sigmas, hi+lo in refinement sigmas) */
+#define HCI_PREC_F16 16 /* HCI_STORE_TILED only: FP32 hi + lo as for HCI_PREC_F32 plus an
+                           FP16 copy of every value (2 B more per entry), read alone in
+                           the first phase of mixed = 2 Davidson runs */
 
 /* how the matrix is held on the device (both keep the reference's pair set;
  * rows/columns are renumbered internally in alpha-major string order) */
@@ -80,6 +83,7 @@ extern "C" {
 /* sigma precision */
 #define HCI_SIGMA_FULL 0 /* stored values at full precision */
 #define HCI_SIGMA_F32 1  /* FP32 values (hi word), FP64 x/y and accumulation */
+#define HCI_SIGMA_F16 2  /* FP16 values (HCI_PREC_F16 storage), FP64 x/y and accumulation */
 
 /* Davidson correction */
 #define HCI_PRECOND_DPR 0 /* t = r / (lambda - H_ii), davidson.rs:391-408 */
@@ -89,7 +93,7 @@ typedef struct hci_ctx hci_ctx;
 typedef struct hci_csr hci_csr;
 
 typedef struct hci_build_opts {
-    int value_prec;  /* HCI_PREC_F64 | HCI_PREC_F32 */
+    int value_prec;  /* HCI_PREC_F64 | HCI_PREC_F32 | HCI_PREC_F16 (tiled) */
     int single_rule; /* HCI_SINGLES_* */
     int mem;         /* where dets/diag live */
     int storage;     /* HCI_STORE_FULL | HCI_STORE_UPPER | HCI_STORE_TILED */
@@ -113,7 +117,8 @@ typedef struct hci_dav_opts {
     int max_sub;       /* subspace limit, reference nvalues + 10 = 11 (davidson.rs:64-68);
                           at most HCI_DAV_MAX_SUB */
     int precond;       /* HCI_PRECOND_* */
-    int mixed;         /* 0: full-precision sigmas; 1: FP32 sigmas, then FP64 refinement */
+    int mixed;         /* 0: full-precision sigmas; 1: FP32 sigmas, then FP64 refinement;
+                          2 (HCI_PREC_F16 storage): FP16 sigmas, then FP32, then refinement */
     double refine_tol; /* mixed: residual below which refinement starts; 0 (default):
                           when the FP32 iteration has converged or stalls */
     int mem;           /* where init / coeffs live */
@@ -122,6 +127,9 @@ typedef struct hci_dav_opts {
                           m - 1 lowest Ritz vectors plus the previous iteration's Ritz
                           vector, formed inside the subspace with no new product;
                           m < max_sub, m <= HCI_DAV_MAX_KEEP */
+    double f16_tol;    /* mixed = 2: residual below which the FP16 phase gives way to FP32
+                          sigmas (restart from the Ritz vector), also when the FP16
+                          iteration converges or stalls; 0 (default): 3 tol */
 } hci_dav_opts;
 
 typedef struct hci_dav_stats {
@@ -135,6 +143,8 @@ typedef struct hci_dav_stats {
     double residual;    /* final |r| */
     double seconds;
     double sigma_ms;    /* summed device time of the sigma products (HIP events) */
+    int n_sigma_f16;    /* FP16-value sigma products (mixed = 2) */
+    int f16_iter;       /* iteration at which FP32 sigmas took over (-1: never) */
 } hci_dav_stats;
 
 /* context ---------------------------------------------------------------- */

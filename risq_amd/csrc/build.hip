@@ -547,6 +547,13 @@ __global__ void __launch_bounds__(64 * kIlWaves)
     }
 }
 
+// FP16 copy of every slot, rounded to nearest from the FP32 hi + lo value
+// (alignment gaps hold whatever the block held; the product never uses them)
+__global__ void k_make_h16(const float* __restrict__ hi, const float* __restrict__ lo, int64_t n, uint16_t* h16) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        h16[e] = __builtin_bit_cast(uint16_t, (_Float16)((double)hi[e] + (double)lo[e]));
+}
+
 __global__ void k_task_remap(TileTask* t, int64_t n, const int32_t* nidx) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -1209,8 +1216,9 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (o.value_prec == HCI_PREC_F64) {
         HIP_TRY(c, recycled_take(c, (void**)&H->d_val, cap * 8));
     } else {
-        HIP_TRY(c, recycled_take(c, (void**)&H->d_hi, cap * 8));
+        HIP_TRY(c, recycled_take(c, (void**)&H->d_hi, cap * (o.value_prec == HCI_PREC_F16 ? 10 : 8)));
         H->d_lo = H->d_hi + cap;
+        if (o.value_prec == HCI_PREC_F16) H->d_h16 = (uint16_t*)(H->d_lo + cap);
     }
     HIP_TRY(c, recycled_take(c, (void**)&H->d_idx16, cap * 2));
     recycled_flush(c);
@@ -1224,6 +1232,9 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (n_win > 0)
         hipLaunchKernelGGL(k_interleave, std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u), 64 * kIlWaves, 0, s,
                            H->d_win, (int64_t)n_win, H->d_idx16, H->d_val, H->d_hi, H->d_lo);
+    if (H->d_h16)
+        hipLaunchKernelGGL(k_make_h16, std::min<unsigned>(grid_for((int64_t)cap, 256), 65536u), 256, 0, s, H->d_hi,
+                           H->d_lo, (int64_t)cap, H->d_h16);
     if (!ta.empty())
         hipLaunchKernelGGL(k_task_remap, grid_for((int64_t)ta.size(), 256), 256, 0, s, H->d_task_a,
                            (int64_t)ta.size(), d_nidx);
@@ -1609,8 +1620,10 @@ extern "C" int hci_build_h(hci_ctx* c, const uint64_t* dets, const double* diag,
     if (!c->d_two) return hci_fail(c, HCI_ERR_STATE, "hci_build_h: integrals not set");
     if (n < 1 || n >= (int64_t)INT32_MAX || !dets || !diag)
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: need 1 <= n < 2^31 and dets/diag");
-    if (opts->value_prec != HCI_PREC_F64 && opts->value_prec != HCI_PREC_F32)
-        return hci_fail(c, HCI_ERR_ARG, "hci_build_h: value_prec must be 64 or 32");
+    if (opts->value_prec != HCI_PREC_F64 && opts->value_prec != HCI_PREC_F32 && opts->value_prec != HCI_PREC_F16)
+        return hci_fail(c, HCI_ERR_ARG, "hci_build_h: value_prec must be 64, 32 or 16");
+    if (opts->value_prec == HCI_PREC_F16 && opts->storage != HCI_STORE_TILED)
+        return hci_fail(c, HCI_ERR_ARG, "hci_build_h: value_prec 16 needs HCI_STORE_TILED");
     if (opts->single_rule != HCI_SINGLES_ABS_GT_1E12 && opts->single_rule != HCI_SINGLES_NONZERO)
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: bad single_rule");
     if (opts->storage != HCI_STORE_FULL && opts->storage != HCI_STORE_UPPER && opts->storage != HCI_STORE_TILED)

@@ -485,11 +485,12 @@ int hci_csr_recycle(hci_csr* h) {
         recycled_flush(c);
         const size_t cap = h->entry_cap;
         c->recycled.push_back({h->d_idx16, cap * 2});
-        c->recycled.push_back({h->d_val ? (void*)h->d_val : (void*)h->d_hi, cap * 8});
+        c->recycled.push_back({h->d_val ? (void*)h->d_val : (void*)h->d_hi, cap * (h->d_h16 ? 10 : 8)});
         h->d_idx16 = nullptr;
         h->d_val = nullptr;
         h->d_hi = nullptr;
         h->d_lo = nullptr;
+        h->d_h16 = nullptr;
     }
     hci_csr_free(h);
     return HCI_OK;

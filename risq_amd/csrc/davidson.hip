@@ -354,6 +354,7 @@ extern "C" void hci_dav_default_opts(hci_dav_opts* o) {
     o->refine_tol = 0.0;
     o->mem = HCI_MEM_HOST;
     o->restart_keep = 1;
+    o->f16_tol = 0.0;
 }
 
 static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const hci_dav_opts* opts,
@@ -369,7 +370,13 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         return hci_fail(c, HCI_ERR_ARG, "hci_davidson: restart_keep must be in [1, min(8, max_sub - 1)]");
     if (opts->precond == HCI_PRECOND_MLP && !mlp_ready(c))
         return hci_fail(c, HCI_ERR_STATE, "hci_davidson: MLP preconditioner requested but no network loaded");
-    const bool use_f32 = opts->mixed && h->prec == HCI_PREC_F32;
+    if (opts->mixed == 2 && h->prec != HCI_PREC_F16)
+        return hci_fail(c, HCI_ERR_ARG, "hci_davidson: mixed = 2 needs HCI_PREC_F16 storage");
+    const bool use_f32 = opts->mixed && (h->prec == HCI_PREC_F32 || h->prec == HCI_PREC_F16);
+    const bool use_f16 = opts->mixed == 2;
+    // FP16 rounding (2^-11 of each element) leaves the residual a few times
+    // tol from the full-precision one: FP32 sigmas take over at 3 tol
+    const double f16_tol = opts->f16_tol > 0.0 ? opts->f16_tol : 3.0 * opts->tol;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
@@ -395,7 +402,8 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
 
     hci_dav_stats st{};
     st.refine_iter = -1;
-    int sigma_mode = use_f32 ? HCI_SIGMA_F32 : HCI_SIGMA_FULL;
+    st.f16_iter = -1;
+    int sigma_mode = use_f16 ? HCI_SIGMA_F16 : use_f32 ? HCI_SIGMA_F32 : HCI_SIGMA_FULL;
     // every sigma product is bracketed by events on the context stream
     std::vector<hipEvent_t> ev;
     struct EvFree {
@@ -413,7 +421,9 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         hipEventRecord(a, s);
         int rc = spmv_device(h, v, out, sigma_mode);
         hipEventRecord(b, s);
-        if (sigma_mode == HCI_SIGMA_F32)
+        if (sigma_mode == HCI_SIGMA_F16)
+            ++st.n_sigma_f16;
+        else if (sigma_mode == HCI_SIGMA_F32)
             ++st.n_sigma_f32;
         else
             ++st.n_sigma_full;
@@ -499,6 +509,22 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         }
         // FP32-value sigmas converge on the rounded matrix; refine once that
         // iteration converges, the residual reaches refine_tol, or it stalls
+        // FP16-value sigmas first (mixed = 2): FP32 ones take over from the
+        // Ritz vector once the FP16 iteration converges, reaches f16_tol, or
+        // stalls at the FP16 rounding floor
+        if (sigma_mode == HCI_SIGMA_F16 && (conv || err < f16_tol || since_best >= 8)) {
+            sigma_mode = HCI_SIGMA_F32;
+            st.f16_iter = it;
+            dim_sub = 1;
+            HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
+            DT(rebuild_projection());
+            err_prev = 1e300;
+            best_err = 1e300;
+            since_best = 0;
+            mlp_prev = false;
+            y_prev.assign(1, 1.0);
+            continue;
+        }
         if (sigma_mode == HCI_SIGMA_F32 && (conv || err < opts->refine_tol || since_best >= 25)) {
             // switch to full-precision sigmas: restart from the Ritz vector
             sigma_mode = HCI_SIGMA_FULL;

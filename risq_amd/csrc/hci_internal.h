@@ -122,6 +122,7 @@ struct hci_csr {
     double* d_val = nullptr;       // nnz_s (HCI_PREC_F64)
     float* d_hi = nullptr;         // nnz_s (HCI_PREC_F32)
     float* d_lo = nullptr;         // nnz_s (HCI_PREC_F32)
+    uint16_t* d_h16 = nullptr;     // FP16 copies (HCI_PREC_F16, TILED; inside the value block)
     double* d_diag = nullptr;      // n, internal order
     uint64_t* d_words = nullptr;   // 4n: determinant words in internal order
     // nnz-balanced SpMV work split: row containing the first entry of every chunk

@@ -222,12 +222,16 @@ template <int MODE>
 struct Oct {
     u16x8 u;
     f64x2 d[MODE == 0 ? 4 : 1];
-    f32x4 h[MODE == 0 ? 1 : 2];
+    f32x4 h[MODE == 0 || MODE == 3 ? 1 : 2];
     f32x4 l[MODE == 2 ? 2 : 1];
+    u16x8 q;  // MODE 3: FP16 values
     __device__ __forceinline__ void load(const uint16_t* idx, const double* val, const float* hi, const float* lo,
                                          int64_t e) {
         u = __builtin_nontemporal_load((const u16x8*)(idx + e));
-        if (MODE == 0) {
+        if (MODE == 3) {
+            // MODE 3 passes the FP16 copies in place of the FP64 values
+            q = __builtin_nontemporal_load((const u16x8*)((const uint16_t*)val + e));
+        } else if (MODE == 0) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) d[k] = __builtin_nontemporal_load((const f64x2*)(val + e + 2 * k));
         } else {
@@ -238,10 +242,11 @@ struct Oct {
             }
         }
     }
-    __device__ __forceinline__ double value(int q) const {
-        if (MODE == 0) return d[q >> 1][q & 1];
-        if (MODE == 1) return (double)h[q >> 2][q & 3];
-        return (double)h[q >> 2][q & 3] + (double)l[q >> 2][q & 3];
+    __device__ __forceinline__ double value(int k) const {
+        if (MODE == 3) return (double)(float)__builtin_bit_cast(_Float16, (uint16_t)q[k]);
+        if (MODE == 0) return d[k >> 1][k & 1];
+        if (MODE == 1) return (double)h[k >> 2][k & 3];
+        return (double)h[k >> 2][k & 3] + (double)l[k >> 2][k & 3];
     }
 };
 
@@ -449,6 +454,8 @@ template <int MODE>
 void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
     hipStream_t s = h->ctx->stream;
     if (h->storage == HCI_STORE_TILED) {
+        // MODE 3 reads the FP16 copies through the value pointer
+        const double* vals = MODE == 3 ? (const double*)h->d_h16 : h->d_val;
         // beta-major part first (x gathered to beta-major order), then the
         // alpha-major part writes y, then the beta-major rows are added
         if (h->n_task_b > 0) {
@@ -456,21 +463,22 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
                 hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, s, h->d_bpos, x, h->n, h->d_xb);
             if (absval)
                 hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);
+                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);
             else
                 hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, h->d_xb, nullptr, h->d_yb);
+                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, h->d_xb, nullptr, h->d_yb);
         }
         if (absval)
             hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, nullptr, nullptr, y);
+                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y);
         else
             hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, h->d_val, h->d_hi, h->d_lo, x, h->d_diag, y);
+                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, x, h->d_diag, y);
         if (h->n_task_b > 0)
             hipLaunchKernelGGL(k_scatter_add, grid_for(h->n, 256), 256, 0, s, h->d_bpos, h->d_yb, h->n, y);
         return;
     }
+    if constexpr (MODE == 3) return;  // FP16 copies exist in tiled storage only
     if (h->storage == HCI_STORE_FULL) {
         const unsigned grid = grid_for(h->n_items, 4);
         if (absval)
@@ -491,8 +499,11 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
 
 int spmv_device(const hci_csr* h, const double* x, double* y, int mode) {
     hci_ctx* c = h->ctx;
+    if (mode == HCI_SIGMA_F16 && !h->d_h16) return hci_fail(c, HCI_ERR_ARG, "sigma: FP16 mode needs HCI_PREC_F16 storage");
     if (h->prec == HCI_PREC_F64)
         launch_spmv<0>(h, x, y, false);
+    else if (mode == HCI_SIGMA_F16)
+        launch_spmv<3>(h, x, y, false);
     else if (mode == HCI_SIGMA_F32)
         launch_spmv<1>(h, x, y, false);
     else
@@ -519,8 +530,10 @@ extern "C" int hci_sigma(const hci_csr* h, int ncol, const double* x, double* y,
     if (!h) return HCI_ERR_ARG;
     hci_ctx* c = h->ctx;
     if (ncol < 0 || (ncol > 0 && (!x || !y))) return hci_fail(c, HCI_ERR_ARG, "hci_sigma: bad arguments");
-    if (sigma_mode != HCI_SIGMA_FULL && sigma_mode != HCI_SIGMA_F32)
+    if (sigma_mode != HCI_SIGMA_FULL && sigma_mode != HCI_SIGMA_F32 && sigma_mode != HCI_SIGMA_F16)
         return hci_fail(c, HCI_ERR_ARG, "hci_sigma: bad sigma_mode");
+    if (sigma_mode == HCI_SIGMA_F16 && !h->d_h16)
+        return hci_fail(c, HCI_ERR_ARG, "hci_sigma: HCI_SIGMA_F16 needs HCI_PREC_F16 storage");
     HIP_TRY(c, hipSetDevice(c->device));
     const size_t bytes = (size_t)h->n * 8;
     double *dx = nullptr, *dy = nullptr, *xi = nullptr, *yi = nullptr;

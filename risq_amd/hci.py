@@ -28,10 +28,12 @@ MEM_HOST = 0
 MEM_DEVICE = 1
 PREC_F64 = 64
 PREC_F32 = 32
+PREC_F16 = 16
 SINGLES_ABS_GT_1E12 = 0
 SINGLES_NONZERO = 1
 SIGMA_FULL = 0
 SIGMA_F32 = 1
+SIGMA_F16 = 2
 PRECOND_DPR = 0
 PRECOND_MLP = 1
 STORE_FULL = 0
@@ -57,14 +59,15 @@ class BuildStats(C.Structure):
 class DavOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("etol", C.c_double), ("max_iters", C.c_int),
                 ("max_sub", C.c_int), ("precond", C.c_int), ("mixed", C.c_int),
-                ("refine_tol", C.c_double), ("mem", C.c_int), ("restart_keep", C.c_int)]
+                ("refine_tol", C.c_double), ("mem", C.c_int), ("restart_keep", C.c_int),
+                ("f16_tol", C.c_double)]
 
 
 class DavStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("n_sigma_f32", C.c_int), ("n_sigma_full", C.c_int),
                 ("converged", C.c_int), ("mlp_applied", C.c_int), ("mlp_fallbacks", C.c_int),
                 ("refine_iter", C.c_int), ("residual", C.c_double), ("seconds", C.c_double),
-                ("sigma_ms", C.c_double)]
+                ("sigma_ms", C.c_double), ("n_sigma_f16", C.c_int), ("f16_iter", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -392,13 +395,16 @@ class HciContext:
 
     def davidson(self, h: "SparseMatUpperTri", init: np.ndarray | None = None, tol: float = 1e-4,
                  etol: float = 1e-8, max_iters: int = 50, max_sub: int = 11,
-                 precond: int = PRECOND_DPR, mixed: bool = False,
-                 refine_tol: float = 0.0, restart_keep: int = 1) -> "DavidsonResult":
+                 precond: int = PRECOND_DPR, mixed: int = 0,
+                 refine_tol: float = 0.0, restart_keep: int = 1, f16_tol: float = 0.0) -> "DavidsonResult":
+        """mixed: 0 full-precision sigmas; 1 (or True) FP32 then refinement;
+        2 FP16 (HCI_PREC_F16 storage) then FP32 then refinement."""
         opts = DavOpts()
         self.lib.hci_dav_default_opts(C.byref(opts))
         opts.tol, opts.etol, opts.max_iters, opts.max_sub = tol, etol, max_iters, max_sub
         opts.precond, opts.mixed, opts.refine_tol, opts.mem = precond, int(mixed), refine_tol, MEM_HOST
         opts.restart_keep = restart_keep
+        opts.f16_tol = f16_tol
         coeffs = np.zeros(h.n)
         e0 = C.c_double(0.0)
         st = DavStats()

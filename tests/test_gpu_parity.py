@@ -391,3 +391,31 @@ def test_tiled_hubbard_large_groups(gpu_ctx, oracle_lib, prec):
     for mode in ([0, 1] if prec == 32 else [0]):
         a, b = f.matrix_vector_prod(x, mode=mode), t.matrix_vector_prod(x, mode=mode)
         assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
+
+
+def test_fp16_copy_and_three_phase_davidson(gpu_ctx, oracle_lib):
+    """HCI_PREC_F16 tiled storage: the FP32 hi + lo values download unchanged,
+    FP16-value products stay within FP16 rounding of the FP64 product, and the
+    FP16 -> FP32 -> refinement Davidson converges to the FP64 eigenvalue."""
+    from risq_amd.hci import PREC_F16, SIGMA_F16
+    ints, dets = mol_space(*MOL0)
+    gpu_ctx.set_integrals(ints)
+    diag = gpu_ctx.ham_diag(dets)
+    f32 = gpu_ctx.build_h(dets, diag, value_prec=32, storage=STORE_TILED)
+    f16 = gpu_ctx.build_h(dets, diag, value_prec=PREC_F16, storage=STORE_TILED)
+    for u, v in zip(f32.download(), f16.download()):
+        assert np.array_equal(u, v)
+    x = np.random.default_rng(11).standard_normal(dets.shape[0])
+    ref = f32.matrix_vector_prod(x)
+    y16 = f16.matrix_vector_prod(x, mode=SIGMA_F16)
+    rel = np.linalg.norm(y16 - ref) / np.linalg.norm(ref - diag * x)
+    assert 0 < rel < 2e-3, rel
+    H = csr_to_dense(*_oracle_build(oracle_lib, ints, dets, diag).export())
+    e_ref = np.linalg.eigvalsh(H)[0]
+    r = gpu_ctx.davidson(f16, tol=1e-5, etol=1e-9, max_iters=400, mixed=2, restart_keep=2)
+    assert r.converged and abs(r.energy - e_ref) < 1e-6, (r.energy, e_ref)
+    assert r.stats["n_sigma_f16"] > 0 and r.stats["f16_iter"] >= 0 and r.stats["n_sigma_full"] > 0
+    with pytest.raises(HciError, match="needs HCI_PREC_F16"):
+        gpu_ctx.davidson(f32, mixed=2)
+    with pytest.raises(HciError, match="needs HCI_STORE_TILED"):
+        gpu_ctx.build_h(dets, diag, value_prec=PREC_F16)
