@@ -147,6 +147,8 @@ struct hci_csr {
     int32_t* d_bpos = nullptr;     // beta-major position -> internal position
     double* d_xb = nullptr;        // work vectors in beta-major order
     double* d_yb = nullptr;
+    float* d_x32 = nullptr;        // FP32 copies of x for FP16-value products (HCI_PREC_F16)
+    float* d_xb32 = nullptr;
 };
 
 // recycled entry arrays (ctx.hip): recycled_take hands out a kept buffer of at

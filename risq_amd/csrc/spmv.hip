@@ -14,6 +14,8 @@
 //     atomic (order-dependent rounding).
 //   TILED storage: k_spmv_win, x slices staged in LDS, rows summed in LDS.
 // y starts as d o x in both cases.
+#include <type_traits>
+
 #include "hci_internal.h"
 
 namespace {
@@ -250,12 +252,17 @@ struct Oct {
     }
 };
 
-template <bool ABS>
-__device__ __forceinline__ void add_entry(double* acc, const double* xw, uint32_t u, double v) {
+// x as the product reads it: FP64, or FP32 with FP16 values (MODE 3), whose
+// own rounding is 2^-11
+template <int MODE>
+using XT = typename std::conditional<MODE == 3, float, double>::type;
+
+template <bool ABS, class X>
+__device__ __forceinline__ void add_entry(double* acc, const X* xw, uint32_t u, double v) {
     if (ABS)
         atomicAdd(&acc[u >> 8], fabs(v));
     else
-        atomicAdd(&acc[u >> 8], v * xw[u & 0xFFu]);
+        atomicAdd(&acc[u >> 8], v * (double)xw[u & 0xFFu]);
 }
 
 template <int MODE, bool ABS>
@@ -263,13 +270,14 @@ __global__ void __launch_bounds__(kTileThreads)
     k_spmv_win(const TileTask* __restrict__ tasks, const Window* __restrict__ wins,
                const uint16_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
                const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
-               double* __restrict__ y) {
+               double* __restrict__ y, const double* __restrict__ xdiag) {
     __shared__ double acc[kTileRows];
-    __shared__ double xs[kTileWaves][kWinCols];
+    __shared__ XT<MODE> xs[kTileWaves][kWinCols];
     __shared__ int next, next_small;
     const TileTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63;
-    double* xw = xs[tid >> 6];
+    XT<MODE>* xw = xs[tid >> 6];
+    const XT<MODE>* xx = (const XT<MODE>*)x;
     for (int r = tid; r < kTileRows; r += kTileThreads) acc[r] = 0.0;
     if (tid == 0) {
         next = 0;
@@ -306,11 +314,11 @@ __global__ void __launch_bounds__(kTileThreads)
             o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
         if (n_full > 1) o1.load(idx, val, hi, lo, mine + kWinChunk);
         if (!ABS) {
-            double xv[kWinCols / 64];
+            XT<MODE> xv[kWinCols / 64];
 #pragma unroll
             for (int j = 0; j < kWinCols / 64; ++j) {
                 const int c = lane + 64 * j;
-                xv[j] = c < nc ? x[W.c0 + c] : 0.0;
+                xv[j] = c < nc ? xx[W.c0 + c] : (XT<MODE>)0;
             }
 #pragma unroll
             for (int j = 0; j < kWinCols / 64; ++j) xw[lane + 64 * j] = xv[j];
@@ -370,7 +378,7 @@ __global__ void __launch_bounds__(kTileThreads)
                         if (ABS)
                             atomicAdd(&acc[u >> 8], fabs(v));
                         else
-                            atomicAdd(&acc[u >> 8], v * x[Ws.c0 + (int)(u & 0xFFu)]);
+                            atomicAdd(&acc[u >> 8], v * (double)xx[Ws.c0 + (int)(u & 0xFFu)]);
                     }
             }
         }
@@ -378,13 +386,24 @@ __global__ void __launch_bounds__(kTileThreads)
     __syncthreads();
     for (int r = tid; r < t.nr; r += kTileThreads) {
         const int64_t p = (int64_t)t.r0 + r;
-        y[p] = (diag && !ABS) ? diag[p] * x[p] + acc[r] : acc[r];
+        y[p] = (diag && !ABS) ? diag[p] * xdiag[p] + acc[r] : acc[r];  // xdiag: x in FP64
     }
 }
 
 __global__ void k_scatter_add(const int32_t* pos, const double* src, int64_t n, double* dst) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) dst[pos[q]] += src[q];
+}
+
+template <class T>
+__global__ void k_gather_x(const int32_t* perm, const T* src, int64_t n, T* dst) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[perm[p]];
+}
+
+__global__ void k_to_f32(const double* x, int64_t n, float* out) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = (float)x[p];
 }
 
 __global__ void k_gather(const int32_t* perm, const double* src, int64_t n, double* dst) {
@@ -454,26 +473,35 @@ template <int MODE>
 void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
     hipStream_t s = h->ctx->stream;
     if (h->storage == HCI_STORE_TILED) {
-        // MODE 3 reads the FP16 copies through the value pointer
+        // MODE 3 reads the FP16 copies through the value pointer, and FP32
+        // copies of x (in both orders) through the x pointers
         const double* vals = MODE == 3 ? (const double*)h->d_h16 : h->d_val;
+        const double* xa = x;
+        double* xb = h->d_xb;
+        if (MODE == 3 && !absval) {
+            hipLaunchKernelGGL(k_to_f32, grid_for(h->n, 256), 256, 0, s, x, h->n, h->d_x32);
+            xa = (const double*)h->d_x32;
+            xb = (double*)h->d_xb32;
+        }
         // beta-major part first (x gathered to beta-major order), then the
         // alpha-major part writes y, then the beta-major rows are added
         if (h->n_task_b > 0) {
             if (!absval)
-                hipLaunchKernelGGL(k_gather, grid_for(h->n, 256), 256, 0, s, h->d_bpos, x, h->n, h->d_xb);
+                hipLaunchKernelGGL(k_gather_x<XT<MODE>>, grid_for(h->n, 256), 256, 0, s, h->d_bpos,
+                                   (const XT<MODE>*)xa, h->n, (XT<MODE>*)xb);
             if (absval)
                 hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb);
+                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb, nullptr);
             else
                 hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, h->d_xb, nullptr, h->d_yb);
+                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xb, nullptr, h->d_yb, nullptr);
         }
         if (absval)
             hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y);
+                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr);
         else
             hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, x, h->d_diag, y);
+                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xa, h->d_diag, y, x);
         if (h->n_task_b > 0)
             hipLaunchKernelGGL(k_scatter_add, grid_for(h->n, 256), 256, 0, s, h->d_bpos, h->d_yb, h->n, y);
         return;
