@@ -497,11 +497,16 @@ __global__ void k_win_compact(const int64_t* wc, const int64_t* woff, const int3
 // shorter) is read in enumeration order into the wave's LDS and written back
 // slot by slot (the inverse of win_slot, whose slots stay inside the piece's
 // own range), so both passes over memory are contiguous.  A window
-// of at most 8 entries is one group whose slots are its positions.
+// of at most 8 entries is one group whose slots are its positions.  With
+// FP16 storage each value's FP16 copy is written at its slot on the way.
 constexpr int kIlWaves = 4;
+// FP16 copy of a value (HCI_PREC_F16), rounded to nearest from FP32 hi + lo
+__device__ __forceinline__ uint16_t fp16_of(float hi, float lo) {
+    return __builtin_bit_cast(uint16_t, (_Float16)((double)hi + (double)lo));
+}
 __global__ void __launch_bounds__(64 * kIlWaves)
     k_interleave(const Window* __restrict__ wins, int64_t n_win, uint16_t* idx, double* val, float* hi,
-                 float* lo) {
+                 float* lo, uint16_t* h16) {
     __shared__ uint16_t si[kIlWaves][kWinChunk];
     __shared__ double sv[kIlWaves][kWinChunk];
     __shared__ float sh[kIlWaves][kWinChunk], sl[kIlWaves][kWinChunk];
@@ -509,7 +514,10 @@ __global__ void __launch_bounds__(64 * kIlWaves)
     for (int64_t k = (int64_t)blockIdx.x * kIlWaves + wv; k < n_win; k += (int64_t)gridDim.x * kIlWaves) {
     const Window W = wins[k];
     const int cnt = W.cnt();
-    if (cnt <= kWinAlign) continue;
+    if (cnt <= kWinAlign) {  // one group: its slots are its positions
+        if (h16 && lane < cnt) h16[W.off + lane] = fp16_of(hi[W.off + lane], lo[W.off + lane]);
+        continue;
+    }
     for (int base = 0; base < cnt; base += kWinChunk) {
         const int n = min(kWinChunk, cnt - base);
         for (int t = lane; t < n; t += 64) {
@@ -538,6 +546,7 @@ __global__ void __launch_bounds__(64 * kIlWaves)
             } else {
                 hi[e] = sh[wv][j];
                 lo[e] = sl[wv][j];
+                if (h16) h16[e] = fp16_of(sh[wv][j], sl[wv][j]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -545,13 +554,6 @@ __global__ void __launch_bounds__(64 * kIlWaves)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     }
-}
-
-// FP16 copy of every slot, rounded to nearest from the FP32 hi + lo value
-// (alignment gaps hold whatever the block held; the product never uses them)
-__global__ void k_make_h16(const float* __restrict__ hi, const float* __restrict__ lo, int64_t n, uint16_t* h16) {
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-        h16[e] = __builtin_bit_cast(uint16_t, (_Float16)((double)hi[e] + (double)lo[e]));
 }
 
 __global__ void k_task_remap(TileTask* t, int64_t n, const int32_t* nidx) {
@@ -1231,10 +1233,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
                        H->d_win);
     if (n_win > 0)
         hipLaunchKernelGGL(k_interleave, std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u), 64 * kIlWaves, 0, s,
-                           H->d_win, (int64_t)n_win, H->d_idx16, H->d_val, H->d_hi, H->d_lo);
-    if (H->d_h16)
-        hipLaunchKernelGGL(k_make_h16, std::min<unsigned>(grid_for((int64_t)cap, 256), 65536u), 256, 0, s, H->d_hi,
-                           H->d_lo, (int64_t)cap, H->d_h16);
+                           H->d_win, (int64_t)n_win, H->d_idx16, H->d_val, H->d_hi, H->d_lo, H->d_h16);
     if (!ta.empty())
         hipLaunchKernelGGL(k_task_remap, grid_for((int64_t)ta.size(), 256), 256, 0, s, H->d_task_a,
                            (int64_t)ta.size(), d_nidx);
