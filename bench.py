@@ -22,8 +22,9 @@ root, residual 1e-4 and energy change 1e-8).  Workloads (--config):
 Timed path (value): tiled storage with FP32 hi+lo values (plus an FP16 copy of
 each value for workload 1: --values), Davidson with FP16 sigmas, then FP32
 sigmas from 3x the tolerance on, then FP64 refinement (FP32 then FP64 for
-workloads 2 and 4), restarts that keep the previous Ritz vector, and the
-correction chosen by --precond.  Reported beside it from the same run: the
+workloads 2 and 4), thick restarts (workload 1: subspace 16, 4 vectors kept;
+others: subspace 11, 2 kept; --max-sub, --keep), and the correction chosen by
+--precond (learned for workload 1, diagonal otherwise).  Reported beside it from the same run: the
 plain FP64 GPU path (FP64 values, the reference's restart-to-one-vector control
 flow, diagonal correction) and its E0, the other correction, the H-build
 nonzeros/s, the sigma products' HBM-roofline fraction (algorithmic bytes / HIP
@@ -144,9 +145,13 @@ def main() -> int:
     ap.add_argument("--values", choices=["f16", "f32"], default=None,
                     help="lowest value precision of the timed path: f16 (FP16 copy, FP16 -> FP32 -> FP64 "
                          "sigmas) or f32 (FP32 -> FP64); default f16 for workload 1, f32 otherwise")
+    ap.add_argument("--max-sub", type=int, default=None,
+                    help="Davidson subspace size (default 16 for workload 1, 11 otherwise)")
+    ap.add_argument("--keep", type=int, default=None,
+                    help="vectors kept on restart (default 4 for workload 1, 2 otherwise)")
     ap.add_argument("--precond", choices=["mlp", "dpr"], default=None,
                     help="correction of the timed path (the other one is reported beside it); "
-                         "default mlp, dpr for the Hubbard workload")
+                         "default mlp for workload 1, dpr otherwise")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has one MI355X; "
@@ -166,7 +171,13 @@ def main() -> int:
     ctx.set_integrals(ints)
     precond_train.load(ctx)
     if args.precond is None:
-        args.precond = "dpr" if args.config == 2 else "mlp"
+        # the correction is trained on n_orb <= 40 molecules: on the lattice
+        # and the n_orb = 80 molecule the diagonal correction converges sooner
+        args.precond = "mlp" if args.config == 1 else "dpr"
+    if args.max_sub is None:
+        args.max_sub = 16 if args.config == 1 else 11
+    if args.keep is None:
+        args.keep = 4 if args.config == 1 else 2
     precond = PRECOND_MLP if args.precond == "mlp" else PRECOND_DPR
     diag = ctx.ham_diag(dets)
     d_dets = DeviceBuffer.from_array(ctx, dets)
@@ -178,11 +189,12 @@ def main() -> int:
     cal_a.free()
     cal_b.free()
 
-    def run(prec, mixed, keep, pc):
+    def run(prec, mixed, keep, pc, max_sub=11):
         ts = time.perf_counter()
         h = ctx.build_h(d_dets, d_diag, n=n, value_prec=prec, storage=STORE_TILED)
         tb = time.perf_counter() - ts
-        res = ctx.davidson(h, tol=1e-4, etol=1e-8, max_iters=400, mixed=mixed, restart_keep=keep, precond=pc)
+        res = ctx.davidson(h, tol=1e-4, etol=1e-8, max_iters=400, mixed=mixed, restart_keep=keep, precond=pc,
+                           max_sub=max_sub)
         ctx.sync()
         return h, res, tb, time.perf_counter() - ts
 
@@ -193,7 +205,7 @@ def main() -> int:
     fast_prec, fast_mixed = (PREC_F16, 2) if args.values == "f16" else (PREC_F32, 1)
 
     def step():
-        return run(fast_prec, fast_mixed, 2, precond)
+        return run(fast_prec, fast_mixed, args.keep, precond, args.max_sub)
 
     for w in range(args.warmup):
         h, res, tb, tt = step()
@@ -237,7 +249,8 @@ def main() -> int:
         "config": {"workload": f"{workload}; H build + Davidson to E0 (tol 1e-4, etol 1e-8)",
                    "n_dets": n, "nnz_u": nnz,
                    "storage": "tiled, fp32 hi+lo" + (" + fp16 copy" if args.values == "f16" else ""),
-                   "davidson": f"mixed precision, restart keeps 2 vectors, correction={args.precond}",
+                   "davidson": f"mixed precision, subspace {args.max_sub}, restart keeps {args.keep} vectors, "
+                               f"correction={args.precond}",
                    "parallelism": "single GPU"},
         "E0": results[-1].energy,
         "davidson": {"iters": last["iters"], "n_sigma_f16": n16, "f16_iter": last["f16_iter"],
@@ -268,7 +281,7 @@ def main() -> int:
         out["abs_dE_vs_fp64_reference"] = abs(ref.energy - out["E0"])
         log(f"fp64 reference path: {tr:.3f}s E0={ref.energy:.10f} iters={st['iters']} sigma {r_ms:.2f} ms")
         other = PRECOND_DPR if precond == PRECOND_MLP else PRECOND_MLP
-        h, alt, tb, ta = run(fast_prec, fast_mixed, 2, other)
+        h, alt, tb, ta = run(fast_prec, fast_mixed, args.keep, other, args.max_sub)
         h.free()
         out["other_correction"] = {
             "correction": "dpr" if other == PRECOND_DPR else "mlp", "s_to_E0": round(ta, 4),
