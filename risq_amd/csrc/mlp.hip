@@ -220,17 +220,43 @@ __global__ void k_prep_weights(const float* P, short* img) {
 // (8 x 8 MFMAs per 16 rows) -> SiLU -> out = H2 . W3 + b3 (bf16 operands, FP32
 // sums over the 16 lanes of a row group).  Writes out (if given) and
 // t = (1 + kGAmp tanh(out)) * dpr (if given).  The same roundings as the
-// layer-by-layer path: operands to bf16, FP32 accumulation.
-constexpr int kInfWaves = 4;
+// layer-by-layer path: operands to bf16 (round to nearest even), FP32
+// accumulation.
+//
+// Eight waves per block share one LDS copy of the weights (46 KB), so two
+// blocks fit a CU; at most 128 VGPRs per lane give four waves per SIMD.  The
+// next 32 rows' features and this task's dpr are loaded while the current
+// rows are computed.
+constexpr int kInfWaves = 8;
+constexpr int kInfMaxBlocks = 2048;  // 524,288 rows per pass of the grid
 // SiLU with the hardware reciprocal (1 ulp) for inference, whose activations
 // are rounded to bfloat16 next anyway
 __device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+// round to nearest even with the hardware conversion (same result as to_bf16)
+__device__ __forceinline__ short bf16_rne(float f) { return __builtin_bit_cast(short, (__bf16)f); }
+
+// raw features of 32 rows: lane holds X[row 16 h + r16][16 ks + kq + j]
+struct XRaw {
+    f32x4 v[2][kK1 / 16];
+};
+__device__ __forceinline__ void load_x(XRaw& x, const float* __restrict__ X, int64_t n, int64_t g0, int r16,
+                                       int kq) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int64_t row = g0 + 16 * h + r16;
+#pragma unroll
+        for (int ks = 0; ks < kK1; ks += 16) {
+            const int k0 = ks + kq;
+            x.v[h][ks / 16] = (row < n && k0 < F) ? *(const f32x4*)&X[row * F + k0] : f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+}
 
 // One wave per 32 determinants (two 16-row halves sharing each weight
 // fragment).  Layer 1 is computed transposed (W1^T X^T), which leaves every
 // lane holding H[row][16 ks + 4 (lane / 16) + i] -- exactly its A fragment of
 // layer 2's K-step ks -- so the hidden activations never leave registers.
-__global__ void __launch_bounds__(64 * kInfWaves)
+__global__ void __launch_bounds__(64 * kInfWaves, 4)
     k_mlp_infer(const float* __restrict__ X, int64_t n, const float* __restrict__ mean,
                 const float* __restrict__ stdv, const short* __restrict__ img, const float* __restrict__ P,
                 const double* __restrict__ dpr, float* out, double* t) {
@@ -253,8 +279,20 @@ __global__ void __launch_bounds__(64 * kInfWaves)
     const short* s_w2 = s_w + kImgW1;
     const int lane = tid & 63, wv = tid >> 6;
     const int r16 = lane & 15, kq = (lane >> 4) * 4;
-    for (int64_t g0 = ((int64_t)blockIdx.x * kInfWaves + wv) * 32; g0 < n;
-         g0 += (int64_t)gridDim.x * kInfWaves * 32) {
+    // the row this lane writes: lanes with r16 < 8 of group kq take row
+    // 16 (r16 / 4) + kq + r16 % 4, whose sum they hold after the reduction
+    const int oh = (r16 >> 2) & 1, oi = r16 & 3;
+    const int orow = 16 * oh + kq + oi;
+    const int64_t stride = (int64_t)gridDim.x * kInfWaves * 32;
+    int64_t g0 = ((int64_t)blockIdx.x * kInfWaves + wv) * 32;
+    XRaw xr;
+    if (g0 < n) load_x(xr, X, n, g0, r16, kq);
+    for (; g0 < n; g0 += stride) {
+        // the weights and biases are read from LDS where they are used: held
+        // across tasks they would take registers the occupancy needs
+        __asm__ volatile("" ::: "memory");
+        double dp = 0.0;
+        if (t && r16 < 8 && g0 + orow < n) dp = dpr[g0 + orow];
         // X fragments (B operand of the transposed layer 1): X[row][ks + kq + j]
         bf16x4 xf[2][kK1 / 16];
 #pragma unroll
@@ -263,14 +301,15 @@ __global__ void __launch_bounds__(64 * kInfWaves)
 #pragma unroll
             for (int ks = 0; ks < kK1; ks += 16) {
                 const int k0 = ks + kq;
-                f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
-                if (row < n && k0 < F) v = *(const f32x4*)&X[row * F + k0];
                 bf16x4 b;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = k0 + j < F && row < n ? to_bf16((v[j] - s_mu[k0 + j]) / s_sd[k0 + j]) : (short)0;
+                for (int j = 0; j < 4; ++j)
+                    b[j] = k0 + j < F && row < n ? bf16_rne((xr.v[h][ks / 16][j] - s_mu[k0 + j]) / s_sd[k0 + j])
+                                                 : (short)0;
                 xf[h][ks / 16] = b;
             }
         }
+        if (g0 + stride < n) load_x(xr, X, n, g0 + stride, r16, kq);
         // layer 1, transposed: lane holds z1[row 16 h + r16][hidden 16 nt + kq + i]
         bf16x4 a2[2][Hd / 16];
 #pragma unroll
@@ -285,13 +324,13 @@ __global__ void __launch_bounds__(64 * kInfWaves)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float bias = s_b1[nt * 16 + kq + i];
-                a2[0][nt][i] = to_bf16(silu_fast(c0[i] + bias));
-                a2[1][nt][i] = to_bf16(silu_fast(c1[i] + bias));
+                a2[0][nt][i] = bf16_rne(silu_fast(c0[i] + bias));
+                a2[1][nt][i] = bf16_rne(silu_fast(c1[i] + bias));
             }
         }
         // layer 2: lane holds z2[row 16 h + kq + i][hidden 16 nt + r16]
         float o[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
+#pragma unroll 2
         for (int nt = 0; nt < Hd / 16; ++nt) {
             f32x4 c0 = f32x4{0.f, 0.f, 0.f, 0.f}, c1 = c0;
 #pragma unroll
@@ -304,28 +343,28 @@ __global__ void __launch_bounds__(64 * kInfWaves)
             const float bias = s_b2[col], w3 = s_w3[col];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                o[0][i] += from_bf16(to_bf16(silu_fast(c0[i] + bias))) * w3;
-                o[1][i] += from_bf16(to_bf16(silu_fast(c1[i] + bias))) * w3;
+                o[0][i] += from_bf16(bf16_rne(silu_fast(c0[i] + bias))) * w3;
+                o[1][i] += from_bf16(bf16_rne(silu_fast(c1[i] + bias))) * w3;
             }
         }
+        // butterfly over the 16 lanes of each group: every lane holds the sums
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int d = 1; d < 16; d <<= 1) o[h][i] += __shfl_xor(o[h][i], d);
-        if (r16 == 0) {
+        float mine = o[0][0];
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int64_t row = g0 + 16 * h + kq + i;
-                    if (row < n) {
-                        const float v = o[h][i] + b3;
-                        if (out) out[row] = v;
-                        if (t) t[row] = (double)(1.f + kGAmp * tanhf(v)) * dpr[row];
-                    }
-                }
+            for (int i = 0; i < 4; ++i)
+                if (oh == h && oi == i) mine = o[h][i];
+        const int64_t row = g0 + orow;
+        if (r16 < 8 && row < n) {
+            const float v = mine + b3;
+            if (out) out[row] = v;
+            if (t) t[row] = (double)(1.f + kGAmp * tanhf(v)) * dp;
         }
     }
 }
@@ -427,7 +466,7 @@ void infer(hci_ctx* c, const float* X, int64_t n, float* out, const double* dpr,
     MlpState* s = c->mlp;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(k_prep_weights, grid_for(kImgW2, 256), 256, 0, st, s->P, s->img);
-    const unsigned grid = (unsigned)std::min<int64_t>(grid_for((n + 31) / 32, kInfWaves), 2048);
+    const unsigned grid = (unsigned)std::min<int64_t>(grid_for((n + 31) / 32, kInfWaves), kInfMaxBlocks);
     hipLaunchKernelGGL(k_mlp_infer, grid, 64 * kInfWaves, 0, st, X, n, s->mean, s->stdv, s->img, s->P, dpr, out, t);
 }
 
