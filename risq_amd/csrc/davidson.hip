@@ -70,20 +70,26 @@ __device__ __forceinline__ void block_store(double (&acc)[NV], int nv, double* p
     }
 }
 
-// partial[b*k + c] = sum over this block's rows of A[c][i] * b[i], c < k
+// The vector kernels below are instantiated for KM = 8, 16 and 32 columns at
+// most (the solve's subspace size rounded up): their per-thread arrays take
+// registers in proportion to KM, and the occupancy that the registers leave
+// decides how many rows' loads are in flight.
+
+// partial[b*k + c] = sum over this block's rows of A[c][i] * b[i], c < k <= KM + 1
+template <int KM>
 __global__ void __launch_bounds__(kRedThreads)
     k_multidot(const double* __restrict__ A, int64_t ld, int k, const double* __restrict__ b, int64_t n,
                double* partial) {
-    double acc[kMaxSub + 1];
+    double acc[KM + 1];
 #pragma unroll
-    for (int c = 0; c <= kMaxSub; ++c) acc[c] = 0.0;
+    for (int c = 0; c <= KM; ++c) acc[c] = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double bi = b[i];
 #pragma unroll
-        for (int c = 0; c <= kMaxSub; ++c)
+        for (int c = 0; c <= KM; ++c)
             if (c < k) acc[c] += A[(size_t)c * ld + i] * bi;
     }
-    block_store<kMaxSub + 1>(acc, k, partial);
+    block_store<KM + 1>(acc, k, partial);
 }
 
 // out[c] = sum_b partial[b*k + c], fixed order (deterministic)
@@ -104,7 +110,8 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(const double* partial, i
 }
 
 // x = V y, wx = W y (= H x), r = wx - lam x, t = r / (lam - d); partial sums of
-// r.r and t.t
+// r.r and t.t; k <= KM
+template <int KM>
 __global__ void __launch_bounds__(kRedThreads)
     k_ritz(const double* __restrict__ V, const double* __restrict__ W, int64_t ld, int k, Coef y,
            double lam, const double* __restrict__ d, int64_t n, double* x, double* wx, double* r, double* t,
@@ -113,7 +120,7 @@ __global__ void __launch_bounds__(kRedThreads)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double xv = 0.0, wv = 0.0;
 #pragma unroll
-        for (int c = 0; c < kMaxSub; ++c)
+        for (int c = 0; c < KM; ++c)
             if (c < k) {
                 xv += V[(size_t)c * ld + i] * y.c[c];
                 wv += W[(size_t)c * ld + i] * y.c[c];
@@ -130,34 +137,46 @@ __global__ void __launch_bounds__(kRedThreads)
     block_store<2>(acc, 2, partial);
 }
 
-// t -= sum_c V[c] h[c]; then partial dots of V[c].t (c < k) and t.t (slot k)
+// t -= sum_c V[c] h[c]; then partial dots of V[c].t (c < k) and t.t (slot k); k <= KM
+template <int KM>
 __global__ void __launch_bounds__(kRedThreads)
     k_orth_sub_dot(const double* __restrict__ V, int64_t ld, int k, Coef h, double* t, int64_t n,
                    double* partial) {
-    double acc[kMaxSub + 1];
+    double acc[KM + 1];
 #pragma unroll
-    for (int c = 0; c <= kMaxSub; ++c) acc[c] = 0.0;
+    for (int c = 0; c <= KM; ++c) acc[c] = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double tv = t[i];
-        double vc[kMaxSub];
+        double vc[KM];
 #pragma unroll
-        for (int c = 0; c < kMaxSub; ++c)
+        for (int c = 0; c < KM; ++c)
             if (c < k) {
                 vc[c] = V[(size_t)c * ld + i];
                 tv -= vc[c] * h.c[c];
             }
         t[i] = tv;
 #pragma unroll
-        for (int c = 0; c < kMaxSub; ++c)
+        for (int c = 0; c < KM; ++c)
             if (c < k) acc[c] += vc[c] * tv;
-        acc[kMaxSub] += tv * tv;
+        acc[KM] += tv * tv;
     }
     // move t.t into slot k for the reduction layout (static indices only)
 #pragma unroll
-    for (int c = 0; c < kMaxSub; ++c)
-        if (c == k) acc[c] = acc[kMaxSub];
-    block_store<kMaxSub + 1>(acc, k + 1, partial);
+    for (int c = 0; c < KM; ++c)
+        if (c == k) acc[c] = acc[KM];
+    block_store<KM + 1>(acc, k + 1, partial);
 }
+
+// launch kern<KM> with the smallest KM in {8, 16, 32} that is >= km
+#define HCI_LAUNCH_KM(kern, km, grid, block, stream, ...)                                   \
+    do {                                                                                    \
+        if ((km) <= 8)                                                                      \
+            hipLaunchKernelGGL(kern<8>, grid, block, 0, stream, __VA_ARGS__);               \
+        else if ((km) <= 16)                                                                \
+            hipLaunchKernelGGL(kern<16>, grid, block, 0, stream, __VA_ARGS__);              \
+        else                                                                                \
+            hipLaunchKernelGGL(kern<32>, grid, block, 0, stream, __VA_ARGS__);              \
+    } while (0)
 
 // In place: V[:, j] = sum_c V[:, c] q[c * kMaxKeep + j] for j < m (same for W).
 // Each thread reads its row of all k columns before writing m of them.
@@ -284,7 +303,7 @@ struct DavWork {
         return HCI_OK;
     }
     int multidot(const double* A, int k, const double* b) {
-        hipLaunchKernelGGL(k_multidot, red_grid(), kRedThreads, 0, c->stream, A, n, k, b, n, partial);
+        HCI_LAUNCH_KM(k_multidot, max_sub, red_grid(), kRedThreads, c->stream, A, n, k, b, n, partial);
         return reduce(k);
     }
 };
@@ -322,7 +341,8 @@ int record_rows(hci_ctx* c, const hci_csr* h, DavWork& w, Recorder& rec, double 
     if ((rc = w.multidot(w.V, dim_sub, rec.e_perp))) return rc;
     Coef hc{};
     for (int k = 0; k < dim_sub; ++k) hc.c[k] = w.h_red[k];
-    hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, dim_sub, hc, rec.e_perp, N, w.partial);
+    HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rec.e_perp, N,
+                  w.partial);
     if ((rc = w.reduce(dim_sub + 1))) return rc;
     const double enorm = std::sqrt(w.h_red[dim_sub]);
     if (!(enorm > 0.0) || !(tnorm > 0.0)) return HCI_OK;
@@ -480,8 +500,8 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         lam = vals[0];
         Coef y{};
         for (int k = 0; k < dim_sub; ++k) y.c[k] = vecs[(size_t)k * dim_sub];
-        hipLaunchKernelGGL(k_ritz, w.red_grid(), kRedThreads, 0, s, w.V, w.W, N, dim_sub, y, lam, h->d_diag, N,
-                           w.x, w.wx, w.r, w.t, w.partial);
+        HCI_LAUNCH_KM(k_ritz, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam, h->d_diag, N,
+                      w.x, w.wx, w.r, w.t, w.partial);
         DT(w.reduce(2));
         err = std::sqrt(w.h_red[0]);
         double tnorm = std::sqrt(w.h_red[1]);
@@ -561,10 +581,12 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             DT(w.multidot(w.V, dim_sub, t));
             Coef hc{};
             for (int k = 0; k < dim_sub; ++k) hc.c[k] = w.h_red[k];
-            hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, dim_sub, hc, t, N, w.partial);
+            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, t, N,
+                          w.partial);
             DT(w.reduce(dim_sub + 1));
             for (int k = 0; k < dim_sub; ++k) hc.c[k] = w.h_red[k];
-            hipLaunchKernelGGL(k_orth_sub_dot, w.red_grid(), kRedThreads, 0, s, w.V, N, dim_sub, hc, t, N, w.partial);
+            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, t, N,
+                          w.partial);
             DT(w.reduce(dim_sub + 1));
             const double nrm = std::sqrt(w.h_red[dim_sub]);
             if (nrm < 1e-9) {

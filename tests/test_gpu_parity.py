@@ -310,6 +310,22 @@ def test_thick_restart_same_energy(gpu_ctx, oracle_lib):
     assert b.stats["iters"] <= a.stats["iters"]
 
 
+@pytest.mark.parametrize("max_sub,keep", [(4, 1), (8, 3), (11, 2), (16, 4), (17, 4), (32, 8)])
+def test_subspace_sizes_same_energy(gpu_ctx, oracle_lib, max_sub, keep):
+    """The subspace's vector kernels come in three widths (up to 8, 16 and 32
+    columns, chosen from max_sub): every width, at its edges, converges to the
+    dense lowest eigenvalue, and its vector is an eigenvector."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 1500)
+    gpu_ctx.set_integrals(ints)
+    h = gpu_ctx.build_h(dets, gpu_ctx.ham_diag(dets))
+    ref = np.linalg.eigvalsh(csr_to_dense(*h.download()))[0]
+    r = gpu_ctx.davidson(h, tol=1e-5, etol=1e-9, max_iters=600, max_sub=max_sub, restart_keep=keep)
+    assert r.converged
+    assert abs(r.energy - ref) < 1e-6
+    hx = h.matrix_vector_prod(r.coeffs)
+    assert np.linalg.norm(hx - r.energy * r.coeffs) < 1e-4
+
+
 @pytest.mark.parametrize("prec", [64, 32])
 def test_tiled_storage_large_groups(gpu_ctx, oracle_lib, prec):
     """Alpha groups larger than a tile (256 rows) and column groups wider than a
