@@ -500,6 +500,7 @@ __global__ void k_win_compact(const int64_t* wc, const int64_t* woff, const int3
 // of at most 8 entries is one group whose slots are its positions.  With
 // FP16 storage each value's FP16 copy is written at its slot on the way.
 constexpr int kIlWaves = 4;
+constexpr int kIlStage = kWinChunk + kWinChunk / 8;  // staged entries with the bank padding
 // FP16 copy of a value (HCI_PREC_F16), rounded to nearest from FP32 hi + lo
 __device__ __forceinline__ uint16_t fp16_of(float hi, float lo) {
     return __builtin_bit_cast(uint16_t, (_Float16)((double)hi + (double)lo));
@@ -507,9 +508,16 @@ __device__ __forceinline__ uint16_t fp16_of(float hi, float lo) {
 __global__ void __launch_bounds__(64 * kIlWaves)
     k_interleave(const Window* __restrict__ wins, int64_t n_win, uint16_t* idx, double* val, float* hi,
                  float* lo, uint16_t* h16) {
-    __shared__ uint16_t si[kIlWaves][kWinChunk];
-    __shared__ double sv[kIlWaves][kWinChunk];
-    __shared__ float sh[kIlWaves][kWinChunk], sl[kIlWaves][kWinChunk];
+    // one staging area per wave: FP64 values, or FP32 hi and lo words, in
+    // the same bytes (a build writes one or the other).  Entry j sits at
+    // j + 8 (j / 64): the slot-order reads (entries 64 apart) then hit 64
+    // different banks.
+    __shared__ uint16_t s_idx[kIlWaves][kIlStage];
+    __shared__ double s_val[kIlWaves][kIlStage];
+    uint16_t* si = s_idx[threadIdx.x >> 6];
+    double* sv = s_val[threadIdx.x >> 6];
+    float* sh = (float*)sv;
+    float* sl = sh + kIlStage;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int64_t k = (int64_t)blockIdx.x * kIlWaves + wv; k < n_win; k += (int64_t)gridDim.x * kIlWaves) {
     const Window W = wins[k];
@@ -522,12 +530,13 @@ __global__ void __launch_bounds__(64 * kIlWaves)
         const int n = min(kWinChunk, cnt - base);
         for (int t = lane; t < n; t += 64) {
             const int64_t e = W.off + base + t;
-            si[wv][t] = idx[e];
+            const int pt = t + ((t >> 6) << 3);
+            si[pt] = idx[e];
             if (val) {
-                sv[wv][t] = val[e];
+                sv[pt] = val[e];
             } else {
-                sh[wv][t] = hi[e];
-                sl[wv][t] = lo[e];
+                sh[pt] = hi[e];
+                sl[pt] = lo[e];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -540,13 +549,15 @@ __global__ void __launch_bounds__(64 * kIlWaves)
             const int j = full ? (t & 7) * 64 + (t >> 3) : (t >> 3) + (t & 7) * R;
             if (j >= n) continue;
             const int64_t e = W.off + base + t;
-            idx[e] = si[wv][j];
+            const int pj = j + ((j >> 6) << 3);
+            idx[e] = si[pj];
             if (val) {
-                val[e] = sv[wv][j];
+                val[e] = sv[pj];
             } else {
-                hi[e] = sh[wv][j];
-                lo[e] = sl[wv][j];
-                if (h16) h16[e] = fp16_of(sh[wv][j], sl[wv][j]);
+                const float h = sh[pj], l = sl[pj];
+                hi[e] = h;
+                lo[e] = l;
+                if (h16) h16[e] = fp16_of(h, l);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
