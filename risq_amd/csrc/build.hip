@@ -411,52 +411,93 @@ __global__ void __launch_bounds__(kBlock)
         if (FILL && lane == 0) atomicAdd(nnz_out, nz);
         return;
     }
-    const int32_t P = sb.nr * sb.nc;
-    for (int32_t fb = 0; fb < P; fb += 64) {
-        const int32_t f = fb + lane;
+    // kinds 0-2: every (row, column) pair, column by column.  Kind 2 holds
+    // beta-major positions and compares alpha strings; kinds 0 and 1 compare
+    // beta strings.  A lane keeps its rows' strings in registers: with 64 or
+    // more rows it owns rows lane + 64 k and the wave walks the columns one
+    // at a time (each column string is one uniform load); with fewer, lane
+    // takes row lane % nr of column lane / nr, 64 / nr columns per step.
+    const bool bpart = sb.kind == 2;
+    const S* rstr = bpart ? B.sb_up : B.sa_dn;
+    const int32_t* ridx = bpart ? B.sb_idx : B.sa_idx;
+    auto test = [&](bool ok, int32_t rl, int32_t cl, S a, int32_t i, S b, int32_t j) {
         bool keep = false;
         double v = 0.0;
-        int32_t rl = 0, cl = 0;
-        if (f < P) {
-            cl = f / sb.nr;
-            rl = f - cl * sb.nr;
-            const int32_t p = sb.r0 + rl, q = sb.c0 + cl;
+        if (ok) {
+            const int c = popc(a ^ b);
             if (sb.kind == 2) {
                 // same beta string: alpha doubles, or alpha singles when every
                 // double vanishes (kinds 1 and 3 are then not built)
-                const int32_t i = B.sb_idx[p], j = B.sb_idx[q];
-                S a = B.sb_up[p], b = B.sb_up[q];
-                keep = popc(a ^ b) == (dbl ? 4 : 2);
+                keep = c == (dbl ? 4 : 2);
                 if (FILL && keep) {
                     swap_if(j < i, a, b);
                     v = dbl ? elem_same_double<S>(T, a, b)
                             : single_or_zero(elem_single<S>(T, a, b, ub[sb.g_row]), single_rule);
                 }
+            } else if (sb.kind == 0) {
+                keep = c == 2 || (c == 4 && dbl);
+                if (FILL && keep) {
+                    const S up = B.ua[sb.g_row];
+                    swap_if(j < i, a, b);
+                    v = c == 2 ? single_or_zero(elem_single<S>(T, a, b, up), single_rule)
+                               : elem_same_double<S>(T, a, b);
+                }
             } else {
-                const int32_t i = B.sa_idx[p], j = B.sa_idx[q];
-                S da = B.sa_dn[p], db = B.sa_dn[q];
-                const int c = popc(da ^ db);
-                if (sb.kind == 0) {
-                    keep = c == 2 || (c == 4 && dbl);
-                    if (FILL && keep) {
-                        const S up = B.ua[sb.g_row];
-                        swap_if(j < i, da, db);
-                        v = c == 2 ? single_or_zero(elem_single<S>(T, da, db, up), single_rule)
-                                   : elem_same_double<S>(T, da, db);
-                    }
-                } else {
-                    keep = c == 0 || (c == 2 && dbl);
-                    if (FILL && keep) {
-                        S ua = B.ua[sb.g_row], uc = B.ua[sb.g_col];
-                        swap_if(j < i, ua, uc);
-                        swap_if(j < i, da, db);
-                        v = c == 0 ? single_or_zero(elem_single<S>(T, ua, uc, da), single_rule)
-                                   : elem_opp_double<S>(T, ua, uc, da, db);
-                    }
+                keep = c == 0 || (c == 2 && dbl);
+                if (FILL && keep) {
+                    S ua = B.ua[sb.g_row], uc = B.ua[sb.g_col];
+                    swap_if(j < i, ua, uc);
+                    swap_if(j < i, a, b);
+                    v = c == 0 ? single_or_zero(elem_single<S>(T, ua, uc, a), single_rule)
+                               : elem_opp_double<S>(T, ua, uc, a, b);
                 }
             }
         }
         emit(keep, rl, cl, v);
+    };
+    if (sb.nr >= 64) {
+        constexpr int KR = kTileRows / 64;
+        S rs[KR];
+        int32_t ri[KR];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int32_t rl = lane + 64 * k;
+            rs[k] = 0;
+            ri[k] = 0;
+            if (rl < sb.nr) {
+                rs[k] = rstr[sb.r0 + rl];
+                ri[k] = ridx[sb.r0 + rl];
+            }
+        }
+        for (int32_t cl = 0; cl < sb.nc; ++cl) {
+            const int32_t q = sb.c0 + cl;
+            const S b = rstr[q];
+            const int32_t j = ridx[q];
+#pragma unroll
+            for (int k = 0; k < KR; ++k)
+                if (64 * k < sb.nr) test(lane + 64 * k < sb.nr, lane + 64 * k, cl, rs[k], ri[k], b, j);
+        }
+    } else {
+        const int32_t per = 64 / sb.nr;
+        const int32_t rl = lane % sb.nr, co = lane / sb.nr;
+        const bool lane_ok = co < per;
+        S a = 0;
+        int32_t i = 0;
+        if (lane_ok) {
+            a = rstr[sb.r0 + rl];
+            i = ridx[sb.r0 + rl];
+        }
+        for (int32_t cb = 0; cb < sb.nc; cb += per) {
+            const int32_t cl = cb + co;
+            const bool ok = lane_ok && cl < sb.nc;
+            S b = 0;
+            int32_t j = 0;
+            if (ok) {
+                b = rstr[sb.c0 + cl];
+                j = ridx[sb.c0 + cl];
+            }
+            test(ok, rl, ok ? cl : 0, a, i, b, j);
+        }
     }
     if (!FILL && lane < n_slices) win_cnt[sb.win_begin + lane] = wcur[lane];
     if (FILL && lane == 0) atomicAdd(nnz_out, nz);
