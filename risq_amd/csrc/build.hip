@@ -420,9 +420,9 @@ __global__ void __launch_bounds__(kBlock)
     const bool bpart = sb.kind == 2;
     const S* rstr = bpart ? B.sb_up : B.sa_dn;
     const int32_t* ridx = bpart ? B.sb_idx : B.sa_idx;
-    auto test = [&](bool ok, int32_t rl, int32_t cl, S a, int32_t i, S b, int32_t j) {
-        bool keep = false;
-        double v = 0.0;
+    auto pair_val = [&](bool ok, S a, int32_t i, S b, int32_t j, bool& keep, double& v) {
+        keep = false;
+        v = 0.0;
         if (ok) {
             const int c = popc(a ^ b);
             if (sb.kind == 2) {
@@ -453,9 +453,22 @@ __global__ void __launch_bounds__(kBlock)
                 }
             }
         }
-        emit(keep, rl, cl, v);
     };
     if (sb.nr >= 64) {
+        // every step stays in the window of its column, and the windows are
+        // visited in order: the running count of the current window is a
+        // wave-uniform register (no LDS cursor), stored when its last column
+        // is done
+        auto emit_col = [&](bool keep, int32_t rl, int32_t cl, double v, int32_t& run, int64_t base) {
+            const uint64_t mk = __ballot(keep);
+            if (FILL) {
+                nz += __popcll(__ballot(keep && v != 0.0));
+                if (keep)
+                    put16(out, base + run + __popcll(mk & lt_mask), ((uint32_t)(rl + sb.rb) << 8) | (uint32_t)(cl % kWinCols),
+                          v);
+            }
+            run += __popcll(mk);
+        };
         constexpr int KR = kTileRows / 64;
         S rs[KR];
         int32_t ri[KR];
@@ -469,14 +482,28 @@ __global__ void __launch_bounds__(kBlock)
                 ri[k] = ridx[sb.r0 + rl];
             }
         }
-        for (int32_t cl = 0; cl < sb.nc; ++cl) {
-            const int32_t q = sb.c0 + cl;
-            const S b = rstr[q];
-            const int32_t j = ridx[q];
+        for (int32_t w0 = 0; w0 < sb.nc; w0 += kWinCols) {
+            const int slice = w0 / kWinCols;
+            const int64_t base = FILL ? out.woff[sb.win_begin + slice] : 0;
+            int32_t run = 0;
+            const int32_t w1 = min(sb.nc, w0 + kWinCols);
+            for (int32_t cl = w0; cl < w1; ++cl) {
+                const int32_t q = sb.c0 + cl;
+                const S b = rstr[q];
+                const int32_t j = ridx[q];
 #pragma unroll
-            for (int k = 0; k < KR; ++k)
-                if (64 * k < sb.nr) test(lane + 64 * k < sb.nr, lane + 64 * k, cl, rs[k], ri[k], b, j);
+                for (int k = 0; k < KR; ++k) {
+                    if (64 * k >= sb.nr) continue;
+                    bool keep;
+                    double v;
+                    pair_val(lane + 64 * k < sb.nr, rs[k], ri[k], b, j, keep, v);
+                    emit_col(keep, lane + 64 * k, cl, v, run, base);
+                }
+            }
+            if (!FILL && lane == 0) win_cnt[sb.win_begin + slice] = run;
         }
+        if (FILL && lane == 0) atomicAdd(nnz_out, nz);
+        return;
     } else {
         const int32_t per = 64 / sb.nr;
         const int32_t rl = lane % sb.nr, co = lane / sb.nr;
@@ -496,7 +523,10 @@ __global__ void __launch_bounds__(kBlock)
                 b = rstr[sb.c0 + cl];
                 j = ridx[sb.c0 + cl];
             }
-            test(ok, rl, ok ? cl : 0, a, i, b, j);
+            bool keep;
+            double v;
+            pair_val(ok, a, i, b, j, keep, v);
+            emit(keep, rl, ok ? cl : 0, v);
         }
     }
     if (!FILL && lane < n_slices) win_cnt[sb.win_begin + lane] = wcur[lane];
