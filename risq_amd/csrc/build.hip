@@ -370,15 +370,27 @@ __global__ void __launch_bounds__(kBlock)
                 }
                 cur = lo;
             }
+            // the walk is software-pipelined: the next key and position are
+            // loaded while the current match is placed
+            S kn = 0;
+            int32_t qn = 0;
+            if (cur < seg_hi) {
+                kn = keys[cur];
+                qn = kq[cur];
+            }
             while (true) {
-                const bool pending = cur < seg_hi && keys[cur] == key;
+                const bool pending = cur < seg_hi && kn == key;
                 if (!__ballot(pending)) break;
                 bool keep = false;
                 double v = 0.0;
                 int32_t cl = 0;
                 if (pending) {
-                    const int32_t q = kq[cur];
+                    const int32_t q = qn;
                     ++cur;
+                    if (cur < seg_hi) {
+                        kn = keys[cur];
+                        qn = kq[cur];
+                    }
                     if (q >= sb.c0 && q < sb.c0 + sb.nc) {
                         cl = q - sb.c0;
                         const int32_t j = cidx[q];
