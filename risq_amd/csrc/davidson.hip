@@ -92,21 +92,19 @@ __global__ void __launch_bounds__(kRedThreads)
     block_store<KM + 1>(acc, k, partial);
 }
 
-// out[c] = sum_b partial[b*k + c], fixed order (deterministic)
+// out[c] = sum_b partial[b*k + c], fixed order (deterministic); one block per c
 __global__ void __launch_bounds__(kRedThreads) k_reduce(const double* partial, int nblocks, int k, double* out) {
     __shared__ double sh[kRedThreads];
-    for (int c = 0; c < k; ++c) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[(size_t)b * k + c];
-        sh[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-            if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[c] = sh[0];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[(size_t)b * k + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[c] = sh[0];
 }
 
 // x = V y, wx = W y (= H x), r = wx - lam x, t = r / (lam - d); partial sums of
@@ -137,11 +135,18 @@ __global__ void __launch_bounds__(kRedThreads)
     block_store<2>(acc, 2, partial);
 }
 
-// t -= sum_c V[c] h[c]; then partial dots of V[c].t (c < k) and t.t (slot k); k <= KM
+// t -= sum_c V[c] h[c]; then partial dots of V[c].t (c < k) and t.t (slot k); k <= KM.
+// The coefficients come from hd (device memory, a previous reduction) when
+// it is given, else from h.
 template <int KM>
 __global__ void __launch_bounds__(kRedThreads)
-    k_orth_sub_dot(const double* __restrict__ V, int64_t ld, int k, Coef h, double* t, int64_t n,
-                   double* partial) {
+    k_orth_sub_dot(const double* __restrict__ V, int64_t ld, int k, Coef h, const double* __restrict__ hd,
+                   double* t, int64_t n, double* partial) {
+    if (hd) {
+#pragma unroll
+        for (int c = 0; c < KM; ++c)
+            if (c < k) h.c[c] = hd[c];
+    }
     double acc[KM + 1];
 #pragma unroll
     for (int c = 0; c <= KM; ++c) acc[c] = 0.0;
@@ -275,8 +280,8 @@ struct DavWork {
     double* r = nullptr;
     double* t = nullptr;
     double* partial = nullptr;
-    double* red = nullptr;
-    double* h_red = nullptr;  // pinned host copy of reductions
+    double* red = nullptr;    // device reductions: 3 slots of kMaxSub + 1 values
+    double* h_red = nullptr;  // pinned host copy of reductions (same layout)
     ~DavWork() {
         hipFree(V);
         hipFree(W);
@@ -296,8 +301,12 @@ struct DavWork {
     }
     double* col(double* M, int c) const { return M + (size_t)c * n; }
     // reductions: results land in h_red[0..k)
+    // reduction into device memory only (no host round trip)
+    void reduce_dev(int k, double* dst) {
+        hipLaunchKernelGGL(k_reduce, k, kRedThreads, 0, c->stream, partial, (int)red_grid(), k, dst);
+    }
     int reduce(int k) {
-        hipLaunchKernelGGL(k_reduce, 1, kRedThreads, 0, c->stream, partial, (int)red_grid(), k, red);
+        hipLaunchKernelGGL(k_reduce, k, kRedThreads, 0, c->stream, partial, (int)red_grid(), k, red);
         HIP_TRY(c, hipMemcpyAsync(h_red, red, (size_t)k * 8, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         return HCI_OK;
@@ -341,8 +350,8 @@ int record_rows(hci_ctx* c, const hci_csr* h, DavWork& w, Recorder& rec, double 
     if ((rc = w.multidot(w.V, dim_sub, rec.e_perp))) return rc;
     Coef hc{};
     for (int k = 0; k < dim_sub; ++k) hc.c[k] = w.h_red[k];
-    HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rec.e_perp, N,
-                  w.partial);
+    HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, nullptr, rec.e_perp,
+                  N, w.partial);
     if ((rc = w.reduce(dim_sub + 1))) return rc;
     const double enorm = std::sqrt(w.h_red[dim_sub]);
     if (!(enorm > 0.0) || !(tnorm > 0.0)) return HCI_OK;
@@ -417,8 +426,8 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
     HIP_TRY(c, hipMalloc(&w.r, vb + 16));
     HIP_TRY(c, hipMalloc(&w.t, vb + 16));
     HIP_TRY(c, hipMalloc(&w.partial, (size_t)kRedBlocks * (kMaxSub + 1) * 8));
-    HIP_TRY(c, hipMalloc(&w.red, (kMaxSub + 1) * 8));
-    HIP_TRY(c, hipHostMalloc(&w.h_red, (kMaxSub + 1) * 8));
+    HIP_TRY(c, hipMalloc(&w.red, 3 * (kMaxSub + 1) * 8));
+    HIP_TRY(c, hipHostMalloc(&w.h_red, 3 * (kMaxSub + 1) * 8));
 
     hci_dav_stats st{};
     st.refine_iter = -1;
@@ -567,28 +576,37 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
                 DT(mlp_correction(c, h, lam, err, tnorm, w.r, w.x, w.t /*dpr*/, t));
                 ++st.mlp_applied;
                 mlp_prev = true;
-                DT(w.multidot(t, 1, t));
-                tnorm = std::sqrt(w.h_red[0]);
             } else {
                 HIP_TRY(c, hipMemcpyAsync(t, w.t, vb, hipMemcpyDeviceToDevice, s));
                 mlp_prev = false;
             }
+            // classical Gram-Schmidt, twice, with the coefficients kept on the
+            // device: t = V[:, dim_sub], so one pass over V[:, 0..dim_sub]
+            // gives V^T t and t.t (slot A); each orthogonalisation pass reads
+            // the previous slot and reduces into the next (B, then C); the
+            // host reads all three once, after the second pass
+            constexpr int kSlot = kMaxSub + 1;
+            double* rA = w.red;
+            double* rB = w.red + kSlot;
+            double* rC = w.red + 2 * kSlot;
+            HCI_LAUNCH_KM(k_multidot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub + 1, t, N,
+                          w.partial);
+            w.reduce_dev(dim_sub + 1, rA);
+            Coef hc{};
+            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rA, t, N,
+                          w.partial);
+            w.reduce_dev(dim_sub + 1, rB);
+            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rB, t, N,
+                          w.partial);
+            w.reduce_dev(dim_sub + 1, rC);
+            HIP_TRY(c, hipMemcpyAsync(w.h_red, w.red, 3 * kSlot * 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+            if (mlp_now) tnorm = std::sqrt(w.h_red[dim_sub]);
             if (tnorm < 1e-9) {
                 converged = true;
                 break;
             }
-            // classical Gram-Schmidt, twice
-            DT(w.multidot(w.V, dim_sub, t));
-            Coef hc{};
-            for (int k = 0; k < dim_sub; ++k) hc.c[k] = w.h_red[k];
-            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, t, N,
-                          w.partial);
-            DT(w.reduce(dim_sub + 1));
-            for (int k = 0; k < dim_sub; ++k) hc.c[k] = w.h_red[k];
-            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, t, N,
-                          w.partial);
-            DT(w.reduce(dim_sub + 1));
-            const double nrm = std::sqrt(w.h_red[dim_sub]);
+            const double nrm = std::sqrt(w.h_red[2 * kSlot + dim_sub]);
             if (nrm < 1e-9) {
                 converged = true;
                 break;
