@@ -545,7 +545,11 @@ int csr_rownorm(hci_csr* h) {
     hci_ctx* c = h->ctx;
     HIP_TRY(c, hipMalloc(&h->d_rownorm, (size_t)h->n * 8 + 16));
     HIP_TRY(c, hipMemsetAsync(h->d_rownorm, 0, (size_t)h->n * 8, c->stream));
-    if (h->prec == HCI_PREC_F64)
+    // a feature of the learned correction (read as FP32): the FP16 copies,
+    // where they exist, are accurate enough and half the bytes
+    if (h->d_h16)
+        launch_spmv<3>(h, nullptr, h->d_rownorm, true);
+    else if (h->prec == HCI_PREC_F64)
         launch_spmv<0>(h, nullptr, h->d_rownorm, true);
     else
         launch_spmv<2>(h, nullptr, h->d_rownorm, true);
