@@ -116,14 +116,16 @@ struct TileOut {
     const int64_t* woff;
 };
 
+// streaming stores: the entries are not read again by this kernel, and
+// keeping them out of the cache leaves it to the integrals and key tables
 __device__ __forceinline__ void put16(const TileOut& o, int64_t e, uint32_t u, double v) {
-    o.idx[e] = (uint16_t)u;
+    __builtin_nontemporal_store((uint16_t)u, o.idx + e);
     if (o.val) {
-        o.val[e] = v;
+        __builtin_nontemporal_store(v, o.val + e);
     } else {
         const float h = (float)v;
-        o.hi[e] = h;
-        o.lo[e] = (float)(v - (double)h);
+        __builtin_nontemporal_store(h, o.hi + e);
+        __builtin_nontemporal_store((float)(v - (double)h), o.lo + e);
     }
 }
 
