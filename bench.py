@@ -154,8 +154,8 @@ def main() -> int:
                          "default mlp for workload 1, dpr otherwise")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box of this build has one MI355X; "
-              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        print("bench.py measures a single MI355X (--gpus 1); sharding the path over several GPUs "
+              "is a design note, not an implemented or measured path", file=sys.stderr)
         return 2
     rank = int(os.environ.get("RANK", "0"))
 

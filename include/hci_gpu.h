@@ -192,7 +192,7 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
 void hci_csr_free(hci_csr* h);
 /* Frees h like hci_csr_free but keeps an HCI_STORE_TILED matrix's entry arrays
  * in its context for the next build there (device memory freed and allocated
- * again is cleared by the driver first: seconds at the 100-GB scale).  The next
+ * again is cleared by the GPU kernel driver first: seconds at the 100-GB scale).  The next
  * TILED build takes what fits and frees the rest; any other build, or
  * hci_destroy, frees them.  No counterpart in the reference, whose sparse
  * matrix is a Rust Vec freed on drop (src/var/sparse.rs:24-36). */

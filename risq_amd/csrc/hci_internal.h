@@ -82,8 +82,8 @@ struct hci_ctx {
     SelectLists* sel = nullptr;
     hipEvent_t events[16] = {};
     // entry arrays of a recycled matrix, offered to the next TILED build
-    // (device memory handed back and allocated again is cleared by the driver
-    // before reuse: about 4 s for 200 GB)
+    // (device memory handed back and allocated again is cleared by the GPU
+    // kernel driver before reuse: about 4 s for 200 GB)
     struct Recycled {
         void* p;
         size_t bytes;
