@@ -1,8 +1,8 @@
 // CPU oracle for the HCI variational hot path -- TEST INFRASTRUCTURE ONLY.
 //
 // A plain C++ restatement of the reference (aaholmes/risq, Rust) used solely as
-// the checker by tests/, __graft_entry__.smoke() and the CPU-baseline leg of
-// bench.py.  The product library never links or calls it.  Every function cites
+// the result to compare against by tests/, __graft_entry__.smoke() and the
+// CPU-baseline leg of bench.py.  The product library never links or calls it.  Every function cites
 // the reference file:line it follows.  Compiled with -ffp-contract=off: Rust
 // never fuses a*b+c, so neither may the restatement.
 //

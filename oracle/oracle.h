@@ -3,7 +3,8 @@
  *
  * A plain C++ restatement of the reference algorithm (aaholmes/risq, Rust), used
  * by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the
- * checker.  Nothing in the product library (risq_amd/) links or calls it.
+ * result to compare against.  Nothing in the product library (risq_amd/) links
+ * or calls it.
  *
  * Determinant words: every determinant is four uint64 words
  *   [up_lo, up_hi, dn_lo, dn_hi]   (bit p = spatial orbital p, 0-based)

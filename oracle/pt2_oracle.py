@@ -1,7 +1,7 @@
 """Deterministic second-order energy by enumeration.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the library's
-hci_pt2_dtm.  Never used by risq_amd/.
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the result the library's
+hci_pt2_dtm is compared against.  Never used by risq_amd/.
 
 Restates dtm_pt (src/pt/mod.rs:81-122) over the candidates of
 dets_excites_and_excited_dets (src/excite/iterator.rs:33-73), but enumerates

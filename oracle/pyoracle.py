@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py as the checker.  Never used by risq_amd/.
+CPU-baseline leg of bench.py as the result to compare against.  Never used by
+risq_amd/.
 """
 from __future__ import annotations
 
