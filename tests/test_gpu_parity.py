@@ -409,12 +409,21 @@ def test_tiled_hubbard_large_groups(gpu_ctx, oracle_lib, prec):
         assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
 
 
-def test_fp16_copy_and_three_phase_davidson(gpu_ctx, oracle_lib):
+@pytest.mark.parametrize("system", ["mol", "u128", "hubbard"])
+def test_fp16_copy_and_three_phase_davidson(gpu_ctx, oracle_lib, system):
     """HCI_PREC_F16 tiled storage: the FP32 hi + lo values download unchanged,
     FP16-value products stay within FP16 rounding of the FP64 product, and the
-    FP16 -> FP32 -> refinement Davidson converges to the FP64 eigenvalue."""
+    FP16 -> FP32 -> refinement Davidson converges to the FP64 eigenvalue.
+    Molecular (one-word strings), 70 orbitals (two-word strings) and a 4x4
+    Hubbard lattice whose alpha groups are wider than the keyed threshold
+    (doubles vanish: singles found by keys inside alpha and beta groups)."""
     from risq_amd.hci import PREC_F16, SIGMA_F16
-    ints, dets = mol_space(*MOL0)
+    if system == "mol":
+        ints, dets = mol_space(*MOL0)
+    elif system == "u128":
+        ints, dets = mol_space(70, 8, 5, 3, 3, 1500)
+    else:
+        ints, dets = hubbard_space(4, 4, 5, 5, 20000)
     gpu_ctx.set_integrals(ints)
     diag = gpu_ctx.ham_diag(dets)
     f32 = gpu_ctx.build_h(dets, diag, value_prec=32, storage=STORE_TILED)
@@ -426,8 +435,15 @@ def test_fp16_copy_and_three_phase_davidson(gpu_ctx, oracle_lib):
     y16 = f16.matrix_vector_prod(x, mode=SIGMA_F16)
     rel = np.linalg.norm(y16 - ref) / np.linalg.norm(ref - diag * x)
     assert 0 < rel < 2e-3, rel
-    H = csr_to_dense(*_oracle_build(oracle_lib, ints, dets, diag).export())
-    e_ref = np.linalg.eigvalsh(H)[0]
+    if dets.shape[0] <= 5000:
+        H = csr_to_dense(*_oracle_build(oracle_lib, ints, dets, diag).export())
+        e_ref = np.linalg.eigvalsh(H)[0]
+    else:  # too large for a dense solve: the FP64 FULL-storage Davidson
+        full = gpu_ctx.build_h(dets, diag)
+        r64 = gpu_ctx.davidson(full, tol=1e-7, etol=1e-11, max_iters=600)
+        assert r64.converged
+        e_ref = r64.energy
+        full.free()
     r = gpu_ctx.davidson(f16, tol=1e-5, etol=1e-9, max_iters=400, mixed=2, restart_keep=2)
     assert r.converged and abs(r.energy - e_ref) < 1e-6, (r.energy, e_ref)
     assert r.stats["n_sigma_f16"] > 0 and r.stats["f16_iter"] >= 0 and r.stats["n_sigma_full"] > 0
