@@ -1068,16 +1068,24 @@ __global__ void k_group_counts(int part, int32_t U, const int32_t* __restrict__ 
 // every sub-block of group g's tiles at sbbase[g], windows from wbase[g]:
 // tile by tile (256 rows; a group of at most 256 rows is one tile, shared with
 // its neighbours at row offset rb[g]), column group by column group, 4096
-// columns at a time
-__global__ void k_gen_subblocks(int part, int32_t U, const int32_t* __restrict__ off,
-                                const int32_t* __restrict__ nbr_off, const int32_t* __restrict__ nbr, int dbl,
-                                const int64_t* __restrict__ wbase, const int64_t* __restrict__ sbbase,
-                                const int32_t* __restrict__ rb, SubBlock* out) {
-    const int32_t g = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (g >= U) return;
+// columns at a time.  One wave per group, one lane per tile: every tile of a
+// group has the same columns, so tile t starts t tiles' worth of sub-blocks
+// and windows after the group's first.
+constexpr int kGenWaves = 4;
+__global__ void __launch_bounds__(64 * kGenWaves)
+    k_gen_subblocks(int part, int32_t U, const int32_t* __restrict__ off, const int32_t* __restrict__ nbr_off,
+                    const int32_t* __restrict__ nbr, int dbl, const int64_t* __restrict__ wbase,
+                    const int64_t* __restrict__ sbbase, const int32_t* __restrict__ rb,
+                    const int32_t* __restrict__ wt, const int32_t* __restrict__ sbt, SubBlock* out) {
+    const int64_t gw = (int64_t)blockIdx.x * kGenWaves + (threadIdx.x >> 6);
+    if (gw >= U) return;
+    const int32_t g = (int32_t)gw;
+    const int lane = threadIdx.x & 63;
     const int32_t g0 = off[g], g1 = off[g + 1];
-    int64_t w = wbase[g], k = sbbase[g];
-    for (int32_t r0 = g0; r0 < g1; r0 += kTileRows) {
+    const int32_t n_tiles = (g1 - g0 + kTileRows - 1) / kTileRows;
+    for (int32_t t = lane; t < n_tiles; t += 64) {
+        const int32_t r0 = g0 + t * kTileRows;
+        int64_t w = wbase[g] + (int64_t)t * wt[g], k = sbbase[g] + (int64_t)t * sbt[g];
         const int32_t nr = min(kTileRows, g1 - r0);
         const int32_t row_base = (g1 - g0 > kTileRows) ? 0 : rb[g];
         ColWalk cw(part, dbl, g, off, nbr_off, nbr);
@@ -1274,8 +1282,8 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMemsetAsync(d_nnz, 0, 8, s));
     for (PartGen& P : parts) {
         if (P.part == 1 && P.U == 0) continue;
-        hipLaunchKernelGGL(k_gen_subblocks, grid_for(P.U, 256), 256, 0, s, P.part, P.U, P.d_off, B.nbr_off, B.nbr, dbl,
-                           P.d_wbase, P.d_sbbase, P.d_rb, d_sb);
+        hipLaunchKernelGGL(k_gen_subblocks, grid_for(P.U, kGenWaves), 64 * kGenWaves, 0, s, P.part, P.U, P.d_off,
+                           B.nbr_off, B.nbr, dbl, P.d_wbase, P.d_sbbase, P.d_rb, P.d_wt, P.d_sbt, d_sb);
     }
     HIP_TRY(c, hipGetLastError());
     H->n_task_a = (int64_t)ta.size();
