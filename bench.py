@@ -45,6 +45,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# (8,157,001 + 1,102,480) KiB x 2 x 1024 read + (15,902 + 15,825) KiB x 1024 written
+MEASURED_F16_PRODUCT_BYTES = 18_995_905_536
 
 
 def log(msg: str) -> None:
@@ -266,6 +268,14 @@ def main() -> int:
                      "kernel": "k_spmv_win (sigma product: all its launches in the timed step: "
                                "FP16-value, FP32-value and FP32 hi+lo)"},
     }
+    # HBM bytes per FP16-value product measured with rocprofv3 on this exact
+    # workload (FETCH_SIZE x2 + WRITE_SIZE of the two k_spmv_win launches,
+    # calibrated against a 256 MiB copy); a counter run cannot happen inside
+    # the timed process, so other workloads report none
+    if args.config == 1 and n == 2_000_000 and args.values == "f16":
+        out["roofline"]["traffic"] = MEASURED_F16_PRODUCT_BYTES
+        out["roofline"]["traffic_source"] = ("profiles/k_spmv_win_config1_2e6dets.md: bytes per FP16-value "
+                                             "product (both triangles at 4 B per entry + FP32 x slices + y)")
     if not args.no_compare:
         h, ref, tb, tr = run(PREC_F64, False, 1, PRECOND_DPR)
         st = ref.stats
