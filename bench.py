@@ -103,7 +103,7 @@ def cpu_baseline(ints, dets) -> dict:
     """The oracle (C++ restatement of the reference path, one core) on a bounded
     prefix of the same determinant list: H build + SpMV + Davidson."""
     from oracle import pyoracle
-    n_s = 20000
+    n_s = 30000  # about 10 s of one core: the all-pairs singles search grows as N^2
     sub = dets[:n_s]
     H = pyoracle.OracleHam(ints)
     diag = H.ham_diag(sub)
