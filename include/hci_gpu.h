@@ -295,6 +295,27 @@ int hci_select(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, const d
 int hci_pt2_dtm(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_t n, double e0, double eps,
                 int64_t batch, double* e2, int64_t* n_ext, int mem);
 
+/* The screened product H|psi> on the external space: replaces
+ * Wf::approx_matmul_external_dtm_only (src/wf/mod.rs:186-339), the
+ * deterministic part of the semistochastic PT2 (src/semistoch/mod.rs:630).
+ * For each determinant i (in order) with local = eps / |c_i|, the excitation
+ * lists of hci_excite_init are walked as the reference walks them (a list is
+ * entered when its maximum is >= local and cut at the first entry below it;
+ * singles are then kept when |H_ai| >= local) and every target a outside
+ * `dets` receives H_ai c_i.  Outputs, in first-discovery order
+ * (add_det_with_coeff, src/wf/mod.rs:144-165): the external determinants
+ * (out, count x 4 words), their coefficients sum_i H_ai c_i summed in
+ * discovery order (out_coeffs), and (out_diag, nullable; needs diag) the
+ * diagonal its first discoverer gives it (Det::new_diag, src/wf/det.rs:249-420).
+ * sum_remaining (nullable, n values): c_i^2 times the sum of the
+ * sum_remaining_h_squared of every list entry where determinant i's walk was
+ * cut (the sampler weights the reference returns).  *count receives the number
+ * of external determinants; when it exceeds cap nothing is written but
+ * sum_remaining and the call fails with *count set. */
+int hci_matmul_external(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
+                        double eps, uint64_t* out, double* out_coeffs, double* out_diag, int64_t cap, int64_t* count,
+                        double* sum_remaining, int mem);
+
 /* Device-memory and timing plumbing (no reference counterpart: for callers
  * that keep inputs resident in HBM, and for timing on the context's stream).
  * Events are 16 slots per context recorded on the context stream. */

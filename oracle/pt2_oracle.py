@@ -115,6 +115,134 @@ def dtm_pt(ints, dets: np.ndarray, coeffs: np.ndarray, e0: float, eps: float, n_
     return e2, len(keys)
 
 
+def _excite_lists(ints, n_core: int, n_up: int, n_dn: int):
+    """The sorted excitation lists of init_excite_generator
+    (src/excite/init.rs:81-249): valence = the orbitals after the n_core lowest
+    h_pp (stable), targets looped in valence order and stably sorted by
+    descending magnitude; each entry carries sum_remaining_h_squared
+    (compute_sum_remaining, :288-309)."""
+    n = ints.n_orb
+    order = sorted(range(n), key=lambda p: ints.h1(p, p))
+    val = order[n_core:]
+
+    def finish(v):
+        v.sort(key=lambda e: -e[0])
+        rem = [0.0] * len(v)
+        for i in range(len(v) - 2, -1, -1):
+            rem[i] = rem[i + 1] + v[i + 1][0] * v[i + 1][0]
+        return [(a, t, r) for (a, t), r in zip(v, rem)]
+
+    opp, same, sing = {}, {}, {}
+    for p in val:
+        for q in val:
+            opp[(p, q)] = finish([(abs(ints.eri(p, r, q, s)), (r, s)) for r in val if r != p for s in val if s != q])
+    for p in val:
+        for q in val:
+            if p < q:
+                same[(p, q)] = finish([(abs(ints.eri(p, r, q, s) - ints.eri(p, s, q, r)), (r, s))
+                                       for r in val if r not in (p, q) for s in val if r < s and s not in (p, q)])
+    for p in val:
+        v = []
+        for r in val:
+            if r == p:
+                continue
+            vs = sorted(ints.eri(p, r, q, q) - ints.eri(p, q, q, r) for q in range(n) if q not in (p, r))
+            vo = sorted(ints.eri(p, r, q, q) for q in range(n) if q not in (p, r))
+            lo = ints.h1(p, r) + sum(vs[:n_up - 1]) + sum(vo[:n_dn])
+            hi = ints.h1(p, r) + sum(vs[len(vs) - (n_up - 1):]) + sum(vo[len(vs) - n_dn:])
+            v.append((abs(lo) if abs(lo) > abs(hi) else abs(hi), r))
+        sing[p] = finish(v)
+    mx = lambda d: max((l[0][0] for l in d.values() if l), default=0.0)
+    return sum(1 << p for p in val), opp, same, sing, mx(opp), mx(same), mx(sing)
+
+
+def _bits_asc(x: int):
+    while x:
+        low = x & -x
+        yield low.bit_length() - 1
+        x ^= low
+
+
+def matmul_external_dtm(ints, dets: np.ndarray, coeffs: np.ndarray, eps: float, n_core: int, n_up: int,
+                        n_dn: int):
+    """Wf::approx_matmul_external_dtm_only (src/wf/mod.rs:186-339): for each
+    determinant (in order) with local = eps / |c|, every list whose maximum is
+    >= local is walked until its first entry below local (that entry's
+    sum_remaining_h_squared is added to the determinant's remainder), targets
+    already occupied are skipped, singles are kept when |H_ai| >= local, and
+    every target outside the space receives H_ai c_i (summed in discovery
+    order; targets in first-discovery order, add_det_with_coeff,
+    :144-165).  Returns (external words (k, 4), coefficients, the index of each
+    target's first discoverer and its excitation, c_i^2 * remainder per
+    determinant).  Elements come from the C++ oracle."""
+    valence, opp, same, sing, max_opp, max_same, max_sing = _excite_lists(ints, n_core, n_up, n_dn)
+    H = pyoracle.OracleHam(ints)
+    space = {(int(w[0]) | int(w[1]) << 64, int(w[2]) | int(w[3]) << 64) for w in dets}
+    cand = []  # (target, source index, excitation, single)
+    rem_out = np.zeros(len(dets))
+    for k, w in enumerate(dets):
+        up, dn, c = int(w[0]) | int(w[1]) << 64, int(w[2]) | int(w[3]) << 64, float(coeffs[k])
+        local = eps / abs(c) if c != 0.0 else np.inf
+        rem = 0.0
+        if max_opp >= local:
+            for i in _bits_asc(valence & up):
+                for j in _bits_asc(valence & dn):
+                    for a, (r, s), rr in opp[(i, j)]:
+                        if a < local:
+                            rem += rr
+                            break
+                        if (up >> r) & 1 or (dn >> s) & 1:
+                            continue
+                        cand.append(((up ^ (1 << i) | (1 << r), dn ^ (1 << j) | (1 << s)), k, ("opp", (i, j), (r, s)),
+                                     False))
+        if max_same >= local:
+            for cfg, alpha in ((up, True), (dn, False)):
+                b = list(_bits_asc(valence & cfg))
+                for x in range(len(b)):
+                    for y in range(x + 1, len(b)):
+                        i, j = b[x], b[y]
+                        for a, (r, s), rr in same[(i, j)]:
+                            if a < local:
+                                rem += rr
+                                break
+                            if (cfg >> r) & 1 or (cfg >> s) & 1:
+                                continue
+                            nw = cfg ^ (1 << i) ^ (1 << j) | (1 << r) | (1 << s)
+                            cand.append(((nw, dn) if alpha else (up, nw), k, ("same", (i, j), (r, s), alpha), False))
+        if max_sing >= local:
+            for cfg, alpha in ((up, True), (dn, False)):
+                for i in _bits_asc(valence & cfg):
+                    for a, r, rr in sing[i]:
+                        if a < local:
+                            rem += rr
+                            break
+                        if (cfg >> r) & 1:
+                            continue
+                        nw = cfg ^ (1 << i) | (1 << r)
+                        cand.append(((nw, dn) if alpha else (up, nw), k, ("sing", i, r, alpha), True))
+        rem_out[k] = c * c * rem
+    if not cand:
+        return np.zeros((0, 4), dtype=np.uint64), np.zeros(0), [], rem_out
+    src = np.array([dets[k] for _, k, _, _ in cand], dtype=np.uint64)
+    tgt = _words([t[0] for t, _, _, _ in cand], [t[1] for t, _, _, _ in cand])
+    h, _ = H.off_diag(src, tgt)
+    out: dict[tuple[int, int], list] = {}
+    for (t, k, exc, single), hv in zip(cand, h):
+        c = float(coeffs[k])
+        local = eps / abs(c)
+        if single and not abs(hv) >= local:
+            continue
+        if t in space:
+            continue
+        if t in out:
+            out[t][0] += float(hv) * c
+        else:
+            out[t] = [float(hv) * c, k, exc]
+    keys = list(out)
+    words = _words([t[0] for t in keys], [t[1] for t in keys])
+    return words, np.array([out[t][0] for t in keys]), [(out[t][1], out[t][2]) for t in keys], rem_out
+
+
 def epstein_nesbet_dense(h_full: np.ndarray, in_space: np.ndarray, coeffs: np.ndarray, e0: float,
                          allowed: np.ndarray | None = None) -> float:
     """Closed form on a complete space: v = H[:, space] c, summed over the

@@ -49,12 +49,18 @@ struct SelectLists {
     double max_opp = 0.0, max_same = 0.0, max_sing = 0.0, eps_start = 0.0;
     uint64_t vmask[2] = {0, 0};
     double *d_oh = nullptr, *d_sh = nullptr, *d_gh = nullptr;
+    // sum of the squared magnitudes after each entry of its list
+    // (sum_remaining_h_squared, src/excite/init.rs:282-309)
+    double *d_or = nullptr, *d_sr = nullptr, *d_gr = nullptr;
     uint16_t *d_ot = nullptr, *d_st = nullptr, *d_gt = nullptr;  // targets r | s << 8
     int32_t *d_vpos = nullptr, *d_spos = nullptr;
     ~SelectLists() {
         hipFree(d_oh);
         hipFree(d_sh);
         hipFree(d_gh);
+        hipFree(d_or);
+        hipFree(d_sr);
+        hipFree(d_gr);
         hipFree(d_ot);
         hipFree(d_st);
         hipFree(d_gt);
@@ -81,6 +87,16 @@ struct DevSel {
     const double *oh, *sh, *gh;
     const uint16_t *ot, *st, *gt;
     const int32_t *vpos, *spos;
+    const double *orem, *srem, *grem;
+};
+
+DevSel dev_sel(const SelectLists& L) {
+    return DevSel{L.nv,   L.Lo,   L.Ls,   L.Lg,     L.max_opp, L.max_same, L.max_sing, L.d_oh, L.d_sh,
+                  L.d_gh, L.d_ot, L.d_st, L.d_gt,   L.d_vpos,  L.d_spos,   L.d_or,     L.d_sr, L.d_gr};
+}
+
+struct NoBreak {
+    __device__ void operator()(const double*, int64_t) const {}
 };
 
 template <class S>
@@ -97,9 +113,13 @@ __device__ __forceinline__ uint32_t exc_code(int kind, int alpha, int p, int q, 
 }
 
 // Walks one determinant's excitations in the reference's order; emit(up, dn,
-// code) receives every candidate.
-template <class S, class Emit>
-__device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, double local, bool singles, Emit emit) {
+// code) receives every candidate.  brk(rem, e) is told where a list was cut
+// (rem = that list's sum_remaining_h_squared array, e = the first entry below
+// local).  A single is kept when |H| > local (find_new_dets), or with
+// singles_ge when |H| >= local (approx_matmul_external_dtm_only).
+template <class S, class Emit, class Brk = NoBreak>
+__device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, double local, bool singles, Emit emit,
+                     Brk brk = Brk(), bool singles_ge = false) {
     if (L.max_opp >= local) {
         for (S a = vmask & up; a; a &= a - (S)1) {
             const int i = ctz(a);
@@ -107,7 +127,10 @@ __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, dou
                 const int j = ctz(b);
                 const int64_t base = ((int64_t)L.vpos[i] * L.nv + L.vpos[j]) * L.Lo;
                 for (int64_t e = 0; e < L.Lo; ++e) {
-                    if (L.oh[base + e] < local) break;
+                    if (L.oh[base + e] < local) {
+                        brk(L.orem, base + e);
+                        break;
+                    }
                     const uint16_t t = L.ot[base + e];
                     const int r = t & 0xFF, s = t >> 8;
                     if (((up >> r) & (S)1) || ((dn >> s) & (S)1)) continue;
@@ -125,7 +148,10 @@ __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, dou
                     const int j = ctz(b);
                     const int64_t base = (int64_t)L.spos[L.vpos[i] * L.nv + L.vpos[j]] * L.Ls;
                     for (int64_t e = 0; e < L.Ls; ++e) {
-                        if (L.sh[base + e] < local) break;
+                        if (L.sh[base + e] < local) {
+                            brk(L.srem, base + e);
+                            break;
+                        }
                         const uint16_t t = L.st[base + e];
                         const int r = t & 0xFF, s = t >> 8;
                         if (((cfg >> r) & (S)1) || ((cfg >> s) & (S)1)) continue;
@@ -147,12 +173,16 @@ __device__ void walk(const DevSel& L, const DevInts& T, S vmask, S up, S dn, dou
                 const int i = ctz(a);
                 const int64_t base = (int64_t)L.vpos[i] * L.Lg;
                 for (int64_t e = 0; e < L.Lg; ++e) {
-                    if (L.gh[base + e] < local) break;
+                    if (L.gh[base + e] < local) {
+                        brk(L.grem, base + e);
+                        break;
+                    }
                     const int r = L.gt[base + e] & 0xFF;
                     if ((cfg >> r) & (S)1) continue;
                     const S nw = (cfg ^ bit<S>(i)) | bit<S>(r);
                     // ham_sing(D, new) (src/ham/mod.rs:135-168)
-                    if (fabs(elem_single<S>(T, cfg, nw, other)) > local) {
+                    const double hs = fabs(elem_single<S>(T, cfg, nw, other));
+                    if (singles_ge ? hs >= local : hs > local) {
                         const uint32_t code = exc_code(2, spin == 0, i, 0, r, 0);
                         if (spin == 0)
                             emit(nw, dn, code);
@@ -384,6 +414,86 @@ __global__ void k_pt_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const
         ++cnt;
     });
     if (!FILL) counts[k] = cnt;
+}
+
+// approx_matmul_external_dtm_only (src/wf/mod.rs:186-339), one thread per
+// source determinant k with local = eps / |c_k|: the selection walk (singles
+// kept when |H| >= local), each candidate with H_ak c_k, its source and its
+// excitation code; the count pass also writes c_k^2 times the sum of
+// sum_remaining_h_squared over the lists it cut (out_sum_remaining).
+template <class S, bool FILL>
+__global__ void k_ext_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const uint64_t* __restrict__ dets,
+                           const double* __restrict__ coeffs, int64_t n, double eps, int64_t* counts,
+                           double* sum_rem, const int64_t* __restrict__ offs, uint64_t* words, double* vals,
+                           int64_t* src, uint32_t* codes) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const S up = load_string<S>(dets + 4 * k), dn = load_string<S>(dets + 4 * k + 2);
+    S vmask;
+    if constexpr (sizeof(S) > 8)
+        vmask = ((S)vm1 << 64) | (S)vm0;
+    else
+        vmask = (S)vm0;
+    const double c = coeffs[k];
+    const double local = eps / fabs(c);  // +inf for c = 0: nothing passes
+    int64_t cnt = 0;
+    double rem = 0.0;
+    const int64_t o = FILL ? offs[k] : 0;
+    walk<S>(
+        L, T, vmask, up, dn, local, true,
+        [&](S nu, S nd, uint32_t code) {
+            if (FILL) {
+                const int kind = code & 3, alpha = (code >> 2) & 1;
+                double h;
+                if (kind == 0)
+                    h = elem_opp_double<S>(T, up, nu, dn, nd);
+                else if (kind == 1)
+                    h = alpha ? elem_same_double<S>(T, up, nu) : elem_same_double<S>(T, dn, nd);
+                else
+                    h = alpha ? elem_single<S>(T, up, nu, dn) : elem_single<S>(T, dn, nd, up);
+                store_string<S>(words + 4 * (o + cnt), nu);
+                store_string<S>(words + 4 * (o + cnt) + 2, nd);
+                vals[o + cnt] = h * c;
+                src[o + cnt] = k;
+                codes[o + cnt] = code;
+            }
+            ++cnt;
+        },
+        [&](const double* r, int64_t e) {
+            if (!FILL) rem += r[e];
+        },
+        true);
+    if (!FILL) {
+        counts[k] = cnt;
+        if (sum_rem) sum_rem[k] = c * c * rem;
+    }
+}
+
+// kept run u (kpos[u] its rank among them): its first discovery, the
+// candidate index of its head in the stable word order
+__global__ void k_run_first(const int64_t* cidx, const int64_t* start, const int64_t* keep, const int64_t* kpos,
+                            int64_t runs, int64_t* first, int64_t* run_id) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= runs || !keep[u]) return;
+    first[kpos[u]] = cidx[start[u]];
+    run_id[kpos[u]] = u;
+}
+
+// output i (in first-discovery order): the run's determinant, its summed
+// coefficient and the diagonal its first discoverer gives it (new_diag)
+template <class S>
+__global__ void k_ext_out(DevInts T, const uint64_t* __restrict__ dets, const double* __restrict__ diag,
+                          const int64_t* __restrict__ src, const uint32_t* __restrict__ codes,
+                          const int64_t* __restrict__ first, const int64_t* __restrict__ run_id,
+                          const uint64_t* __restrict__ rw, const double* __restrict__ rv, int64_t k,
+                          uint64_t* out_w, double* out_v, double* out_d) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const int64_t u = run_id[i], f = first[i], j = src[f];
+    for (int q = 0; q < 4; ++q) out_w[4 * i + q] = rw[4 * u + q];
+    out_v[i] = rv[u];
+    if (out_d)
+        out_d[i] = new_diag<S>(T, load_string<S>(dets + 4 * j), load_string<S>(dets + 4 * j + 2), diag[j], codes[f]);
 }
 
 __global__ void k_iota64(int64_t* v, int64_t n) {
@@ -673,6 +783,25 @@ int hci_excite_init(hci_ctx* c, int n_core, int n_up, int n_dn, double eps_var, 
         }
         if (!v.empty()) L->max_sing = std::max(L->max_sing, v[0].a);
     }
+    // sum_remaining_h_squared of every list entry (compute_sum_remaining,
+    // src/excite/init.rs:288-309): 0 for the last entry, then backwards
+    // rem[e] = rem[e + 1] + a[e + 1] * a[e + 1] (no fused multiply-add, as in Rust)
+    auto suffix = [](const std::vector<double>& a, int64_t len, std::vector<double>& rem) {
+        rem.assign(a.size(), 0.0);
+        if (len <= 0) return;
+        for (size_t base = 0; base < a.size(); base += (size_t)len) {
+#pragma clang fp contract(off)
+            rem[base + len - 1] = 0.0;
+            for (int64_t e = len - 2; e >= 0; --e) {
+                const double sq = a[base + e + 1] * a[base + e + 1];
+                rem[base + e] = rem[base + e + 1] + sq;
+            }
+        }
+    };
+    std::vector<double> orem, srem, grem;
+    suffix(oh, L->Lo, orem);
+    suffix(sh, L->Ls, srem);
+    suffix(gh, L->Lg, grem);
     // initial epsilon from Hartree-Fock (src/wf/eps.rs:44-171, z_sym = 1)
     {
         auto occ = [](int p, int ne) { return p < ne; };  // Hartree-Fock: the lowest orbitals by index
@@ -725,7 +854,8 @@ int hci_excite_init(hci_ctx* c, int n_core, int n_up, int n_dn, double eps_var, 
     int rc;
     if ((rc = up(&L->d_oh, oh)) || (rc = up(&L->d_ot, ot)) || (rc = up(&L->d_sh, sh)) || (rc = up(&L->d_st, st)) ||
         (rc = up(&L->d_gh, gh)) || (rc = up(&L->d_gt, gt)) || (rc = up(&L->d_vpos, vpos)) ||
-        (rc = up(&L->d_spos, spos)))
+        (rc = up(&L->d_spos, spos)) || (rc = up(&L->d_or, orem)) || (rc = up(&L->d_sr, srem)) ||
+        (rc = up(&L->d_gr, grem)))
         return rc;
     if (eps_start) *eps_start = L->eps_start;
     return HCI_OK;
@@ -748,8 +878,7 @@ int hci_select(hci_ctx* c, const uint64_t* dets, const double* coeffs, const dou
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const SelectLists& L = *c->sel;
-    DevSel D{L.nv, L.Lo, L.Ls, L.Lg, L.max_opp, L.max_same, L.max_sing, L.d_oh, L.d_sh, L.d_gh,
-             L.d_ot, L.d_st, L.d_gt, L.d_vpos, L.d_spos};
+    const DevSel D = dev_sel(L);
     const DevInts T = c->ints();
     Bufs b;
     Tmp tmp;
@@ -869,8 +998,7 @@ int hci_pt2_dtm(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t 
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const SelectLists& L = *c->sel;
-    DevSel D{L.nv, L.Lo, L.Ls, L.Lg, L.max_opp, L.max_same, L.max_sing, L.d_oh, L.d_sh, L.d_gh,
-             L.d_ot, L.d_st, L.d_gt, L.d_vpos, L.d_spos};
+    const DevSel D = dev_sel(L);
     const DevInts T = c->ints();
     const bool wide = c->n_orb > 64;
     const int nw = wide ? 4 : 2;
@@ -987,6 +1115,148 @@ int hci_pt2_dtm(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t 
     }
     *e2 = sum;
     if (n_ext) *n_ext = n_out;
+    return HCI_OK;
+}
+
+int hci_matmul_external(hci_ctx* c, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
+                        double eps, uint64_t* out, double* out_coeffs, double* out_diag, int64_t cap, int64_t* count,
+                        double* sum_remaining, int mem) {
+    if (!c) return HCI_ERR_ARG;
+    if (!c->sel) return hci_fail(c, HCI_ERR_STATE, "hci_matmul_external: call hci_excite_init first");
+    if (n < 1 || !dets || !coeffs || !count || cap < 0 || (cap > 0 && (!out || !out_coeffs)) || !(eps > 0.0) ||
+        (out_diag && !diag))
+        return hci_fail(c, HCI_ERR_ARG, "hci_matmul_external: bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const SelectLists& L = *c->sel;
+    const DevSel D = dev_sel(L);
+    const DevInts T = c->ints();
+    const bool wide = c->n_orb > 64;
+    const int nw = wide ? 4 : 2;
+    const hipMemcpyKind out_kind = mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    Bufs b;
+    Tmp tmp;
+    const uint64_t* d_dets = dets;
+    const double* d_coef = coeffs;
+    const double* d_diag = diag;
+    if (mem == HCI_MEM_HOST) {
+        uint64_t* w = nullptr;
+        double* q = nullptr;
+        HIP_TRY(c, b.alloc(&w, (size_t)n * 4));
+        HIP_TRY(c, b.alloc(&q, (size_t)n));
+        HIP_TRY(c, hipMemcpyAsync(w, dets, (size_t)n * 32, hipMemcpyHostToDevice, s));
+        HIP_TRY(c, hipMemcpyAsync(q, coeffs, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        d_dets = w;
+        d_coef = q;
+        if (out_diag) {
+            double* g = nullptr;
+            HIP_TRY(c, b.alloc(&g, (size_t)n));
+            HIP_TRY(c, hipMemcpyAsync(g, diag, (size_t)n * 8, hipMemcpyHostToDevice, s));
+            d_diag = g;
+        }
+    }
+    auto scan = [&](const int64_t* in, int64_t* o, int64_t len) -> hipError_t {
+        size_t need = 0;
+        hipError_t e = rocprim::exclusive_scan(nullptr, need, in, o, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
+        if (e != hipSuccess) return e;
+        if ((e = tmp.ensure(need)) != hipSuccess) return e;
+        return rocprim::exclusive_scan(tmp.p, need, in, o, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
+    };
+    int64_t *cnt = nullptr, *off = nullptr, *didx = nullptr;
+    double* srem = nullptr;
+    HIP_TRY(c, b.alloc(&cnt, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&off, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&didx, (size_t)n));
+    if (sum_remaining) HIP_TRY(c, b.alloc(&srem, (size_t)n));
+    HIP_TRY(c, sort_dets(tmp, b, d_dets, n, nw, didx, s));
+    if (wide)
+        hipLaunchKernelGGL((k_ext_walk<u128, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                           d_dets, d_coef, n, eps, cnt, srem, nullptr, nullptr, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_ext_walk<uint64_t, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                           d_dets, d_coef, n, eps, cnt, srem, nullptr, nullptr, nullptr, nullptr, nullptr);
+    HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
+    HIP_TRY(c, scan(cnt, off, n + 1));
+    if (sum_remaining) HIP_TRY(c, hipMemcpyAsync(sum_remaining, srem, (size_t)n * 8, out_kind, s));
+    int64_t m = 0;
+    HIP_TRY(c, hipMemcpyAsync(&m, off + n, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (m == 0) {
+        *count = 0;
+        return HCI_OK;
+    }
+    uint64_t *cw = nullptr, *rw = nullptr;
+    double *cv = nullptr, *rv = nullptr;
+    int64_t *csrc = nullptr, *cidx = nullptr, *head = nullptr, *hpos = nullptr, *start = nullptr, *keep = nullptr,
+            *kpos = nullptr;
+    uint32_t* ccode = nullptr;
+    HIP_TRY(c, b.alloc(&cw, (size_t)m * 4));
+    HIP_TRY(c, b.alloc(&cv, (size_t)m));
+    HIP_TRY(c, b.alloc(&csrc, (size_t)m));
+    HIP_TRY(c, b.alloc(&ccode, (size_t)m));
+    if (wide)
+        hipLaunchKernelGGL((k_ext_walk<u128, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                           d_dets, d_coef, n, eps, nullptr, nullptr, off, cw, cv, csrc, ccode);
+    else
+        hipLaunchKernelGGL((k_ext_walk<uint64_t, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
+                           d_dets, d_coef, n, eps, nullptr, nullptr, off, cw, cv, csrc, ccode);
+    HIP_TRY(c, hipGetLastError());
+    // stable sort by words: each target's contributions stay in discovery
+    // order (add_det_with_coeff, src/wf/mod.rs:144-165), summed per run; runs
+    // of determinants in the space are dropped
+    HIP_TRY(c, b.alloc(&cidx, (size_t)m));
+    HIP_TRY(c, sort_dets(tmp, b, cw, m, nw, cidx, s));
+    HIP_TRY(c, b.alloc(&head, (size_t)m + 1));
+    HIP_TRY(c, b.alloc(&hpos, (size_t)m + 1));
+    hipLaunchKernelGGL(k_run_heads, grid_for(m, 256), 256, 0, s, cw, cidx, m, head);
+    HIP_TRY(c, hipMemsetAsync(head + m, 0, 8, s));
+    HIP_TRY(c, scan(head, hpos, m + 1));
+    int64_t runs = 0;
+    HIP_TRY(c, hipMemcpyAsync(&runs, hpos + m, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    HIP_TRY(c, b.alloc(&start, (size_t)runs + 1));
+    HIP_TRY(c, b.alloc(&rw, (size_t)runs * 4));
+    HIP_TRY(c, b.alloc(&rv, (size_t)runs));
+    HIP_TRY(c, b.alloc(&keep, (size_t)runs + 1));
+    HIP_TRY(c, b.alloc(&kpos, (size_t)runs + 1));
+    hipLaunchKernelGGL(k_run_starts, grid_for(m + 1, 256), 256, 0, s, head, hpos, m, start);
+    hipLaunchKernelGGL(k_run_sum, grid_for(runs, 256), 256, 0, s, cw, cv, cidx, start, runs, d_dets, didx, n, rw, rv,
+                       keep);
+    HIP_TRY(c, hipMemsetAsync(keep + runs, 0, 8, s));
+    HIP_TRY(c, scan(keep, kpos, runs + 1));
+    int64_t k = 0;
+    HIP_TRY(c, hipMemcpyAsync(&k, kpos + runs, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    *count = k;
+    if (k > cap) return hci_fail(c, HCI_ERR_ARG, "hci_matmul_external: output capacity too small (count holds the need)");
+    if (k == 0) return HCI_OK;
+    // first-discovery order: kept runs sorted by their head's candidate index
+    int64_t *first = nullptr, *rid = nullptr, *first2 = nullptr, *rid2 = nullptr;
+    HIP_TRY(c, b.alloc(&first, (size_t)k));
+    HIP_TRY(c, b.alloc(&rid, (size_t)k));
+    HIP_TRY(c, b.alloc(&first2, (size_t)k));
+    HIP_TRY(c, b.alloc(&rid2, (size_t)k));
+    hipLaunchKernelGGL(k_run_first, grid_for(runs, 256), 256, 0, s, cidx, start, keep, kpos, runs, first, rid);
+    size_t need = 0;
+    HIP_TRY(c, rocprim::radix_sort_pairs(nullptr, need, first, first2, rid, rid2, (size_t)k, 0, 64, s));
+    HIP_TRY(c, tmp.ensure(need));
+    HIP_TRY(c, rocprim::radix_sort_pairs(tmp.p, need, first, first2, rid, rid2, (size_t)k, 0, 64, s));
+    uint64_t* ow = nullptr;
+    double *ov = nullptr, *od = nullptr;
+    HIP_TRY(c, b.alloc(&ow, (size_t)k * 4));
+    HIP_TRY(c, b.alloc(&ov, (size_t)k));
+    if (out_diag) HIP_TRY(c, b.alloc(&od, (size_t)k));
+    if (wide)
+        hipLaunchKernelGGL(k_ext_out<u128>, grid_for(k, 256), 256, 0, s, T, d_dets, d_diag, csrc, ccode, first2, rid2,
+                           rw, rv, k, ow, ov, od);
+    else
+        hipLaunchKernelGGL(k_ext_out<uint64_t>, grid_for(k, 256), 256, 0, s, T, d_dets, d_diag, csrc, ccode, first2,
+                           rid2, rw, rv, k, ow, ov, od);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(out, ow, (size_t)k * 32, out_kind, s));
+    HIP_TRY(c, hipMemcpyAsync(out_coeffs, ov, (size_t)k * 8, out_kind, s));
+    if (out_diag) HIP_TRY(c, hipMemcpyAsync(out_diag, od, (size_t)k * 8, out_kind, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
     return HCI_OK;
 }
 

@@ -128,6 +128,8 @@ SIGNATURES = {
                              C.POINTER(C.c_int64), C.c_int]),
     "hci_pt2_dtm": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_double, C.c_int64, C.POINTER(C.c_double),
                               C.POINTER(C.c_int64), C.c_int]),
+    "hci_matmul_external": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, _P, _P, _P, C.c_int64,
+                                      C.POINTER(C.c_int64), _P, C.c_int]),
 }
 
 _lib = None
@@ -295,6 +297,34 @@ class HciContext:
         self._check(self.lib.hci_pt2_dtm(self.handle, _ptr(dets), _ptr(coeffs), dets.shape[0], e0, eps, batch,
                                          C.byref(e2), C.byref(n_ext), MEM_HOST))
         return e2.value, n_ext.value
+
+    def matmul_external(self, dets: np.ndarray, coeffs: np.ndarray, diag: np.ndarray, eps: float):
+        """The screened external product of Wf::approx_matmul_external_dtm_only
+        (src/wf/mod.rs:186-339): (external determinants (k, 4) uint64 in
+        first-discovery order, their coefficients sum_i H_ai c_i, their
+        diagonals from the first discoverer, and per source determinant
+        c_i^2 times the squared magnitudes left below the cut).  Needs
+        excite_init on the same integrals."""
+        dets = np.ascontiguousarray(dets, dtype=np.uint64)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        n = dets.shape[0]
+        cnt = C.c_int64()
+        rem = np.empty(n)
+        cap = max(1024, 8 * n)
+        while True:
+            out = np.empty((cap, 4), dtype=np.uint64)
+            oc = np.empty(cap)
+            od = np.empty(cap)
+            rc = self.lib.hci_matmul_external(self.handle, _ptr(dets), _ptr(coeffs), _ptr(diag), n, eps, _ptr(out),
+                                              _ptr(oc), _ptr(od), cap, C.byref(cnt), _ptr(rem), MEM_HOST)
+            if rc == HCI_OK:
+                k = cnt.value
+                return out[:k], oc[:k], od[:k], rem
+            if cnt.value > cap:
+                cap = cnt.value
+                continue
+            self._check(rc)
 
     # -- timing ------------------------------------------------------------------
     def record(self, slot: int) -> None:

@@ -94,3 +94,68 @@ def test_pt2_after_hci_on_reference_examples(gpu_ctx, name):
         assert n_ext == on
         assert abs(e2 - o2) <= 1e-11 * abs(o2)
     assert n_ext > 0 and e2 < 0
+
+
+def _check_external(gpu_ctx, ints, dets, c, eps, n_core, n_up, n_dn):
+    from risq_amd.hci_select import new_diag
+    diag = gpu_ctx.ham_diag(dets)
+    gpu_ctx.excite_init(n_core, n_up, n_dn, 1e-3)
+    w, v, d, rem = gpu_ctx.matmul_external(dets, c, diag, eps)
+    ow, ov, first, orem = pt2_oracle.matmul_external_dtm(ints, dets, c, eps, n_core, n_up, n_dn)
+    print(f"eps={eps:g} n_core={n_core}: {w.shape[0]} external determinants (oracle {ow.shape[0]})")
+    assert w.shape[0] == ow.shape[0] > 0
+    assert np.array_equal(w, ow)  # the same determinants in the same first-discovery order
+    # each coefficient sums the same products in the same (discovery) order
+    assert np.max(np.abs(v - ov) / np.maximum(np.abs(ov), 1e-300)) < 1e-14
+    # the first discoverer's new_diag, as the host restatement computes it
+    ref = np.array([new_diag(ints, (int(dets[k][0]) | int(dets[k][1]) << 64, int(dets[k][2]) | int(dets[k][3]) << 64),
+                             float(diag[k]), exc) for k, exc in first])
+    assert np.array_equal(d, ref)
+    assert np.max(np.abs(d - gpu_ctx.ham_diag(w))) < 1e-10
+    assert np.max(np.abs(rem - orem) / np.maximum(np.abs(orem), 1e-300)) < 1e-13
+
+
+@pytest.mark.parametrize("eps", [1e-5, 1e-7])
+@pytest.mark.parametrize("n_core", [0, 1])
+def test_external_product_matches_oracle(gpu_ctx, eps, n_core):
+    """hci_matmul_external (approx_matmul_external_dtm_only) against the
+    list-walking oracle: determinants and order exact, coefficients and
+    remainders to the last bits, diagonals bit-identical to the new_diag
+    restatement."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    if n_core:
+        dets = dets[(dets[:, 0] & dets[:, 2] & np.uint64(1)) == 1][:600]
+    else:
+        dets = dets[:600]  # the oracle walks ~1e5 candidates per 100 sources in Python
+    e0, c = _solve(gpu_ctx, ints, dets)
+    _check_external(gpu_ctx, ints, dets, c, eps, n_core, 4, 4)
+
+
+def test_external_product_two_word_strings(gpu_ctx):
+    ints, dets = mol_space(70, 8, 5, 3, 3, 1500)
+    dets = dets[:120]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    _check_external(gpu_ctx, ints, dets, c, 1e-5, 0, 3, 3)
+
+
+def test_external_product_capacity(gpu_ctx):
+    """Too small an output capacity fails with the needed count; the
+    remainders are written either way."""
+    import ctypes as C
+    from risq_amd.hci import HCI_OK, MEM_HOST
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    dets = dets[:300]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    diag = gpu_ctx.ham_diag(dets)
+    gpu_ctx.excite_init(0, 4, 4, 1e-3)
+    w, v, d, rem = gpu_ctx.matmul_external(dets, c, diag, 1e-6)
+    out = np.empty((4, 4), dtype=np.uint64)
+    oc, od, rem2 = np.empty(4), np.empty(4), np.empty(dets.shape[0])
+    cnt = C.c_int64()
+    dw = np.ascontiguousarray(dets, dtype=np.uint64)
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    rc = gpu_ctx.lib.hci_matmul_external(gpu_ctx.handle, dw.ctypes.data, cc.ctypes.data, diag.ctypes.data,
+                                         dets.shape[0], 1e-6, out.ctypes.data, oc.ctypes.data, od.ctypes.data, 4,
+                                         C.byref(cnt), rem2.ctypes.data, MEM_HOST)
+    assert rc != HCI_OK and cnt.value == w.shape[0] > 4
+    assert np.array_equal(rem2, rem)
