@@ -120,40 +120,49 @@ def _excite_lists(ints, n_core: int, n_up: int, n_dn: int):
     (src/excite/init.rs:81-249): valence = the orbitals after the n_core lowest
     h_pp (stable), targets looped in valence order and stably sorted by
     descending magnitude; each entry carries sum_remaining_h_squared
-    (compute_sum_remaining, :288-309)."""
+    (compute_sum_remaining, :288-309: 0 for the last entry, then a running sum
+    of the following entries' squares from the end, added one at a time)."""
     n = ints.n_orb
+    h, eri = ints.dense()
     order = sorted(range(n), key=lambda p: ints.h1(p, p))
-    val = order[n_core:]
+    val = np.array(order[n_core:], dtype=np.int64)
 
-    def finish(v):
-        v.sort(key=lambda e: -e[0])
-        rem = [0.0] * len(v)
-        for i in range(len(v) - 2, -1, -1):
-            rem[i] = rem[i + 1] + v[i + 1][0] * v[i + 1][0]
-        return [(a, t, r) for (a, t), r in zip(v, rem)]
+    def finish(a, targets):
+        idx = np.argsort(-a, kind="stable")
+        a = a[idx]
+        rem = np.zeros(a.size)
+        if a.size > 1:
+            rem[:-1] = np.cumsum((a[1:] * a[1:])[::-1])[::-1]  # sequential, from the end
+        return [(float(x), targets[i], float(r)) for x, i, r in zip(a, idx, rem)]
 
+    vr, vs = np.meshgrid(val, val, indexing="ij")  # r outer, s inner
     opp, same, sing = {}, {}, {}
-    for p in val:
-        for q in val:
-            opp[(p, q)] = finish([(abs(ints.eri(p, r, q, s)), (r, s)) for r in val if r != p for s in val if s != q])
-    for p in val:
-        for q in val:
+    for p in val.tolist():
+        for q in val.tolist():
+            m = (vr != p) & (vs != q)
+            a = np.abs(eri[p, vr[m], q, vs[m]])
+            opp[(p, q)] = finish(a, list(zip(vr[m].tolist(), vs[m].tolist())))
+    for p in val.tolist():
+        for q in val.tolist():
             if p < q:
-                same[(p, q)] = finish([(abs(ints.eri(p, r, q, s) - ints.eri(p, s, q, r)), (r, s))
-                                       for r in val if r not in (p, q) for s in val if r < s and s not in (p, q)])
-    for p in val:
-        v = []
-        for r in val:
+                m = (vr != p) & (vr != q) & (vs != p) & (vs != q) & (vr < vs)  # r < s as orbital indices
+                a = np.abs(eri[p, vr[m], q, vs[m]] - eri[p, vs[m], q, vr[m]])
+                same[(p, q)] = finish(a, list(zip(vr[m].tolist(), vs[m].tolist())))
+    for p in val.tolist():
+        v, tg = [], []
+        for r in val.tolist():
             if r == p:
                 continue
-            vs = sorted(ints.eri(p, r, q, q) - ints.eri(p, q, q, r) for q in range(n) if q not in (p, r))
-            vo = sorted(ints.eri(p, r, q, q) for q in range(n) if q not in (p, r))
-            lo = ints.h1(p, r) + sum(vs[:n_up - 1]) + sum(vo[:n_dn])
-            hi = ints.h1(p, r) + sum(vs[len(vs) - (n_up - 1):]) + sum(vo[len(vs) - n_dn:])
-            v.append((abs(lo) if abs(lo) > abs(hi) else abs(hi), r))
-        sing[p] = finish(v)
+            qs = [q for q in range(n) if q not in (p, r)]
+            vsame = sorted((eri[p, r, qs, qs] - eri[p, qs, qs, r]).tolist())
+            vopp = sorted(eri[p, r, qs, qs].tolist())
+            lo = float(h[p, r]) + sum(vsame[:n_up - 1]) + sum(vopp[:n_dn])
+            hi = float(h[p, r]) + sum(vsame[len(vsame) - (n_up - 1):]) + sum(vopp[len(vsame) - n_dn:])
+            v.append(abs(lo) if abs(lo) > abs(hi) else abs(hi))
+            tg.append(r)
+        sing[p] = finish(np.array(v), tg)
     mx = lambda d: max((l[0][0] for l in d.values() if l), default=0.0)
-    return sum(1 << p for p in val), opp, same, sing, mx(opp), mx(same), mx(sing)
+    return sum(1 << int(p) for p in val), opp, same, sing, mx(opp), mx(same), mx(sing)
 
 
 def _bits_asc(x: int):
