@@ -311,7 +311,9 @@ int hci_pt2_dtm(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_
  * sum_remaining_h_squared of every list entry where determinant i's walk was
  * cut (the sampler weights the reference returns).  *count receives the number
  * of external determinants; when it exceeds cap nothing is written but
- * sum_remaining and the call fails with *count set. */
+ * sum_remaining and the call fails with *count set.  Every candidate
+ * excitation is held on the device at once (about 52 B each plus sort
+ * scratch; no splitting into parts as in hci_pt2_dtm). */
 int hci_matmul_external(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
                         double eps, uint64_t* out, double* out_coeffs, double* out_diag, int64_t cap, int64_t* count,
                         double* sum_remaining, int mem);
