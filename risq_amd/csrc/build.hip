@@ -590,59 +590,90 @@ constexpr int kIlStage = kWinChunk + kWinChunk / 8;  // staged entries with the 
 __device__ __forceinline__ uint16_t fp16_of(float hi, float lo) {
     return __builtin_bit_cast(uint16_t, (_Float16)((double)hi + (double)lo));
 }
+typedef unsigned short il_u16x8 __attribute__((ext_vector_type(8)));
+typedef float il_f32x4 __attribute__((ext_vector_type(4)));
+typedef double il_f64x2 __attribute__((ext_vector_type(2)));
+template <bool F64>
 __global__ void __launch_bounds__(64 * kIlWaves)
     k_interleave(const Window* __restrict__ wins, int64_t n_win, uint16_t* idx, double* val, float* hi,
                  float* lo, uint16_t* h16) {
     // one staging area per wave: FP64 values, or FP32 hi and lo words, in
     // the same bytes (a build writes one or the other).  Entry j sits at
     // j + 8 (j / 64): the slot-order reads (entries 64 apart) then hit 64
-    // different banks.
-    __shared__ uint16_t s_idx[kIlWaves][kIlStage];
-    __shared__ double s_val[kIlWaves][kIlStage];
+    // different banks, and each lane's 8 consecutive entries stay contiguous.
+    // Each lane moves 8 entries with 16-byte vectors both ways.
+    __shared__ __align__(16) uint16_t s_idx[kIlWaves][kIlStage];
+    __shared__ __align__(16) double s_val[kIlWaves][kIlStage];
     uint16_t* si = s_idx[threadIdx.x >> 6];
     double* sv = s_val[threadIdx.x >> 6];
     float* sh = (float*)sv;
     float* sl = sh + kIlStage;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lp = 8 * lane + 8 * (lane >> 3);  // staged position of this lane's first entry
     for (int64_t k = (int64_t)blockIdx.x * kIlWaves + wv; k < n_win; k += (int64_t)gridDim.x * kIlWaves) {
     const Window W = wins[k];
     const int cnt = W.cnt();
     if (cnt <= kWinAlign) {  // one group: its slots are its positions
-        if (h16 && lane < cnt) h16[W.off + lane] = fp16_of(hi[W.off + lane], lo[W.off + lane]);
+        if (!F64 && h16 && lane < cnt) h16[W.off + lane] = fp16_of(hi[W.off + lane], lo[W.off + lane]);
         continue;
     }
     for (int base = 0; base < cnt; base += kWinChunk) {
         const int n = min(kWinChunk, cnt - base);
-        for (int t = lane; t < n; t += 64) {
-            const int64_t e = W.off + base + t;
-            const int pt = t + ((t >> 6) << 3);
-            si[pt] = idx[e];
-            if (val) {
-                sv[pt] = val[e];
+        // fill order in: lane l takes entries 8 l .. 8 l + 7 (slots up to a
+        // multiple of 8 past the last entry belong to the window)
+        const int64_t e0 = W.off + base + 8 * lane;
+        if (8 * lane < n) {
+            *(il_u16x8*)(si + lp) = *(const il_u16x8*)(idx + e0);
+            if constexpr (F64) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) *(il_f64x2*)(sv + lp + 2 * v) = *(const il_f64x2*)(val + e0 + 2 * v);
             } else {
-                sh[pt] = hi[e];
-                sl[pt] = lo[e];
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    *(il_f32x4*)(sh + lp + 4 * v) = *(const il_f32x4*)(hi + e0 + 4 * v);
+                    *(il_f32x4*)(sl + lp + 4 * v) = *(const il_f32x4*)(lo + e0 + 4 * v);
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // slot s of the piece holds entry inv(s): writes stay contiguous
+        // slot order out: lane l writes slots 8 l .. 8 l + 7, which hold
+        // entries 64 q + l of a full piece, or l + q R of the last one (slots
+        // past its entries are written as zeros)
         const bool full = n == kWinChunk;
         const int R = (n + kWinAlign - 1) / kWinAlign;
-        for (int t = lane; t < R * kWinAlign; t += 64) {
-            const int j = full ? (t & 7) * 64 + (t >> 3) : (t >> 3) + (t & 7) * R;
-            if (j >= n) continue;
-            const int64_t e = W.off + base + t;
-            const int pj = j + ((j >> 6) << 3);
-            idx[e] = si[pj];
-            if (val) {
-                val[e] = sv[pj];
+        if (full || lane < R) {
+            il_u16x8 ui, uh;
+            il_f32x4 h4[2], l4[2];
+            il_f64x2 d2[4];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int j = full ? q * 64 + lane : lane + q * R;
+                const bool ok = j < n;
+                const int pj = j + ((j >> 6) << 3);
+                ui[q] = ok ? si[pj] : (unsigned short)0;
+                if constexpr (F64) {
+                    d2[q >> 1][q & 1] = ok ? sv[pj] : 0.0;
+                } else {
+                    const float hq = ok ? sh[pj] : 0.f, lq = ok ? sl[pj] : 0.f;
+                    h4[q >> 2][q & 3] = hq;
+                    l4[q >> 2][q & 3] = lq;
+                    uh[q] = fp16_of(hq, lq);
+                }
+            }
+            const int64_t e = W.off + base + 8 * lane;
+            *(il_u16x8*)(idx + e) = ui;
+            if constexpr (F64) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) *(il_f64x2*)(val + e + 2 * v) = d2[v];
             } else {
-                const float h = sh[pj], l = sl[pj];
-                hi[e] = h;
-                lo[e] = l;
-                if (h16) h16[e] = fp16_of(h, l);
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    *(il_f32x4*)(hi + e + 4 * v) = h4[v];
+                    *(il_f32x4*)(lo + e + 4 * v) = l4[v];
+                }
+                if (h16) *(il_u16x8*)(h16 + e) = uh;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1335,9 +1366,15 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
                            o.single_rule, dbl, nullptr, fo, (unsigned long long*)d_nnz);
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
                        H->d_win);
-    if (n_win > 0)
-        hipLaunchKernelGGL(k_interleave, std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u), 64 * kIlWaves, 0, s,
-                           H->d_win, (int64_t)n_win, H->d_idx16, H->d_val, H->d_hi, H->d_lo, H->d_h16);
+    if (n_win > 0) {
+        const unsigned ilg = std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u);
+        if (H->d_val)
+            hipLaunchKernelGGL(k_interleave<true>, ilg, 64 * kIlWaves, 0, s, H->d_win, (int64_t)n_win, H->d_idx16,
+                               H->d_val, H->d_hi, H->d_lo, H->d_h16);
+        else
+            hipLaunchKernelGGL(k_interleave<false>, ilg, 64 * kIlWaves, 0, s, H->d_win, (int64_t)n_win, H->d_idx16,
+                               H->d_val, H->d_hi, H->d_lo, H->d_h16);
+    }
     if (!ta.empty())
         hipLaunchKernelGGL(k_task_remap, grid_for((int64_t)ta.size(), 256), 256, 0, s, H->d_task_a,
                            (int64_t)ta.size(), d_nidx);
