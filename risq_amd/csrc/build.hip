@@ -1420,7 +1420,6 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMalloc(&H->d_bpos, (size_t)N * 4 + 16));
     HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
     HIP_TRY(c, hipMalloc(&H->d_xb, (size_t)N * 8 + 16));
-    HIP_TRY(c, hipMalloc(&H->d_yb, (size_t)N * 8 + 16));
     if (H->d_h16) {
         HIP_TRY(c, hipMalloc(&H->d_x32, (size_t)N * 4 + 16));
         HIP_TRY(c, hipMalloc(&H->d_xb32, (size_t)N * 4 + 16));

@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(kTileThreads)
     k_spmv_win(const TileTask* __restrict__ tasks, const Window* __restrict__ wins,
                const uint16_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
                const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
-               double* __restrict__ y, const double* __restrict__ xdiag) {
+               double* __restrict__ y, const double* __restrict__ xdiag, const int32_t* __restrict__ ypos) {
     __shared__ double acc[kTileRows];
     __shared__ XT<MODE> xs[kTileWaves][kWinCols];
     __shared__ int next, next_small;
@@ -386,13 +386,11 @@ __global__ void __launch_bounds__(kTileThreads)
     __syncthreads();
     for (int r = tid; r < t.nr; r += kTileThreads) {
         const int64_t p = (int64_t)t.r0 + r;
-        y[p] = (diag && !ABS) ? diag[p] * xdiag[p] + acc[r] : acc[r];  // xdiag: x in FP64
+        if (ypos)  // beta-major rows: added to their alpha-major rows, which one launch owns
+            y[ypos[p]] += acc[r];
+        else
+            y[p] = (diag && !ABS) ? diag[p] * xdiag[p] + acc[r] : acc[r];  // xdiag: x in FP64
     }
-}
-
-__global__ void k_scatter_add(const int32_t* pos, const double* src, int64_t n, double* dst) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) dst[pos[q]] += src[q];
 }
 
 template <class T>
@@ -483,27 +481,27 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
             xa = (const double*)h->d_x32;
             xb = (double*)h->d_xb32;
         }
-        // beta-major part first (x gathered to beta-major order), then the
-        // alpha-major part writes y, then the beta-major rows are added
-        if (h->n_task_b > 0) {
-            if (!absval)
-                hipLaunchKernelGGL(k_gather_x<XT<MODE>>, grid_for(h->n, 256), 256, 0, s, h->d_bpos,
-                                   (const XT<MODE>*)xa, h->n, (XT<MODE>*)xb);
-            if (absval)
-                hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, h->d_yb, nullptr);
-            else
-                hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xb, nullptr, h->d_yb, nullptr);
-        }
+        // x gathered to beta-major order, the alpha-major part writes y,
+        // then the beta-major part adds each of its rows into its alpha-major
+        // row (stream order: the writes come first)
+        if (h->n_task_b > 0 && !absval)
+            hipLaunchKernelGGL(k_gather_x<XT<MODE>>, grid_for(h->n, 256), 256, 0, s, h->d_bpos,
+                               (const XT<MODE>*)xa, h->n, (XT<MODE>*)xb);
         if (absval)
             hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr);
+                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr, nullptr);
         else
             hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xa, h->d_diag, y, x);
-        if (h->n_task_b > 0)
-            hipLaunchKernelGGL(k_scatter_add, grid_for(h->n, 256), 256, 0, s, h->d_bpos, h->d_yb, h->n, y);
+                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xa, h->d_diag, y, x, nullptr);
+        if (h->n_task_b > 0) {
+            if (absval)
+                hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
+                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr,
+                                   h->d_bpos);
+            else
+                hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
+                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xb, nullptr, y, nullptr, h->d_bpos);
+        }
         return;
     }
     if constexpr (MODE == 3) return;  // FP16 copies exist in tiled storage only
