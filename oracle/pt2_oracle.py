@@ -115,17 +115,34 @@ def dtm_pt(ints, dets: np.ndarray, coeffs: np.ndarray, e0: float, eps: float, n_
     return e2, len(keys)
 
 
+class _Lists(dict):
+    """Excitation lists built on first use (a walk touches only the pairs of
+    occupied orbitals)."""
+
+    def __init__(self, make):
+        super().__init__()
+        self._make = make
+
+    def __missing__(self, key):
+        v = self._make(key)
+        self[key] = v
+        return v
+
+
 def _excite_lists(ints, n_core: int, n_up: int, n_dn: int):
     """The sorted excitation lists of init_excite_generator
     (src/excite/init.rs:81-249): valence = the orbitals after the n_core lowest
     h_pp (stable), targets looped in valence order and stably sorted by
     descending magnitude; each entry carries sum_remaining_h_squared
     (compute_sum_remaining, :288-309: 0 for the last entry, then a running sum
-    of the following entries' squares from the end, added one at a time)."""
+    of the following entries' squares from the end, added one at a time).
+    Lists are built when first read; the three maxima over all lists are
+    computed up front."""
     n = ints.n_orb
     h, eri = ints.dense()
     order = sorted(range(n), key=lambda p: ints.h1(p, p))
     val = np.array(order[n_core:], dtype=np.int64)
+    vlist = val.tolist()
 
     def finish(a, targets):
         idx = np.argsort(-a, kind="stable")
@@ -136,33 +153,48 @@ def _excite_lists(ints, n_core: int, n_up: int, n_dn: int):
         return [(float(x), targets[i], float(r)) for x, i, r in zip(a, idx, rem)]
 
     vr, vs = np.meshgrid(val, val, indexing="ij")  # r outer, s inner
-    opp, same, sing = {}, {}, {}
-    for p in val.tolist():
-        for q in val.tolist():
-            m = (vr != p) & (vs != q)
-            a = np.abs(eri[p, vr[m], q, vs[m]])
-            opp[(p, q)] = finish(a, list(zip(vr[m].tolist(), vs[m].tolist())))
-    for p in val.tolist():
-        for q in val.tolist():
+
+    def make_opp(key):
+        p, q = key
+        m = (vr != p) & (vs != q)
+        return finish(np.abs(eri[p, vr[m], q, vs[m]]), list(zip(vr[m].tolist(), vs[m].tolist())))
+
+    def make_same(key):
+        p, q = key
+        m = (vr != p) & (vr != q) & (vs != p) & (vs != q) & (vr < vs)  # r < s as orbital indices
+        return finish(np.abs(eri[p, vr[m], q, vs[m]] - eri[p, vs[m], q, vr[m]]), list(zip(vr[m].tolist(), vs[m].tolist())))
+
+    def sing_bound(p, r):
+        qs = [q for q in range(n) if q not in (p, r)]
+        vsame = sorted((eri[p, r, qs, qs] - eri[p, qs, qs, r]).tolist())
+        vopp = sorted(eri[p, r, qs, qs].tolist())
+        lo = float(h[p, r]) + sum(vsame[:n_up - 1]) + sum(vopp[:n_dn])
+        hi = float(h[p, r]) + sum(vsame[len(vsame) - (n_up - 1):]) + sum(vopp[len(vsame) - n_dn:])
+        return abs(lo) if abs(lo) > abs(hi) else abs(hi)
+
+    sing_all = {p: [(sing_bound(p, r), r) for r in vlist if r != p] for p in vlist}
+
+    def make_sing(p):
+        v = sing_all[p]
+        return finish(np.array([x for x, _ in v]), [r for _, r in v])
+
+    # maxima over every list (the reference's max_opp_doub, max_same_doub, max_sing)
+    sub = eri[np.ix_(val, val, val, val)]  # [p, r, q, s] over valence positions
+    k = np.arange(val.size)
+    a_opp = np.abs(sub).copy()
+    a_opp[k, k, :, :] = 0.0  # r == p
+    a_opp[:, :, k, k] = 0.0  # s == q
+    max_opp = float(a_opp.max()) if val.size > 1 else 0.0
+    max_same = 0.0
+    for p in vlist:
+        for q in vlist:
             if p < q:
-                m = (vr != p) & (vr != q) & (vs != p) & (vs != q) & (vr < vs)  # r < s as orbital indices
-                a = np.abs(eri[p, vr[m], q, vs[m]] - eri[p, vs[m], q, vr[m]])
-                same[(p, q)] = finish(a, list(zip(vr[m].tolist(), vs[m].tolist())))
-    for p in val.tolist():
-        v, tg = [], []
-        for r in val.tolist():
-            if r == p:
-                continue
-            qs = [q for q in range(n) if q not in (p, r)]
-            vsame = sorted((eri[p, r, qs, qs] - eri[p, qs, qs, r]).tolist())
-            vopp = sorted(eri[p, r, qs, qs].tolist())
-            lo = float(h[p, r]) + sum(vsame[:n_up - 1]) + sum(vopp[:n_dn])
-            hi = float(h[p, r]) + sum(vsame[len(vsame) - (n_up - 1):]) + sum(vopp[len(vsame) - n_dn:])
-            v.append(abs(lo) if abs(lo) > abs(hi) else abs(hi))
-            tg.append(r)
-        sing[p] = finish(np.array(v), tg)
-    mx = lambda d: max((l[0][0] for l in d.values() if l), default=0.0)
-    return sum(1 << int(p) for p in val), opp, same, sing, mx(opp), mx(same), mx(sing)
+                lst = make_same((p, q))
+                if lst:
+                    max_same = max(max_same, lst[0][0])
+    max_sing = max((max(x for x, _ in v) for v in sing_all.values() if v), default=0.0)
+    return (sum(1 << int(p) for p in val), _Lists(make_opp), _Lists(make_same), _Lists(make_sing), max_opp,
+            max_same, max_sing)
 
 
 def _bits_asc(x: int):
