@@ -159,3 +159,27 @@ def test_external_product_capacity(gpu_ctx):
                                          C.byref(cnt), rem2.ctypes.data, MEM_HOST)
     assert rc != HCI_OK and cnt.value == w.shape[0] > 4
     assert np.array_equal(rem2, rem)
+
+
+def test_external_product_device_memory(gpu_ctx):
+    """Inputs and outputs in device memory (HCI_MEM_DEVICE) give the same
+    arrays as the host-memory call."""
+    import ctypes as C
+    from risq_amd.hci import DeviceBuffer, HCI_OK, MEM_DEVICE
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    dets = np.ascontiguousarray(dets[:400])
+    e0, c = _solve(gpu_ctx, ints, dets)
+    diag = gpu_ctx.ham_diag(dets)
+    gpu_ctx.excite_init(0, 4, 4, 1e-3)
+    w, v, d, rem = gpu_ctx.matmul_external(dets, c, diag, 1e-6)
+    k = w.shape[0]
+    dd, dc, dg = (DeviceBuffer.from_array(gpu_ctx, a) for a in (dets, np.ascontiguousarray(c), diag))
+    ow, ov, od, orem = (DeviceBuffer(gpu_ctx, nb) for nb in (k * 32, k * 8, k * 8, dets.shape[0] * 8))
+    cnt = C.c_int64()
+    rc = gpu_ctx.lib.hci_matmul_external(gpu_ctx.handle, dd.ptr, dc.ptr, dg.ptr, dets.shape[0], 1e-6, ow.ptr, ov.ptr,
+                                         od.ptr, k, C.byref(cnt), orem.ptr, MEM_DEVICE)
+    assert rc == HCI_OK and cnt.value == k
+    assert np.array_equal(ow.to_array(np.uint64, 4 * k).reshape(k, 4), w)
+    assert np.array_equal(ov.to_array(np.float64, k), v)
+    assert np.array_equal(od.to_array(np.float64, k), d)
+    assert np.array_equal(orem.to_array(np.float64, dets.shape[0]), rem)
