@@ -34,7 +34,7 @@ constexpr int kTileCols = 4096;  // build sub-block width
 constexpr int kWinCols = 256;
 constexpr int kWinPerSb = kTileCols / kWinCols;
 constexpr int kWinAlign = 8;
-constexpr int kTileWaves = 8;
+constexpr int kTileWaves = 4;
 constexpr int kTileThreads = 64 * kTileWaves;
 struct SubBlock {
     int32_t r0, nr, c0, nc;
