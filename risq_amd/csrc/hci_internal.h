@@ -59,7 +59,7 @@ __host__ __device__ inline int64_t win_slot(int64_t j, int64_t cnt) {
 // one workgroup: rows [r0, r0+nr) and windows [w_begin, w_end); windows of
 // more than kSmallWin entries come first and are taken one per wavefront,
 // those from w_split on are taken one per lane
-constexpr int kSmallWin = 32;
+constexpr int kSmallWin = 8;
 struct TileTask {
     int32_t r0, nr, w_begin, w_end, w_split;
 };
