@@ -880,17 +880,28 @@ __global__ void k_prev_fill(const int64_t* prp, const int32_t* pcol, const doubl
 }
 
 template <class S>
-__global__ void k_beta_r1(const S* sa_dn, int64_t n, int nbk, S* key, int32_t* kq) {
+__global__ void k_beta_r1(const S* sa_dn, int64_t n, int nbk, S* key, int32_t* kq, const int32_t* gof,
+                          const int32_t* idx, int gshift) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const S b = sa_dn[q];
+    // with gof: the position's group index above bit gshift, so that one
+    // stable sort of the whole array orders it by group, then by key
+    const S g = gof ? ((S)(uint32_t)gof[idx[q]]) << gshift : (S)0;
     S m = b;
     for (int t = 0; t < nbk; ++t) {
         const int o = ctz(m);
         m &= m - (S)1;
-        key[q * nbk + t] = b & ~(((S)1) << o);
+        key[q * nbk + t] = g | (b & ~(((S)1) << o));
         kq[q * nbk + t] = (int32_t)q;
     }
+}
+
+// clears the group bits k_beta_r1 set above bit gshift
+template <class S>
+__global__ void k_low_bits(S* key, int64_t n, int gshift) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) key[i] &= (((S)1) << gshift) - (S)1;
 }
 
 __global__ void k_scale_i32(const int32_t* a, int64_t n, int f, int32_t* out) {
@@ -1141,6 +1152,34 @@ __global__ void __launch_bounds__(64 * kGenWaves)
     }
 }
 
+static int group_bits(int32_t n_groups) {
+    int b = 0;
+    while (b < 31 && (1 << b) < n_groups) ++b;
+    return b;
+}
+
+// One-word keys whose group index fits above their n_orb string bits are
+// sorted within groups by one stable radix sort of (group, key) over the
+// whole array: the order a per-group sort gives, at a fraction of its cost
+// when there are many small groups.
+template <class S>
+bool grouped_sort_fits(int n_orb, int32_t n_groups) {
+    return sizeof(S) == 8 && n_orb < 64 && n_orb + group_bits(n_groups) <= 64;
+}
+
+template <class S>
+hipError_t sort_grouped(hci_ctx* c, Scratch& tmp, S* k_in, S* k_out, int32_t* q_in, int32_t* q_out, int64_t n,
+                 int32_t n_groups) {
+    if constexpr (sizeof(S) == 8) {
+        const hipError_t e = sort_pairs(tmp, k_in, k_out, q_in, q_out, (size_t)n, c->n_orb + group_bits(n_groups),
+                                        c->stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_low_bits<S>, grid_for(n, 256), 256, 0, c->stream, k_out, n, c->n_orb);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
 template <class S>
 int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t Ua, int32_t Ub,
                 int32_t n_pairs, const hci_build_opts& o, hci_csr* H) {
@@ -1163,14 +1202,20 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, kb.alloc(&q_out, (size_t)NK));
     HIP_TRY(c, kb.alloc(&seg, (size_t)Ua + 1));
     if (nbk > 0) {
-        hipLaunchKernelGGL(k_beta_r1<S>, grid_for(N, 256), 256, 0, s, B.sa_dn, N, nbk, k_in, q_in);
-        hipLaunchKernelGGL(k_scale_i32, grid_for(Ua + 1, 256), 256, 0, s, B.ga_off, (int64_t)Ua + 1, nbk, seg);
-        size_t need = 0;
-        HIP_TRY(c, rocprim::segmented_radix_sort_pairs(nullptr, need, k_in, k_out, q_in, q_out, (size_t)NK,
-                                                       (unsigned)Ua, seg, seg + 1, 0, c->n_orb, s));
-        HIP_TRY(c, tmp.ensure(need));
-        HIP_TRY(c, rocprim::segmented_radix_sort_pairs(tmp.p, need, k_in, k_out, q_in, q_out, (size_t)NK,
-                                                       (unsigned)Ua, seg, seg + 1, 0, c->n_orb, s));
+        const bool grouped = grouped_sort_fits<S>(c->n_orb, Ua);
+        hipLaunchKernelGGL(k_beta_r1<S>, grid_for(N, 256), 256, 0, s, B.sa_dn, N, nbk, k_in, q_in,
+                           grouped ? B.ga_of : nullptr, B.sa_idx, c->n_orb);
+        if (grouped) {
+            HIP_TRY(c, sort_grouped(c, tmp, k_in, k_out, q_in, q_out, NK, Ua));
+        } else {
+            hipLaunchKernelGGL(k_scale_i32, grid_for(Ua + 1, 256), 256, 0, s, B.ga_off, (int64_t)Ua + 1, nbk, seg);
+            size_t need = 0;
+            HIP_TRY(c, rocprim::segmented_radix_sort_pairs(nullptr, need, k_in, k_out, q_in, q_out, (size_t)NK,
+                                                           (unsigned)Ua, seg, seg + 1, 0, c->n_orb, s));
+            HIP_TRY(c, tmp.ensure(need));
+            HIP_TRY(c, rocprim::segmented_radix_sort_pairs(tmp.p, need, k_in, k_out, q_in, q_out, (size_t)NK,
+                                                           (unsigned)Ua, seg, seg + 1, 0, c->n_orb, s));
+        }
     }
     B.bkey = k_out;
     B.bkq = q_out;
@@ -1194,14 +1239,23 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         HIP_TRY(c, kb.alloc(&aq_out, (size_t)NA));
         HIP_TRY(c, kb.alloc(&aseg, (size_t)Ub + 1));
         if (nak > 0) {
-            hipLaunchKernelGGL(k_beta_r1<S>, grid_for(N, 256), 256, 0, s, B.sb_up, N, nak, a_in, aq_in);
-            hipLaunchKernelGGL(k_scale_i32, grid_for(Ub + 1, 256), 256, 0, s, B.gb_off, (int64_t)Ub + 1, nak, aseg);
-            size_t need = 0;
-            HIP_TRY(c, rocprim::segmented_radix_sort_pairs(nullptr, need, a_in, a_out, aq_in, aq_out, (size_t)NA,
-                                                           (unsigned)Ub, aseg, aseg + 1, 0, c->n_orb, s));
-            HIP_TRY(c, tmp.ensure(need));
-            HIP_TRY(c, rocprim::segmented_radix_sort_pairs(tmp.p, need, a_in, a_out, aq_in, aq_out, (size_t)NA,
-                                                           (unsigned)Ub, aseg, aseg + 1, 0, c->n_orb, s));
+            const bool grouped = grouped_sort_fits<S>(c->n_orb, Ub);
+            hipLaunchKernelGGL(k_beta_r1<S>, grid_for(N, 256), 256, 0, s, B.sb_up, N, nak, a_in, aq_in,
+                               grouped ? B.gb_of : nullptr, B.sb_idx, c->n_orb);
+            if (grouped) {
+                HIP_TRY(c, sort_grouped(c, tmp, a_in, a_out, aq_in, aq_out, NA, Ub));
+            } else {
+                hipLaunchKernelGGL(k_scale_i32, grid_for(Ub + 1, 256), 256, 0, s, B.gb_off, (int64_t)Ub + 1, nak,
+                                   aseg);
+                size_t need = 0;
+                HIP_TRY(c, rocprim::segmented_radix_sort_pairs(nullptr, need, a_in, a_out, aq_in, aq_out,
+                                                               (size_t)NA, (unsigned)Ub, aseg, aseg + 1, 0,
+                                                               c->n_orb, s));
+                HIP_TRY(c, tmp.ensure(need));
+                HIP_TRY(c, rocprim::segmented_radix_sort_pairs(tmp.p, need, a_in, a_out, aq_in, aq_out,
+                                                               (size_t)NA, (unsigned)Ub, aseg, aseg + 1, 0,
+                                                               c->n_orb, s));
+            }
         }
         B.akey = a_out;
         B.akq = aq_out;
