@@ -111,6 +111,7 @@ struct hci_csr {
     hci_ctx* ctx = nullptr;
     int64_t n = 0;
     int64_t nnz_u = 0;             // reference count (upper pairs)
+    int n_up = -1, n_dn = -1;      // electrons per spin, read from d_words on first use
     int64_t nnz_s = 0;             // stored entries (2*nnz_u for FULL)
     int64_t nnz_alloc = 0;         // length of the entry arrays (nnz_s plus TILED alignment gaps)
     int prec = HCI_PREC_F64;

@@ -173,7 +173,7 @@ __global__ void k_features(const uint64_t* words, const double* diag, const doub
     if (p >= n) return;
     const S up = load_string<S>(words + 4 * p), dn = load_string<S>(words + 4 * p + 2);
     const double d = diag[p] - lam;
-    float* o = X + p * F;
+    float o[F];
     o[0] = (float)d;
     o[1] = (float)log(fabs(d) + 1e-6);
     o[2] = (float)(r[p] * rs);
@@ -188,6 +188,10 @@ __global__ void k_features(const uint64_t* words, const double* diag, const doub
     }
     const S hfu = lowmask<S>(n_up), hfd = lowmask<S>(n_dn);
     o[23] = 0.125f * (float)(popc(up ^ hfu) + popc(dn ^ hfd));
+    // a row is 96 bytes: six 16-byte stores
+    float4* out = (float4*)(X + p * F);
+#pragma unroll
+    for (int j = 0; j < F / 4; ++j) out[j] = make_float4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
 }
 
 // bfloat16 weight image for k_mlp_infer, in its LDS layout: W1 and W2
@@ -569,10 +573,14 @@ int mlp_features(hci_ctx* c, const hci_csr* h, double lam, double rnorm, double 
     int rc = csr_rownorm(const_cast<hci_csr*>(h));
     if (rc) return rc;
     const int64_t n = h->n;
-    uint64_t w0[4];
-    HIP_TRY(c, hipMemcpy(w0, h->d_words, 32, hipMemcpyDeviceToHost));
-    const int n_up = __builtin_popcountll(w0[0]) + __builtin_popcountll(w0[1]);
-    const int n_dn = __builtin_popcountll(w0[2]) + __builtin_popcountll(w0[3]);
+    if (h->n_up < 0) {  // electron counts read once per matrix (a read here would stall the stream)
+        uint64_t w0[4];
+        HIP_TRY(c, hipMemcpy(w0, h->d_words, 32, hipMemcpyDeviceToHost));
+        hci_csr* hm = const_cast<hci_csr*>(h);
+        hm->n_up = __builtin_popcountll(w0[0]) + __builtin_popcountll(w0[1]);
+        hm->n_dn = __builtin_popcountll(w0[2]) + __builtin_popcountll(w0[3]);
+    }
+    const int n_up = h->n_up, n_dn = h->n_dn;
     int f0 = (n_up + n_dn) / 2 - 8;
     f0 = std::max(0, std::min(f0, c->n_orb - 16));
     const double sn = std::sqrt((double)n);
