@@ -209,9 +209,13 @@ def main() -> int:
     def step():
         return run(fast_prec, fast_mixed, args.keep, precond, args.max_sub)
 
+    failures = []  # steps whose Davidson did not converge
+
     for w in range(args.warmup):
         h, res, tb, tt = step()
         log(f"warmup {w}: build {tb:.3f}s total {tt:.3f}s E0={res.energy:.10f} iters={res.stats['iters']}")
+        if not res.converged:
+            failures.append(f"warmup {w} did not converge")
         h.free(recycle=True)
     ctx.sync()
     times, builds, results = [], [], []
@@ -222,6 +226,8 @@ def main() -> int:
         builds.append(tb)
         results.append(res)
         log(f"step {k}: build {tb:.3f}s total {tt:.3f}s E0={res.energy:.10f} {res.stats}")
+        if not res.converged:
+            failures.append(f"step {k} did not converge")
         if k + 1 < args.steps:
             h.free(recycle=True)
     ctx.sync()
@@ -300,12 +306,18 @@ def main() -> int:
             "n_sigma_full": alt.stats["n_sigma_full"], "mlp_applied": alt.stats["mlp_applied"],
             "mlp_fallbacks": alt.stats["mlp_fallbacks"]}
         log(f"other correction: {ta:.3f}s E0={alt.energy:.10f} {alt.stats}")
+        if not ref.converged:
+            failures.append("FP64 reference path did not converge")
+        elif out["abs_dE_vs_fp64_reference"] > 1e-6:
+            failures.append(f"|E0 - E0(FP64)| = {out['abs_dE_vs_fp64_reference']:.2e} Ha exceeds 1 uHa")
     if rank == 0 and not args.no_cpu:
         log("cpu baseline (oracle, 1 core, bounded sample)...")
         out["cpu_baseline"] = cpu_baseline(ints, dets)
+    if failures:
+        out["error"] = "; ".join(failures)
     print(json.dumps(out), flush=True)
     ctx.close()
-    return 0
+    return 1 if failures else 0
 
 
 if __name__ == "__main__":

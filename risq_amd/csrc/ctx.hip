@@ -60,7 +60,13 @@ hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes) {
         return hipSuccess;
     }
     recycled_flush(c);  // nothing fits: give the memory back before allocating
-    return hipMalloc(p, bytes);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory && c->work_dev) {
+        (void)hipGetLastError();
+        work_release(c);  // the Davidson work vectors are cheaper to allocate again
+        e = hipMalloc(p, bytes);
+    }
+    return e;
 }
 
 void recycled_flush(hci_ctx* c) {
@@ -90,6 +96,7 @@ void hci_destroy(hci_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     recycled_flush(c);
+    work_release(c);
     mlp_free_state(c);
     select_free(c);
     hipFree(c->d_two);

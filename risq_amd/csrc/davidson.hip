@@ -282,18 +282,25 @@ struct DavWork {
     double* partial = nullptr;
     double* red = nullptr;    // device reductions: 3 slots of kMaxSub + 1 values
     double* h_red = nullptr;  // pinned host copy of reductions (same layout)
-    ~DavWork() {
-        hipFree(V);
-        hipFree(W);
-        hipFree(x);
-        hipFree(wx);
-        hipFree(q);
-        hipFree(r);
-        hipFree(t);
-        hipFree(partial);
-        hipFree(red);
-        if (h_red) hipHostFree(h_red);
-        if (h_q) hipHostFree(h_q);
+    // carves every buffer out of the context's work block (kept between calls)
+    int carve(int nsub) {
+        const size_t vb = ((size_t)n * 8 + 255) / 256 * 256;
+        const size_t sizes[] = {vb * (size_t)nsub, vb * (size_t)nsub, vb, vb, vb, vb,
+                                (size_t)kMaxSub * kMaxKeep * 8, (size_t)kRedBlocks * (kMaxSub + 1) * 8,
+                                (size_t)3 * (kMaxSub + 1) * 8};
+        size_t total = 0;
+        for (size_t b : sizes) total += (b + 255) / 256 * 256;
+        const size_t host = (size_t)kMaxSub * kMaxKeep * 8 + 3 * (kMaxSub + 1) * 8 + 256;
+        if (int rc = work_reserve(c, total, host)) return rc;
+        double** dst[] = {&V, &W, &x, &wx, &r, &t, &q, &partial, &red};
+        char* p = (char*)c->work_dev;
+        for (int k = 0; k < 9; ++k) {
+            *dst[k] = (double*)p;
+            p += (sizes[k] + 255) / 256 * 256;
+        }
+        h_q = (double*)c->work_host;
+        h_red = h_q + (size_t)kMaxSub * kMaxKeep;
+        return HCI_OK;
     }
     unsigned red_grid() const {
         int64_t g = (n + kRedThreads - 1) / kRedThreads;
@@ -372,6 +379,38 @@ int record_rows(hci_ctx* c, const hci_csr* h, DavWork& w, Recorder& rec, double 
 
 }  // namespace
 
+int work_reserve(hci_ctx* c, size_t dev_bytes, size_t host_bytes) {
+    if (c->work_dev_bytes < dev_bytes) {
+        if (c->work_dev) {
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            hipFree(c->work_dev);
+        }
+        c->work_dev = nullptr;
+        c->work_dev_bytes = 0;
+        HIP_TRY(c, hipMalloc(&c->work_dev, dev_bytes));
+        c->work_dev_bytes = dev_bytes;
+    }
+    if (c->work_host_bytes < host_bytes) {
+        if (c->work_host) {
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            hipHostFree(c->work_host);
+        }
+        c->work_host = nullptr;
+        c->work_host_bytes = 0;
+        HIP_TRY(c, hipHostMalloc(&c->work_host, host_bytes));
+        c->work_host_bytes = host_bytes;
+    }
+    return HCI_OK;
+}
+
+void work_release(hci_ctx* c) {
+    if (c->work_dev || c->work_host) hipStreamSynchronize(c->stream);
+    if (c->work_dev) hipFree(c->work_dev);
+    if (c->work_host) hipHostFree(c->work_host);
+    c->work_dev = c->work_host = nullptr;
+    c->work_dev_bytes = c->work_host_bytes = 0;
+}
+
 extern "C" void hci_dav_default_opts(hci_dav_opts* o) {
     if (!o) return;
     o->tol = 1e-4;
@@ -417,17 +456,7 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
     w.n = N;
     w.max_sub = max_sub;
     const size_t vb = (size_t)N * 8;
-    HIP_TRY(c, hipMalloc(&w.V, vb * (size_t)std::max(max_sub, 2) + 16));
-    HIP_TRY(c, hipMalloc(&w.W, vb * (size_t)std::max(max_sub, 2) + 16));
-    HIP_TRY(c, hipMalloc(&w.x, vb + 16));
-    HIP_TRY(c, hipMalloc(&w.wx, vb + 16));
-    HIP_TRY(c, hipMalloc(&w.q, (size_t)kMaxSub * kMaxKeep * 8));
-    HIP_TRY(c, hipHostMalloc(&w.h_q, (size_t)kMaxSub * kMaxKeep * 8));
-    HIP_TRY(c, hipMalloc(&w.r, vb + 16));
-    HIP_TRY(c, hipMalloc(&w.t, vb + 16));
-    HIP_TRY(c, hipMalloc(&w.partial, (size_t)kRedBlocks * (kMaxSub + 1) * 8));
-    HIP_TRY(c, hipMalloc(&w.red, 3 * (kMaxSub + 1) * 8));
-    HIP_TRY(c, hipHostMalloc(&w.h_red, 3 * (kMaxSub + 1) * 8));
+    if (int rc = w.carve(std::max(max_sub, 2))) return rc;
 
     hci_dav_stats st{};
     st.refine_iter = -1;
@@ -499,11 +528,36 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
     bool mlp_prev = false;
     double err_prev = 1e300, best_err = 1e300;
     int since_best = 0;
-    MlpState* net = (opts->precond == HCI_PRECOND_MLP) ? c->mlp : nullptr;
+    int it_now = 0;
     std::vector<double> y_prev;  // previous Ritz coefficients in the current basis
+    // the next step of the precision ladder (FP16 -> FP32 -> full-precision
+    // values): the search space restarts from the current Ritz vector w.x.
+    // Only full-precision sigmas can declare convergence: every exit of a
+    // reduced-precision phase (its own convergence, a stall, or the 1e-9
+    // floors on the correction) comes here instead
+    auto next_precision = [&]() -> int {
+        if (sigma_mode == HCI_SIGMA_F16) {
+            sigma_mode = HCI_SIGMA_F32;
+            st.f16_iter = it_now;
+        } else {
+            sigma_mode = HCI_SIGMA_FULL;
+            st.refine_iter = it_now;
+        }
+        dim_sub = 1;
+        HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
+        DT(rebuild_projection());
+        err_prev = 1e300;
+        best_err = 1e300;
+        since_best = 0;
+        mlp_prev = false;
+        y_prev.assign(1, 1.0);
+        return HCI_OK;
+    };
+    MlpState* net = (opts->precond == HCI_PRECOND_MLP) ? c->mlp : nullptr;
     if (net) DT(csr_rownorm(const_cast<hci_csr*>(h)));
     int it;
     for (it = 0; it < opts->max_iters; ++it) {
+        it_now = it;
         std::vector<double> vals, vecs;
         jacobi_eigen(dim_sub, P, vals, vecs);
         lam = vals[0];
@@ -541,31 +595,9 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         // FP16-value sigmas first (mixed = 2): FP32 ones take over from the
         // Ritz vector once the FP16 iteration converges, reaches f16_tol, or
         // stalls at the FP16 rounding floor
-        if (sigma_mode == HCI_SIGMA_F16 && (conv || err < f16_tol || since_best >= 8)) {
-            sigma_mode = HCI_SIGMA_F32;
-            st.f16_iter = it;
-            dim_sub = 1;
-            HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
-            DT(rebuild_projection());
-            err_prev = 1e300;
-            best_err = 1e300;
-            since_best = 0;
-            mlp_prev = false;
-            y_prev.assign(1, 1.0);
-            continue;
-        }
-        if (sigma_mode == HCI_SIGMA_F32 && (conv || err < opts->refine_tol || since_best >= 25)) {
-            // switch to full-precision sigmas: restart from the Ritz vector
-            sigma_mode = HCI_SIGMA_FULL;
-            st.refine_iter = it;
-            dim_sub = 1;
-            HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
-            DT(rebuild_projection());
-            err_prev = 1e300;
-            best_err = 1e300;
-            since_best = 0;
-            mlp_prev = false;
-            y_prev.assign(1, 1.0);
+        if ((sigma_mode == HCI_SIGMA_F16 && (conv || err < f16_tol || since_best >= 8)) ||
+            (sigma_mode == HCI_SIGMA_F32 && (conv || err < opts->refine_tol || since_best >= 25))) {
+            DT(next_precision());
             continue;
         }
         err_prev = err;
@@ -602,12 +634,13 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             HIP_TRY(c, hipMemcpyAsync(w.h_red, w.red, 3 * kSlot * 8, hipMemcpyDeviceToHost, s));
             HIP_TRY(c, hipStreamSynchronize(s));
             if (mlp_now) tnorm = std::sqrt(w.h_red[dim_sub]);
-            if (tnorm < 1e-9) {
-                converged = true;
-                break;
-            }
             const double nrm = std::sqrt(w.h_red[2 * kSlot + dim_sub]);
-            if (nrm < 1e-9) {
+            // correction floors (davidson.rs:210-218, modified_gram_schmidt.rs:29)
+            if (tnorm < 1e-9 || nrm < 1e-9) {
+                if (sigma_mode != HCI_SIGMA_FULL) {
+                    DT(next_precision());
+                    continue;
+                }
                 converged = true;
                 break;
             }

@@ -89,6 +89,12 @@ struct hci_ctx {
         size_t bytes;
     };
     std::vector<Recycled> recycled;
+    // Davidson and sigma work vectors, kept between calls (davidson.hip):
+    // one device block and one pinned host block, grown on demand
+    void* work_dev = nullptr;
+    size_t work_dev_bytes = 0;
+    void* work_host = nullptr;
+    size_t work_host_bytes = 0;
 
     DevInts ints() const {
         DevInts t;
@@ -155,6 +161,12 @@ struct hci_csr {
 // least `bytes` or allocates; recycled_flush frees what is left
 hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes);
 void recycled_flush(hci_ctx* c);
+// the context's Davidson / sigma work block (davidson.hip): work_reserve
+// returns a device block of at least dev_bytes and a pinned host block of at
+// least host_bytes, reallocating only when they grow; work_release frees both
+// (builds call it before they allocate when an allocation fails)
+int work_reserve(hci_ctx* c, size_t dev_bytes, size_t host_bytes);
+void work_release(hci_ctx* c);
 
 // status helpers ------------------------------------------------------------
 int hci_fail(hci_ctx* ctx, int code, const std::string& msg);

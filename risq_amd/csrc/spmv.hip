@@ -566,20 +566,15 @@ extern "C" int hci_sigma(const hci_csr* h, int ncol, const double* x, double* y,
         return hci_fail(c, HCI_ERR_ARG, "hci_sigma: HCI_SIGMA_F16 needs HCI_PREC_F16 storage");
     HIP_TRY(c, hipSetDevice(c->device));
     const size_t bytes = (size_t)h->n * 8;
-    double *dx = nullptr, *dy = nullptr, *xi = nullptr, *yi = nullptr;
+    // four vectors of the context's work block (kept between calls)
+    const size_t vb = (bytes + 255) / 256 * 256;
+    if (int rc0 = work_reserve(c, 4 * vb, 256)) return rc0;
+    double* xi = (double*)c->work_dev;
+    double* yi = (double*)((char*)c->work_dev + vb);
+    double* dx = (double*)((char*)c->work_dev + 2 * vb);
+    double* dy = (double*)((char*)c->work_dev + 3 * vb);
     int rc = HCI_OK;
-    auto cleanup = [&]() {
-        hipStreamSynchronize(c->stream);
-        hipFree(dx);
-        hipFree(dy);
-        hipFree(xi);
-        hipFree(yi);
-    };
-    if (hipMalloc(&xi, bytes + 16) != hipSuccess || hipMalloc(&yi, bytes + 16) != hipSuccess ||
-        (mem == HCI_MEM_HOST && (hipMalloc(&dx, bytes + 16) != hipSuccess || hipMalloc(&dy, bytes + 16) != hipSuccess))) {
-        cleanup();
-        return hci_fail(c, HCI_ERR_OOM, "hci_sigma: vector allocation");
-    }
+    auto cleanup = [&]() { hipStreamSynchronize(c->stream); };
     for (int k = 0; k < ncol && rc == HCI_OK; ++k) {
         const double* xk = x + (size_t)k * h->n;
         double* yk = y + (size_t)k * h->n;

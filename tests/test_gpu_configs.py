@@ -1,0 +1,198 @@
+"""GPU parity on the benchmark workloads' own recipes.
+
+Each workload's determinant list is the one bench.py times, cut to a prefix
+the oracle builds in seconds:
+
+  molecular n_orb=40, 6a6b, K=80, seed 1: the first 20,000 determinants of the
+      2e6-determinant energy-truncated space (the list is ordered by
+      excitation energy, so its prefix is the smaller space);
+  Hubbard 6x6, 36 sites, 18a18b, t=1, U=4, disorder seed 2: the first 25,000
+      determinants the HCI selection adds (the bench's space after 10
+      iterations begins with the space after 5);
+  molecular n_orb=80, 10a10b, K=160, seed 4: the first 20,000 determinants of
+      its energy-truncated space (two-word strings).
+
+Against the oracle (C++ restatement of the reference, compiled
+-ffp-contract=off): sparsity pattern, column order and FP64 values
+bit-identical for FULL and TILED storage, FP32 hi+lo within 1e-13 relative,
+sigma within 1e-12 relative, and the fast path's mixed-precision Davidson
+(tiled storage, the bench's precision ladder, restart and correction) within
+1 uHa of the oracle's Davidson (reference control flow, tol 1e-6,
+etol 1e-10).  The full-size molecular workload (2e6 determinants) is checked
+by a size-independent property: the fast path's E0 equals the FP64
+FULL-storage E0 within 1 uHa, with refinement products taken.
+"""
+from functools import lru_cache
+
+import numpy as np
+import pytest
+
+from helpers import mol_space
+from risq_amd.hci import (PREC_F16, PREC_F32, PRECOND_DPR, PRECOND_MLP, SIGMA_F16, SIGMA_F32, STORE_FULL,
+                          STORE_TILED)
+
+pytestmark = pytest.mark.gpu
+
+MOL40 = (40, 80, 1, 6, 6)
+MOL80 = (80, 160, 4, 10, 10)
+N_PREFIX = 20000
+N_HUB = 25000
+
+
+@lru_cache(maxsize=None)
+def _hubbard_prefix_words():
+    """The first N_HUB determinants of the 6x6 lattice's HCI space (library
+    selection, eps_var 1e-3, five iterations)."""
+    from risq_amd import hci_select as hs
+    from risq_amd.fcidump import hubbard_2d
+    from risq_amd.hci import HciContext
+    ints = hubbard_2d(6, 6, 1.0, 4.0, 0.1, 2)
+    ctx = HciContext(0)
+    try:
+        solver = hs.GpuSolver(ctx, ints)
+        r = hs.variational(solver, ints, 18, 18, 1e-3, max_iters=5)
+        solver.close()
+    finally:
+        ctx.close()
+    words = hs._words(r.dets)
+    assert words.shape[0] >= N_HUB, words.shape
+    return ints, words[:N_HUB]
+
+
+def _workload(name):
+    if name == "mol40":
+        return mol_space(*MOL40, N_PREFIX)
+    if name == "hubbard6x6":
+        return _hubbard_prefix_words()
+    return mol_space(*MOL80, N_PREFIX)
+
+
+@lru_cache(maxsize=None)
+def _oracle(name):
+    """(diag, exported upper CSR, oracle sparse matrix, oracle E0)."""
+    from oracle import pyoracle
+    ints, dets = _workload(name)
+    H = pyoracle.OracleHam(ints)
+    diag = H.ham_diag(dets)
+    S = pyoracle.OracleSparse()
+    S.build(H, dets, diag)
+    rc, e0, _, _, _ = S.davidson(tol=1e-6, etol=1e-10, max_iters=3000)
+    assert rc == 0
+    return diag, S.export(), S, e0
+
+
+WORKLOADS = ["mol40", "hubbard6x6", "mol80"]
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_TILED])
+@pytest.mark.parametrize("name", WORKLOADS)
+def test_workload_build_and_sigma_match_oracle(gpu_ctx, name, storage):
+    ints, dets = _workload(name)
+    diag, (orp, ocol, oval, odg), S, _ = _oracle(name)
+    gpu_ctx.set_integrals(ints)
+    assert np.array_equal(gpu_ctx.ham_diag(dets).view(np.uint64), diag.view(np.uint64))
+    h = gpu_ctx.build_h(dets, diag, storage=storage)
+    rp, col, val, dg = h.download()
+    print(f"{name}: n={dets.shape[0]} nnz_u={h.nnz} (oracle {S.nnz}) storage={storage}")
+    assert h.nnz == S.nnz
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(val.view(np.uint64), oval.view(np.uint64))
+    assert np.array_equal(dg, odg)
+    x = np.random.default_rng(21).standard_normal(dets.shape[0])
+    ref = S.spmv(x)
+    y = h.matrix_vector_prod(x)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
+    h.free()
+    # FP32 hi + lo storage (the fast path's values)
+    h32 = gpu_ctx.build_h(dets, diag, value_prec=PREC_F32, storage=storage)
+    rp2, col2, val2, _ = h32.download()
+    assert np.array_equal(rp2, orp) and np.array_equal(col2, ocol)
+    assert np.max(np.abs(val2 - oval) / np.abs(oval)) < 1e-13
+    y2 = h32.matrix_vector_prod(x)
+    assert np.max(np.abs(y2 - ref)) <= 1e-12 * np.max(np.abs(ref))
+    h32.free()
+
+
+# the bench's fast path per workload: (value storage, mixed ladder, subspace,
+# vectors kept on restart, correction)
+FAST = {"mol40": (PREC_F16, 2, 16, 4, PRECOND_MLP),
+        "hubbard6x6": (PREC_F32, 1, 11, 2, PRECOND_DPR),
+        "mol80": (PREC_F32, 1, 11, 2, PRECOND_DPR)}
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("name", WORKLOADS)
+def test_workload_fast_path_energy(gpu_ctx, name):
+    from risq_amd import precond_train
+    ints, dets = _workload(name)
+    diag, _, _, e_oracle = _oracle(name)
+    gpu_ctx.set_integrals(ints)
+    precond_train.load(gpu_ctx)
+    prec, mixed, max_sub, keep, pc = FAST[name]
+    h = gpu_ctx.build_h(dets, diag, value_prec=prec, storage=STORE_TILED)
+    r = gpu_ctx.davidson(h, tol=1e-4, etol=1e-8, max_iters=400, mixed=mixed, max_sub=max_sub, restart_keep=keep,
+                         precond=pc)
+    print(f"{name}: E0={r.energy:.10f} oracle {e_oracle:.10f} diff {r.energy - e_oracle:.2e} {r.stats}")
+    assert r.converged
+    assert r.stats["n_sigma_full"] > 0 and r.stats["refine_iter"] >= 0
+    assert abs(r.energy - e_oracle) < 1e-6
+    h.free()
+
+
+@pytest.mark.timeout(300)
+def test_learned_correction_on_molecular_workload(gpu_ctx):
+    """The committed network on the n_orb=40 workload's prefix (held out: seed
+    1 is not a training system): FP16 -> FP32 -> refinement with the learned
+    correction converges to the oracle's E0, with network steps taken, and
+    takes at most 10% more iterations than the diagonal correction on the same
+    ladder (it is not trained on this system)."""
+    from risq_amd import precond_train
+    ints, dets = _workload("mol40")
+    diag, _, _, e_oracle = _oracle("mol40")
+    gpu_ctx.set_integrals(ints)
+    precond_train.load(gpu_ctx)
+    h = gpu_ctx.build_h(dets, diag, value_prec=PREC_F16, storage=STORE_TILED)
+    kw = dict(tol=1e-4, etol=1e-8, max_iters=400, mixed=2, max_sub=16, restart_keep=4)
+    a = gpu_ctx.davidson(h, precond=PRECOND_MLP, **kw)
+    b = gpu_ctx.davidson(h, precond=PRECOND_DPR, **kw)
+    print(f"mlp {a.stats}\ndpr {b.stats}")
+    assert a.converged and b.converged
+    assert a.stats["mlp_applied"] > 0
+    assert abs(a.energy - e_oracle) < 1e-6 and abs(b.energy - e_oracle) < 1e-6
+    assert a.stats["iters"] <= 1.1 * b.stats["iters"] + 2
+    h.free()
+
+
+@pytest.mark.timeout(600)
+def test_molecular_workload_full_size(gpu_ctx):
+    """2e6 determinants (the bench's default workload): the fast path (tiled
+    FP16 -> FP32 -> refinement, learned correction) against the FP64
+    FULL-storage Davidson, E0 within 1 uHa, refinement products taken; the
+    tiled FP16 product within FP16 rounding of the FP64 one."""
+    from risq_amd import precond_train
+    ints, dets = mol_space(*MOL40, 2_000_000)
+    gpu_ctx.set_integrals(ints)
+    precond_train.load(gpu_ctx)
+    diag = gpu_ctx.ham_diag(dets)
+    full = gpu_ctx.build_h(dets, diag, storage=STORE_FULL)
+    r64 = gpu_ctx.davidson(full, tol=1e-4, etol=1e-8, max_iters=600)
+    x = np.random.default_rng(22).standard_normal(dets.shape[0])
+    y64 = full.matrix_vector_prod(x)
+    nnz = full.nnz
+    full.free()
+    t = gpu_ctx.build_h(dets, diag, value_prec=PREC_F16, storage=STORE_TILED)
+    assert t.nnz == nnz
+    y32 = t.matrix_vector_prod(x, mode=SIGMA_F32)
+    y16 = t.matrix_vector_prod(x, mode=SIGMA_F16)
+    off = np.linalg.norm(y64 - diag * x)
+    assert np.linalg.norm(y32 - y64) / off < 1e-6
+    assert np.linalg.norm(y16 - y64) / off < 2e-3
+    r = gpu_ctx.davidson(t, tol=1e-4, etol=1e-8, max_iters=400, mixed=2, max_sub=16, restart_keep=4,
+                         precond=PRECOND_MLP)
+    print(f"full size: nnz_u={nnz} fast E0={r.energy:.10f} {r.stats}; FP64 E0={r64.energy:.10f} {r64.stats}")
+    t.free()
+    assert r64.converged and r.converged
+    assert r.stats["n_sigma_full"] > 0
+    assert abs(r.energy - r64.energy) < 1e-6

@@ -250,6 +250,28 @@ def test_fp32_storage_and_mixed_davidson(gpu_ctx, oracle_lib):
     assert abs(mix.energy - ref.energy) < 1e-6
 
 
+def test_mixed_floor_hands_over_to_refinement(gpu_ctx, oracle_lib):
+    """With etol = 0 no phase can meet the energy test, so a reduced-precision
+    phase ends at a stall or at the 1e-9 correction floors
+    (davidson.rs:210-218, modified_gram_schmidt.rs:29).  Those floors must hand
+    over to full-precision sigmas (only they may declare convergence): the
+    result is the eigenvalue of the stored hi + lo matrix, not of its FP32 or
+    FP16 roundings."""
+    from risq_amd.hci import PREC_F16
+    ints, dets = mol_space(16, 32, 0, 4, 4, 1500)
+    gpu_ctx.set_integrals(ints)
+    diag = oracle_lib.OracleHam(ints).ham_diag(dets)
+    h32 = gpu_ctx.build_h(dets, diag, value_prec=PREC_F32)
+    ref = np.linalg.eigvalsh(csr_to_dense(*h32.download()))[0]
+    r = gpu_ctx.davidson(h32, tol=1e-5, etol=0.0, max_iters=800, mixed=1)
+    assert r.converged and r.stats["n_sigma_full"] > 0 and r.stats["refine_iter"] >= 0, r.stats
+    assert abs(r.energy - ref) < 1e-10, (r.energy, ref)
+    h16 = gpu_ctx.build_h(dets, diag, value_prec=PREC_F16, storage=STORE_TILED)
+    r = gpu_ctx.davidson(h16, tol=1e-5, etol=0.0, max_iters=800, mixed=2, restart_keep=2)
+    assert r.converged and r.stats["n_sigma_full"] > 0 and r.stats["f16_iter"] >= 0, r.stats
+    assert abs(r.energy - ref) < 1e-10, (r.energy, ref)
+
+
 def test_full_ci_tiny(gpu_ctx, oracle_lib):
     """Full CI of a 6-orbital system: every determinant, dense eigh reference."""
     from risq_amd.fcidump import synthetic_molecular
