@@ -444,6 +444,112 @@ int or_build_fast_lookup(or_sparse* h, const or_ham* ham, const uint64_t* dets_w
     return 0;
 }
 
+// gen_sparse_ham_fast with opp_algo = same_algo = 1 (the reference's
+// defaults, src/config.rs:98-99), the build sparse_optimize uses while the
+// space holds at most 100 determinants (src/var/davidson.rs:55-61):
+// src/var/ham_gen.rs:60-292 (set-up), :374-433 (opposite-spin algo 1),
+// :621-637 (same-spin algo 1), with add_el / add_el_and_spin_flipped
+// (src/var/off_diag.rs:26-102) and remove_1e (src/var/utils.rs:10-12).
+// Needs a space closed under spin flip: the reference panics when the partner
+// of a determinant is missing (off_diag.rs:88-98); here that returns -2.
+int or_build_small(or_sparse* h, const or_ham* ham, const uint64_t* dets_w, int64_t n, const double* diag,
+                   int64_t n_stored) {
+    if (n < h->n || n_stored > h->n) return -1;
+    std::vector<Config> dets((size_t)n);
+    std::unordered_map<Config, int64_t, ConfigHash> inds;
+    for (int64_t i = 0; i < n; ++i) {
+        dets[i] = load_det(dets_w + 4 * i);
+        inds[dets[i]] = i;
+    }
+    // 1. up string -> [(index, dn)] for all and for new determinants (:73-90)
+    typedef std::vector<std::pair<int64_t, u128>> DetList;
+    std::unordered_map<u128, DetList, U128Hash> unique_up_dict, new_unique_up_dict;
+    std::vector<u128> up_order, new_up_order;  // first-seen order of the keys
+    for (int64_t i = 0; i < n; ++i) {
+        if (unique_up_dict.find(dets[i].up) == unique_up_dict.end()) up_order.push_back(dets[i].up);
+        unique_up_dict[dets[i].up].push_back({i, dets[i].dn});
+        if (i >= n_stored) {
+            if (new_unique_up_dict.find(dets[i].up) == new_unique_up_dict.end()) new_up_order.push_back(dets[i].up);
+            new_unique_up_dict[dets[i].up].push_back({i, dets[i].dn});
+        }
+    }
+    // 2. unique ups sorted by decreasing number of determinants (:97-118)
+    auto by_count = [&](std::vector<u128>& v, std::unordered_map<u128, DetList, U128Hash>& m) {
+        std::stable_sort(v.begin(), v.end(), [&](u128 a, u128 b) { return m[a].size() > m[b].size(); });
+    };
+    by_count(up_order, unique_up_dict);
+    by_count(new_up_order, new_unique_up_dict);
+    // 4. up_singles: every new unique up -> the unique ups one electron away (:130-157)
+    std::unordered_map<u128, std::vector<size_t>, U128Hash> up_single_excite_constructor;
+    for (size_t ind = 0; ind < up_order.size(); ++ind)
+        for_bits(up_order[ind], [&](int o) {
+            up_single_excite_constructor[up_order[ind] & ~((u128)1 << o)].push_back(ind);
+        });
+    std::unordered_map<u128, std::vector<u128>, U128Hash> up_singles;
+    for (u128 u : new_up_order)
+        for_bits(u, [&](int o) {
+            auto it = up_single_excite_constructor.find(u & ~((u128)1 << o));
+            if (it == up_single_excite_constructor.end()) return;
+            for (size_t ci : it->second) {
+                if (u == up_order[ci]) continue;  // the diagonal
+                up_singles[u].push_back(up_order[ci]);
+            }
+        });
+    // new_sparse_ham / expand_sparse_ham_rows (src/wf/mod.rs:1646-1683)
+    for (int64_t i = h->n; i < n; ++i) {
+        h->diag.push_back(diag[i]);
+        h->nnz.push_back(0);
+        h->off_diag.emplace_back();
+    }
+    h->n = n;
+    auto add = [&](int64_t i, int64_t j, double e) {  // add_off_diag_elem, off_diag.rs:45-53
+        if (i < j)
+            h->off_diag[i].push_back({j, e});
+        else
+            h->off_diag[j].push_back({i, e});
+    };
+    // opposite-spin algo 1 (:374-433): the dn strings of the new determinants
+    // of this up string, looked up as up strings (the space is closed under
+    // spin flip, so every dn string is also an up string)
+    for (u128 u : new_up_order) {
+        auto ups = up_singles.find(u);
+        if (ups == up_singles.end()) continue;
+        std::unordered_map<u128, std::vector<int64_t>, U128Hash> dn_candidates;
+        for (auto& dn : new_unique_up_dict[u]) {
+            auto dns = up_singles.find(dn.second);
+            if (dns == up_singles.end()) continue;
+            for (u128 dn2 : dns->second) dn_candidates[dn2].push_back(dn.first);
+        }
+        for (u128 up : ups->second)
+            for (auto& dn : unique_up_dict[up]) {
+                auto cand = dn_candidates.find(dn.second);
+                if (cand == dn_candidates.end()) continue;
+                const int64_t ind1 = inds[Config{up, dn.second}];
+                for (int64_t c : cand->second)
+                    if (c >= n_stored) add_el(h, ham, dets, ind1, c);
+            }
+    }
+    // same-spin algo 1 (:621-637): pairs of one up string, at least one new,
+    // each stored with its spin-flipped pair (add_el_and_spin_flipped)
+    for (u128 u : up_order) {
+        auto nd = new_unique_up_dict.find(u);
+        if (nd == new_unique_up_dict.end()) continue;
+        for (auto& a : unique_up_dict[u])
+            for (auto& b : nd->second) {
+                if (!(b.first > a.first)) continue;
+                const double e = ham->off_diag_no_excite(dets[a.first], dets[b.first]);
+                if (e == 0.0) continue;
+                add(a.first, b.first, e);
+                auto fa = inds.find(Config{dets[a.first].dn, dets[a.first].up});
+                auto fb = inds.find(Config{dets[b.first].dn, dets[b.first].up});
+                if (fa == inds.end() || fb == inds.end()) return -2;
+                if (fa->second != fb->second) add(fa->second, fb->second, e);
+            }
+    }
+    h->sort_remove_duplicates();
+    return 0;
+}
+
 int64_t or_sparse_n(const or_sparse* s) { return s->n; }
 int64_t or_sparse_nnz(const or_sparse* s) {
     int64_t t = 0;

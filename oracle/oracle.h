@@ -53,6 +53,12 @@ void or_off_diag(const or_ham*, const uint64_t* det1, const uint64_t* det2, int6
 or_sparse* or_sparse_create(int64_t n, const double* diag);
 int or_build_fast_lookup(or_sparse*, const or_ham*, const uint64_t* dets, int64_t n,
                          const double* diag, int64_t n_stored);
+/* or_build_small follows gen_sparse_ham_fast with opp_algo = same_algo = 1
+ * (src/var/ham_gen.rs:60-292,374-433,621-637), the reference's build while the
+ * space holds <= 100 determinants: singles are stored iff H != 0, and the space
+ * must be closed under spin flip (returns -2 otherwise; the reference panics). */
+int or_build_small(or_sparse*, const or_ham*, const uint64_t* dets, int64_t n, const double* diag,
+                   int64_t n_stored);
 int64_t or_sparse_n(const or_sparse*);
 int64_t or_sparse_nnz(const or_sparse*);
 /* row_ptr[n+1], col[nnz], val[nnz], diag[n] */

@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
             "or_off_diag": (None, [_P, _P, _P, C.c_int64, _P, _P]),
             "or_sparse_create": (_P, [C.c_int64, _P]),
             "or_build_fast_lookup": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.c_int64]),
+            "or_build_small": (C.c_int, [_P, _P, _P, C.c_int64, _P, C.c_int64]),
             "or_sparse_n": (C.c_int64, [_P]),
             "or_sparse_nnz": (C.c_int64, [_P]),
             "or_sparse_export": (None, [_P, _P, _P, _P, _P]),
@@ -128,6 +129,17 @@ class OracleSparse:
         dets = np.ascontiguousarray(dets, dtype=np.uint64)
         diag = np.ascontiguousarray(diag, dtype=np.float64)
         rc = lib().or_build_fast_lookup(self.s, ham.h, _p(dets), dets.shape[0], _p(diag), n_stored)
+        assert rc == 0
+        return self
+
+    def build_small(self, ham: OracleHam, dets, diag, n_stored: int = 0):
+        """gen_sparse_ham_fast, opp_algo = same_algo = 1 (the n <= 100 build).
+        Raises ValueError when the space is not closed under spin flip."""
+        dets = np.ascontiguousarray(dets, dtype=np.uint64)
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        rc = lib().or_build_small(self.s, ham.h, _p(dets), dets.shape[0], _p(diag), n_stored)
+        if rc == -2:
+            raise ValueError("space not closed under spin flip")
         assert rc == 0
         return self
 

@@ -155,15 +155,59 @@ def test_build_rejects_bad_input(gpu_ctx, oracle_lib):
         gpu_ctx.build_h(dets[:10], diag[:10], n_stored=3)
 
 
-def test_small_n_single_rule(gpu_ctx, oracle_lib):
-    """n <= 100 path of the reference stores singles iff H != 0 (off_diag.rs:33)."""
-    ints, dets = mol_space(16, 32, 0, 4, 4, 90)
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER, STORE_TILED])
+def test_small_n_build_matches_oracle(gpu_ctx, oracle_lib, storage):
+    """The reference's n <= 100 build (gen_sparse_ham_fast, algo 1/1,
+    src/var/ham_gen.rs:60-292) stores singles iff H != 0 (off_diag.rs:33).  On a
+    spin-flip-closed space whose integrals make some singles exactly 1e-13 and
+    others exact zeros, the library's HCI_SINGLES_NONZERO build equals the
+    oracle's restatement of that build bit for bit (and keeps the tiny singles
+    the |H| > 1e-12 rule drops)."""
+    from helpers import tiny_single_molecular
+    ints = tiny_single_molecular()
+    dets = full_space(8, 3, 3)
     gpu_ctx.set_integrals(ints)
-    diag = oracle_lib.OracleHam(ints).ham_diag(dets)
-    a = gpu_ctx.build_h(dets, diag, single_rule=SINGLES_NONZERO).download()
-    b = gpu_ctx.build_h(dets, diag).download()
-    # with generic random integrals no single lies in (0, 1e-12], so the patterns agree
-    assert np.array_equal(a[1], b[1])
+    H = oracle_lib.OracleHam(ints)
+    diag = H.ham_diag(dets)
+    o = oracle_lib.OracleSparse().build_small(H, dets, diag).export()
+    h = gpu_ctx.build_h(dets, diag, single_rule=SINGLES_NONZERO, storage=storage)
+    g = h.download()
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+    assert np.array_equal(g[2].view(np.uint64), o[2].view(np.uint64))
+    assert np.any(np.abs(g[2]) == 1e-13)
+    x = np.random.default_rng(12).standard_normal(dets.shape[0])
+    ref = oracle_lib.spmv_csr(*o, x)
+    y = h.matrix_vector_prod(x)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER])
+def test_small_n_then_fast_incremental(gpu_ctx, oracle_lib, storage):
+    """The HCI loop's rule switch: rows built while the space held <= 100
+    determinants keep their H != 0 singles; pairs added later, with a new index
+    >= n_stored, follow the |H| > 1e-12 rule (src/var/davidson.rs:55-61).  The
+    library's two incremental builds equal the oracle's n <= 100 build followed
+    by its fast-lookup build."""
+    from helpers import spin_flip_closed, tiny_single_molecular
+    ints = tiny_single_molecular()
+    dets = spin_flip_closed(full_space(8, 3, 3))
+    n1 = 99  # a spin-flip-closed prefix
+    assert np.array_equal(spin_flip_closed(dets[:n1]), dets[:n1])
+    gpu_ctx.set_integrals(ints)
+    H = oracle_lib.OracleHam(ints)
+    diag = H.ham_diag(dets)
+    S = oracle_lib.OracleSparse()
+    S.build_small(H, dets[:n1], diag[:n1])
+    S.build(H, dets, diag, n_stored=n1)
+    o = S.export()
+    h1 = gpu_ctx.build_h(dets[:n1], diag[:n1], single_rule=SINGLES_NONZERO, storage=storage)
+    h2 = gpu_ctx.build_h(dets, diag, prev=h1, n_stored=n1, storage=storage)
+    g = h2.download()
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+    assert np.array_equal(g[2].view(np.uint64), o[2].view(np.uint64))
+    rows = np.repeat(np.arange(dets.shape[0]), np.diff(g[0]))
+    tiny = np.abs(g[2]) <= 1e-12
+    assert tiny.any() and np.all(rows[tiny] < n1) and np.all(g[1][tiny] < n1)
 
 
 @pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER, STORE_TILED])
