@@ -202,6 +202,17 @@ int hci_csr_recycle(hci_csr* h);
  * (matrix_vector_prod / matrix_matrix_prod, src/var/sparse.rs:64-88). */
 int hci_sigma(const hci_csr* h, int ncol, const double* x, double* y, int sigma_mode, int mem);
 
+/* hci_sigma without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  x and y are device
+ * pointers (ncol column-major vectors of length n).  The two internal-order
+ * vectors live in a fixed device array of the library (32 MiB per GPU, held
+ * until the process ends), so n may be at most 2097152 (2^21); larger
+ * matrices are rejected with HCI_ERR_ARG (use hci_sigma).  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (several contexts or
+ * threads) race.  Returns after the product is complete. */
+int hci_sigma_noalloc(const hci_csr* h, int ncol, const double* x, double* y, int sigma_mode);
+
 /* Davidson, lowest root (Davidson::new via sparse_optimize).  init (nullable)
  * is a length-n start vector (the warm start of src/var/davidson.rs:71-83);
  * coeffs receives the eigenvector.  Returns HCI_OK, HCI_NOT_CONVERGED, or an

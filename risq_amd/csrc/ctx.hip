@@ -88,6 +88,11 @@ int hci_create(int device, hci_ctx** out) {
         delete c;
         return HCI_ERR_HIP;
     }
+    if (sigma_scratch_init(c) != HCI_OK) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return HCI_ERR_HIP;
+    }
     *out = c;
     return HCI_OK;
 }

@@ -95,6 +95,8 @@ struct hci_ctx {
     size_t work_dev_bytes = 0;
     void* work_host = nullptr;
     size_t work_host_bytes = 0;
+    // device address of the fixed scratch of hci_sigma_noalloc (spmv.hip)
+    double* scratch = nullptr;
 
     DevInts ints() const {
         DevInts t;
@@ -187,6 +189,7 @@ int spmv_prepare(hci_csr* h);      // builds d_chunk_row
 // y = H x on device vectors in internal order, on the context stream
 int spmv_device(const hci_csr* h, const double* x, double* y, int mode);
 int csr_rownorm(hci_csr* h);       // fills d_rownorm[p] = sum_q |H_pq| (off-diagonal)
+int sigma_scratch_init(hci_ctx* c);  // device address of hci_sigma_noalloc's scratch
 // caller order <-> internal order
 void permute_to_internal(const hci_csr* h, const double* x_caller, double* x_int);
 void permute_to_caller(const hci_csr* h, const double* x_int, double* x_caller);

@@ -596,3 +596,46 @@ extern "C" int hci_sigma(const hci_csr* h, int ncol, const double* x, double* y,
     cleanup();
     return rc;
 }
+
+// ---- allocation-free sigma -------------------------------------------------
+// hci_sigma_noalloc keeps its two internal-order vectors in this file-scope
+// array instead of the context's work block: callers that may not allocate
+// (graph capture, serving loops) get a product that calls no allocator.  In
+// host code `scratch` names a host-side stand-in: its device address comes
+// from hipGetSymbolAddress, once per context (each GPU holds its own copy).
+namespace {
+constexpr int64_t kNoAllocMaxDets = (int64_t)1 << 21;  // two FP64 vectors: 32 MiB
+__device__ __align__(256) double scratch[2 * kNoAllocMaxDets];
+}  // namespace
+
+int sigma_scratch_init(hci_ctx* c) {
+    void* p = nullptr;
+    HIP_TRY(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    c->scratch = (double*)p;
+    return HCI_OK;
+}
+
+extern "C" int hci_sigma_noalloc(const hci_csr* h, int ncol, const double* x, double* y, int sigma_mode) {
+    if (!h) return HCI_ERR_ARG;
+    hci_ctx* c = h->ctx;
+    if (ncol < 0 || (ncol > 0 && (!x || !y))) return hci_fail(c, HCI_ERR_ARG, "hci_sigma_noalloc: bad arguments");
+    if (sigma_mode != HCI_SIGMA_FULL && sigma_mode != HCI_SIGMA_F32 && sigma_mode != HCI_SIGMA_F16)
+        return hci_fail(c, HCI_ERR_ARG, "hci_sigma_noalloc: bad sigma_mode");
+    if (sigma_mode == HCI_SIGMA_F16 && !h->d_h16)
+        return hci_fail(c, HCI_ERR_ARG, "hci_sigma_noalloc: HCI_SIGMA_F16 needs HCI_PREC_F16 storage");
+    if (h->n > kNoAllocMaxDets)
+        return hci_fail(c, HCI_ERR_ARG, "hci_sigma_noalloc: the matrix has more than 2097152 determinants "
+                                        "(the size of the library's fixed scratch); use hci_sigma");
+    if (!c->scratch) return hci_fail(c, HCI_ERR_STATE, "hci_sigma_noalloc: scratch address not initialised");
+    HIP_TRY(c, hipSetDevice(c->device));
+    double* xi = c->scratch;
+    double* yi = c->scratch + kNoAllocMaxDets;
+    for (int k = 0; k < ncol; ++k) {
+        permute_to_internal(h, x + (size_t)k * h->n, xi);
+        if (int rc = spmv_device(h, xi, yi, sigma_mode)) return rc;
+        permute_to_caller(h, yi, y + (size_t)k * h->n);
+    }
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HCI_OK;
+}

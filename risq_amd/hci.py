@@ -105,6 +105,7 @@ SIGNATURES = {
     "hci_csr_free": (None, [_P]),
     "hci_csr_recycle": (C.c_int, [_P]),
     "hci_sigma": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.c_int]),
+    "hci_sigma_noalloc": (C.c_int, [_P, C.c_int, _P, _P, C.c_int]),
     "hci_dav_default_opts": (None, [C.POINTER(DavOpts)]),
     "hci_davidson": (C.c_int, [_P, _P, _P, C.POINTER(DavOpts), C.POINTER(C.c_double), _P,
                                C.POINTER(DavStats)]),
