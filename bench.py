@@ -94,22 +94,88 @@ def make_inputs(config: int, n_dets: int | None, ctx):
     raise ValueError(f"unknown workload {config}")
 
 
+def hci_space_run(ctx, ints, failures) -> dict:
+    """Workload 1's integrals over the space the HCI selection reaches
+    (eps_var 2e-3, six iterations; library selection with FP64 FULL builds and
+    reference-flow Davidson, untimed), then one untimed pass of the timed path
+    (tiled FP16 -> FP32 -> refinement, subspace 16 keeping 4, learned
+    correction) and of the plain FP64 path on it."""
+    from risq_amd import hci_select as hs
+    from risq_amd.hci import PREC_F16, PREC_F64, PRECOND_DPR, PRECOND_MLP, STORE_TILED
+    t = time.perf_counter()
+    solver = hs.GpuSolver(ctx, ints)
+    r = hs.variational(solver, ints, 6, 6, 2e-3, max_iters=6, log=log)
+    solver.close()
+    t_sel = time.perf_counter() - t
+    words = hs._words(r.dets)
+    diag = np.asarray(r.diag if hasattr(r, "diag") else ctx.ham_diag(words))
+    ctx.set_integrals(ints)
+    from risq_amd import precond_train
+    precond_train.load(ctx)
+    res = {"n_dets": int(words.shape[0]), "selection_s": round(t_sel, 2),
+           "space": "HCI selection at eps_var 2e-3, 6 iterations (untimed)"}
+    for name, prec, mixed, keep, pc, max_sub in (("fast", PREC_F16, 2, 4, PRECOND_MLP, 16),
+                                                 ("fp64_reference_path", PREC_F64, 0, 1, PRECOND_DPR, 11)):
+        ts = time.perf_counter()
+        h = ctx.build_h(words, diag, value_prec=prec, storage=STORE_TILED)
+        tb = time.perf_counter() - ts
+        d = ctx.davidson(h, tol=1e-4, etol=1e-8, max_iters=400, mixed=mixed, restart_keep=keep, precond=pc,
+                         max_sub=max_sub)
+        ctx.sync()
+        tt = time.perf_counter() - ts
+        res["nnz_u"] = h.nnz
+        h.free(recycle=True)
+        res[name] = {"s_to_E0": round(tt, 4), "build_s": round(tb, 4), "E0": d.energy, "iters": d.stats["iters"],
+                     "converged": bool(d.converged)}
+        log(f"hci space {name}: {tt:.3f}s build {tb:.3f}s E0={d.energy:.10f} {d.stats}")
+        if not d.converged:
+            failures.append(f"HCI-selected space, {name}: not converged")
+    res["speedup_vs_fp64_reference"] = round(res["fp64_reference_path"]["s_to_E0"] / res["fast"]["s_to_E0"], 3)
+    res["abs_dE"] = abs(res["fast"]["E0"] - res["fp64_reference_path"]["E0"])
+    if res["abs_dE"] > 1e-6:
+        failures.append("HCI-selected space: |E0 - E0(FP64)| exceeds 1 uHa")
+    return res
+
+
 def spmv_bytes(nnz_u: int, n: int, value_bytes: int) -> int:
     # upper-triangle (col + value) + row_ptr + diag + x + y
     return nnz_u * (value_bytes + 4) + 8 * (n + 1) + 8 * n + 8 * n + 8 * n
 
 
-def cpu_baseline(ints, dets) -> dict:
-    """The oracle (C++ restatement of the reference path, one core) on a bounded
-    prefix of the same determinant list: H build + SpMV + Davidson."""
+def host_cpu() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(ints, dets, n_full: int, nnz_full: int, fp64_iters: int | None) -> dict:
+    """The oracle (C++ restatement of the reference path, one core) on bounded
+    prefixes of the same determinant list.  The build is timed at four prefix
+    sizes and fitted as t = a N^b (its singles search is all-pairs, so b is
+    near 2), then extrapolated to the full list; the product is timed on the
+    largest prefix and charged per stored pair; Davidson is timed on that
+    prefix.  The full-size time to E0 is extrapolated as build(N) +
+    iterations x product(N), with the iteration count of the plain FP64 GPU
+    path (the same control flow)."""
     from oracle import pyoracle
-    n_s = 30000  # about 10 s of one core: the all-pairs singles search grows as N^2
-    sub = dets[:n_s]
     H = pyoracle.OracleHam(ints)
-    diag = H.ham_diag(sub)
-    t = time.perf_counter()
-    S = pyoracle.OracleSparse().build(H, sub, diag)
-    t_build = time.perf_counter() - t
+    sizes = [n for n in (5000, 10000, 20000, 30000) if n <= dets.shape[0]]
+    builds = []
+    S = None
+    for n_s in sizes:
+        sub = dets[:n_s]
+        diag = H.ham_diag(sub)
+        t = time.perf_counter()
+        S = pyoracle.OracleSparse().build(H, sub, diag)
+        builds.append((n_s, time.perf_counter() - t, S.nnz))
+        log(f"cpu oracle build: N={n_s} {builds[-1][1]:.2f}s nnz_u={S.nnz}")
+    n_s, t_build, nnz = builds[-1]
     x = np.random.default_rng(0).standard_normal(n_s)
     reps = 0
     t = time.perf_counter()
@@ -120,16 +186,32 @@ def cpu_baseline(ints, dets) -> dict:
     t = time.perf_counter()
     rc, e0, _, iters, nsig = S.davidson(tol=1e-4, etol=1e-8, max_iters=300)
     t_dav = time.perf_counter() - t
-    nnz = S.nnz
-    return {
+    out = {
         "value": round(t_build + t_dav, 4), "unit": "s to E0 (sample)", "cores": 1, "kind": "port",
         "sample": f"first {n_s} determinants of the same list (nnz_u={nnz}); "
                   f"oracle build {t_build:.2f}s + Davidson {t_dav:.2f}s ({iters} iters, {nsig} sigmas)",
+        "host": host_cpu(),
         "h_build_nnz_per_s": nnz / t_build,
         "spmv_gbps": spmv_bytes(nnz, n_s, 8) / t_spmv / 1e9,
         "spmv_nnz_per_s": nnz / t_spmv,
         "davidson_converged": rc == 0,
+        "build_sweep": [{"n": a, "s": round(b, 4), "nnz_u": c} for a, b, c in builds],
     }
+    if len(builds) >= 2:
+        lx = np.log([b[0] for b in builds])
+        ly = np.log([b[1] for b in builds])
+        slope, icpt = np.polyfit(lx, ly, 1)
+        t_build_full = float(np.exp(icpt) * n_full ** slope)
+        t_prod_full = nnz_full / out["spmv_nnz_per_s"]
+        out["extrapolated"] = {
+            "label": "extrapolated, not measured",
+            "n": n_full, "build_fit_exponent": round(float(slope), 3),
+            "build_s": round(t_build_full, 1),
+            "product_s": round(t_prod_full, 3),
+            "davidson_iters_assumed": fp64_iters,
+            "s_to_E0": None if fp64_iters is None else round(t_build_full + fp64_iters * t_prod_full, 1),
+        }
+    return out
 
 
 def main() -> int:
@@ -142,6 +224,8 @@ def main() -> int:
     ap.add_argument("--n-dets", type=int, default=None,
                     help="determinants of the energy-truncated space (workloads 1 and 4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-hci-space", action="store_true",
+                    help="workload 1: skip the run on the HCI-selected space of the same integrals")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the untimed comparison runs (FP64 reference path, other preconditioner)")
     ap.add_argument("--values", choices=["f16", "f32"], default=None,
@@ -282,9 +366,11 @@ def main() -> int:
         out["roofline"]["traffic"] = MEASURED_F16_PRODUCT_BYTES
         out["roofline"]["traffic_source"] = ("profiles/k_spmv_win_config1_2e6dets.md: bytes per FP16-value "
                                              "product (both triangles at 4 B per entry + FP32 x slices + y)")
+    fp64_iters = None
     if not args.no_compare:
         h, ref, tb, tr = run(PREC_F64, False, 1, PRECOND_DPR)
         st = ref.stats
+        fp64_iters = st["iters"]
         r_ms = st["sigma_ms"] / max(1, st["n_sigma_full"])
         r_ach = spmv_bytes(h.nnz, n, 8) / (r_ms * 1e-3) / 1e9
         h.free(recycle=True)
@@ -296,23 +382,41 @@ def main() -> int:
         out["speedup_vs_fp64_reference"] = round(tr / out["value"], 3)
         out["abs_dE_vs_fp64_reference"] = abs(ref.energy - out["E0"])
         log(f"fp64 reference path: {tr:.3f}s E0={ref.energy:.10f} iters={st['iters']} sigma {r_ms:.2f} ms")
-        other = PRECOND_DPR if precond == PRECOND_MLP else PRECOND_MLP
-        h, alt, tb, ta = run(fast_prec, fast_mixed, args.keep, other, args.max_sub)
-        h.free()
-        out["other_correction"] = {
-            "correction": "dpr" if other == PRECOND_DPR else "mlp", "s_to_E0": round(ta, 4),
-            "E0": alt.energy, "iters": alt.stats["iters"], "n_sigma_f16": alt.stats["n_sigma_f16"],
-            "n_sigma_f32": alt.stats["n_sigma_f32"],
-            "n_sigma_full": alt.stats["n_sigma_full"], "mlp_applied": alt.stats["mlp_applied"],
-            "mlp_fallbacks": alt.stats["mlp_fallbacks"]}
-        log(f"other correction: {ta:.3f}s E0={alt.energy:.10f} {alt.stats}")
         if not ref.converged:
             failures.append("FP64 reference path did not converge")
         elif out["abs_dE_vs_fp64_reference"] > 1e-6:
             failures.append(f"|E0 - E0(FP64)| = {out['abs_dE_vs_fp64_reference']:.2e} Ha exceeds 1 uHa")
+
+        def compare(name, prec, mixed, keep, pc, max_sub, what):
+            h, r, tb, tt = run(prec, mixed, keep, pc, max_sub)
+            h.free(recycle=True)
+            st = r.stats
+            out[name] = {"s_to_E0": round(tt, 4), "build_s": round(tb, 4), "E0": r.energy,
+                         "abs_dE_vs_fp64_reference": abs(r.energy - ref.energy), "iters": st["iters"],
+                         "n_sigma_f16": st["n_sigma_f16"], "n_sigma_f32": st["n_sigma_f32"],
+                         "n_sigma_full": st["n_sigma_full"], "mlp_applied": st["mlp_applied"],
+                         "mlp_fallbacks": st["mlp_fallbacks"], "converged": bool(st["converged"]), "what": what}
+            log(f"{name}: {tt:.3f}s E0={r.energy:.10f} {st}")
+            if not st["converged"]:
+                failures.append(f"{name} did not converge")
+            elif abs(r.energy - ref.energy) > 1e-6:
+                failures.append(f"{name}: |E0 - E0(FP64)| exceeds 1 uHa")
+
+        other = PRECOND_DPR if precond == PRECOND_MLP else PRECOND_MLP
+        oname = "dpr" if other == PRECOND_DPR else "mlp"
+        compare("other_correction", fast_prec, fast_mixed, args.keep, other, args.max_sub,
+                f"the timed path with the {oname} correction")
+        compare("fp64_same_restart", PREC_F64, 0, args.keep, precond, args.max_sub,
+                f"FP64 values with the timed path's restart (subspace {args.max_sub}, keep {args.keep}) and "
+                f"correction: separates the restart's gain from the precision ladder's")
+        if args.config == 1:
+            compare("fp32_learned_path", PREC_F32, 1, args.keep, PRECOND_MLP, args.max_sub,
+                    "FP32 values (FP32 -> FP64 refinement, no FP16 phase) with the learned correction")
+    if args.config == 1 and n == 2_000_000 and not args.no_hci_space:
+        out["hci_selected_space"] = hci_space_run(ctx, ints, failures)
     if rank == 0 and not args.no_cpu:
         log("cpu baseline (oracle, 1 core, bounded sample)...")
-        out["cpu_baseline"] = cpu_baseline(ints, dets)
+        out["cpu_baseline"] = cpu_baseline(ints, dets, n, nnz, fp64_iters)
     if failures:
         out["error"] = "; ".join(failures)
     print(json.dumps(out), flush=True)
