@@ -72,8 +72,13 @@ extern "C" {
                              of one alpha (or beta) string group holds its
                              entries per column group with 16-bit local row and
                              column indices; sigma stages x slices in LDS and
-                             sums rows in LDS (order-dependent rounding).  No
-                             incremental builds and no hci_csr_row. */
+                             sums rows in LDS with atomics in no fixed order, so
+                             products (and Davidson iteration counts) are not
+                             bitwise reproducible from run to run (they agree to
+                             rounding).  An incremental build rebuilds the whole
+                             tiled matrix; pairs among the previous
+                             determinants keep the single rule they were stored
+                             under.  No hci_csr_row. */
 
 /* which singles are stored (doubles are always stored iff H != 0,
  * src/var/off_diag.rs:33) */
@@ -173,8 +178,10 @@ int hci_elements(hci_ctx* ctx, const uint64_t* det1, const uint64_t* det2, int64
  * (new_sparse_ham / expand_sparse_ham_rows, src/wf/mod.rs:1646-1683).
  * Incremental form: with prev != NULL (built over dets[0..n_prev)) and
  * n_stored == n_prev, prev's rows are kept and only pairs with at least one
- * index >= n_stored are added (fast_ham.rs:397,429,444-447,480).  With
- * prev == NULL, n_stored must be 0. */
+ * index >= n_stored are added (fast_ham.rs:397,429,444-447,480), under
+ * opts->single_rule; the previous pairs keep the rule of the build that added
+ * them (at most 8 alternations of the rule along a chain of builds).  prev
+ * must have the same storage kind.  With prev == NULL, n_stored must be 0. */
 int hci_build_h(hci_ctx* ctx, const uint64_t* dets, const double* diag, int64_t n,
                 const hci_csr* prev, int64_t n_stored, const hci_build_opts* opts,
                 hci_csr** out, hci_build_stats* stats);
@@ -328,6 +335,39 @@ int hci_pt2_dtm(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, int64_
 int hci_matmul_external(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
                         double eps, uint64_t* out, double* out_coeffs, double* out_diag, int64_t cap, int64_t* count,
                         double* sum_remaining, int mem);
+
+/* The deterministic stage of the reference's default PT2,
+ * new_semistoch_enpt2_dtm_diag_singles (src/semistoch/mod.rs:262-318; chosen by
+ * perturbative when use_new_semistoch is set, src/pt/mod.rs:52-56, the default
+ * of src/config.rs:253).  For each determinant i and each of its valence
+ * electron pairs and electrons (valence_elecs_and_epairs order), the list of
+ * hci_excite_init is read skipping targets already occupied or inside the
+ * space (excites_from_det_and_orbs, src/excite/iterator.rs:113-130); the first
+ * remaining entry with |stored magnitude| * |c_i| < eps ends it and yields a
+ * sample record (DetOrbSample, src/stoch/mod.rs:104-124, the input of
+ * generate_screened_sampler, :155).  Every entry before it gives
+ * h = H_ai c_i: rel_probs[i] = max |h| (the off-diagonal sampler's weights),
+ * and h joins h_psi unless it is a single with |h| < eps.  *e2 receives
+ * pt(h_psi, e0) = sum_a h_psi_a^2 / (e0 - H_aa) with H_aa the diagonal the
+ * first discoverer of a gives it (Det::new_diag, src/wf/det.rs:249-420; needs
+ * the space's diagonal `diag`), *n_ext the size of h_psi.  rel_probs (n
+ * values) may be NULL.  samples may be NULL; otherwise at most cap records are
+ * written in the reference's push order, and a larger need fails the call
+ * with *n_samples set.  Needs hci_excite_init on the same integrals. */
+typedef struct hci_det_orb_sample {
+    int64_t det;           /* index of the source determinant i */
+    int32_t kind;          /* 0 opposite-spin pair (p up, q down), 1 same-spin pair p < q, 2 single p */
+    int32_t is_alpha;      /* 1 alpha, 0 beta, -1 none (opposite-spin pair) */
+    int32_t p, q;          /* initial orbitals (q = -1 for a single) */
+    double sum_abs_h;      /* sum_remaining_abs_h of the entry that ended the list */
+    double sum_h_squared;  /* its sum_remaining_h_squared */
+    double sum_abs_hc;     /* |c_i| * sum_abs_h */
+    double sum_hc_squared; /* c_i^2 * sum_h_squared */
+} hci_det_orb_sample;
+
+int hci_semistoch_dtm(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
+                      double e0, double eps, double* e2, int64_t* n_ext, double* rel_probs,
+                      hci_det_orb_sample* samples, int64_t cap, int64_t* n_samples, int mem);
 
 /* Device-memory and timing plumbing (no reference counterpart: for callers
  * that keep inputs resident in HBM, and for timing on the context's stream).

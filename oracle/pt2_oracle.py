@@ -137,7 +137,8 @@ def _excite_lists(ints, n_core: int, n_up: int, n_dn: int):
     (compute_sum_remaining, :288-309: 0 for the last entry, then a running sum
     of the following entries' squares from the end, added one at a time).
     Lists are built when first read; the three maxima over all lists are
-    computed up front."""
+    computed up front.  Entries are (magnitude, target, sum_remaining_h_squared,
+    sum_remaining_abs_h)."""
     n = ints.n_orb
     h, eri = ints.dense()
     order = sorted(range(n), key=lambda p: ints.h1(p, p))
@@ -148,9 +149,11 @@ def _excite_lists(ints, n_core: int, n_up: int, n_dn: int):
         idx = np.argsort(-a, kind="stable")
         a = a[idx]
         rem = np.zeros(a.size)
+        rabs = np.zeros(a.size)
         if a.size > 1:
             rem[:-1] = np.cumsum((a[1:] * a[1:])[::-1])[::-1]  # sequential, from the end
-        return [(float(x), targets[i], float(r)) for x, i, r in zip(a, idx, rem)]
+            rabs[:-1] = np.cumsum(a[1:][::-1])[::-1]  # sum_remaining_abs_h
+        return [(float(x), targets[i], float(r), float(b)) for x, i, r, b in zip(a, idx, rem, rabs)]
 
     vr, vs = np.meshgrid(val, val, indexing="ij")  # r outer, s inner
 
@@ -228,7 +231,7 @@ def matmul_external_dtm(ints, dets: np.ndarray, coeffs: np.ndarray, eps: float, 
         if max_opp >= local:
             for i in _bits_asc(valence & up):
                 for j in _bits_asc(valence & dn):
-                    for a, (r, s), rr in opp[(i, j)]:
+                    for a, (r, s), rr, _ in opp[(i, j)]:
                         if a < local:
                             rem += rr
                             break
@@ -242,7 +245,7 @@ def matmul_external_dtm(ints, dets: np.ndarray, coeffs: np.ndarray, eps: float, 
                 for x in range(len(b)):
                     for y in range(x + 1, len(b)):
                         i, j = b[x], b[y]
-                        for a, (r, s), rr in same[(i, j)]:
+                        for a, (r, s), rr, _ in same[(i, j)]:
                             if a < local:
                                 rem += rr
                                 break
@@ -253,7 +256,7 @@ def matmul_external_dtm(ints, dets: np.ndarray, coeffs: np.ndarray, eps: float, 
         if max_sing >= local:
             for cfg, alpha in ((up, True), (dn, False)):
                 for i in _bits_asc(valence & cfg):
-                    for a, r, rr in sing[i]:
+                    for a, r, rr, _ in sing[i]:
                         if a < local:
                             rem += rr
                             break
@@ -294,3 +297,97 @@ def epstein_nesbet_dense(h_full: np.ndarray, in_space: np.ndarray, coeffs: np.nd
     if allowed is not None:
         out &= allowed
     return float(np.sum(v[out] ** 2 / (e0 - np.diag(h_full)[out])))
+
+
+def semistoch_dtm(ints, dets: np.ndarray, coeffs: np.ndarray, e0: float, eps: float, n_core: int, n_up: int,
+                  n_dn: int):
+    """The deterministic stage of new_semistoch_enpt2_dtm_diag_singles
+    (src/semistoch/mod.rs:221-334), the reference's default PT2
+    (src/config.rs:253, src/pt/mod.rs:52-56).  For each determinant i (in
+    order) and each of its valence electron pairs and electrons in the order
+    of valence_elecs_and_epairs (opposite-spin pairs, same-spin pairs up then
+    down, electrons up then down; src/utils/bits.rs:76-123), the list is read
+    through excites_from_det_and_orbs (src/excite/iterator.rs:113-130): targets
+    already occupied and targets inside the space are skipped before anything
+    else.  The first remaining entry with |stored magnitude| * |c_i| < eps ends
+    the list and records (i, orbitals, spin, sum_remaining_abs_h,
+    sum_remaining_h_squared, |c_i| sum_remaining_abs_h,
+    c_i^2 sum_remaining_h_squared) -- the DetOrbSample the screened sampler is
+    built from (src/stoch/mod.rs:155).  Every entry before it gives
+    h = H_ai c_i: rel_probs[i] = max |h|, and h joins h_psi unless the entry
+    is a single with |h| < eps (the first discoverer of a target creates it).
+    Returns (h_psi words in first-discovery order, their sums, first
+    discoverers (i, excitation), rel_probs, records, e2 with H_aa from
+    ham_diag).  Elements from the C++ oracle."""
+    valence, opp, same, sing, max_opp, max_same, max_sing = _excite_lists(ints, n_core, n_up, n_dn)
+    H = pyoracle.OracleHam(ints)
+    space = {(int(w[0]) | int(w[1]) << 64, int(w[2]) | int(w[3]) << 64) for w in dets}
+    n = len(dets)
+    rel = np.zeros(n)
+    records = []
+    psi: dict[tuple[int, int], list] = {}
+
+    def elem(src_w, tu, td):
+        v, _ = H.off_diag(np.asarray(src_w, dtype=np.uint64)[None, :], _words([tu], [td]))
+        return float(v[0])
+
+    for k, w in enumerate(dets):
+        up, dn, c = int(w[0]) | int(w[1]) << 64, int(w[2]) | int(w[3]) << 64, float(coeffs[k])
+        ac = abs(c)
+        groups = []  # (kind, is_alpha, p, q, list, target -> (up', dn'))
+        for i in _bits_asc(valence & up):
+            for j in _bits_asc(valence & dn):
+                groups.append((0, -1, i, j, opp[(i, j)],
+                               lambda t, i=i, j=j: (up ^ (1 << i) | (1 << t[0]), dn ^ (1 << j) | (1 << t[1]))))
+        for cfg, alpha in ((up, 1), (dn, 0)):
+            b = list(_bits_asc(valence & cfg))
+            for x in range(len(b)):
+                for y in range(x + 1, len(b)):
+                    i, j = b[x], b[y]
+
+                    def mk(t, i=i, j=j, cfg=cfg, alpha=alpha):
+                        nw = cfg ^ (1 << i) ^ (1 << j) | (1 << t[0]) | (1 << t[1])
+                        return (nw, dn) if alpha else (up, nw)
+                    groups.append((1, alpha, i, j, same[(i, j)], mk))
+        for cfg, alpha in ((up, 1), (dn, 0)):
+            for i in _bits_asc(valence & cfg):
+                def mk1(r, i=i, cfg=cfg, alpha=alpha):
+                    nw = cfg ^ (1 << i) | (1 << r)
+                    return (nw, dn) if alpha else (up, nw)
+                groups.append((2, alpha, i, -1, sing[i], mk1))
+        for kind, alpha, i, j, lst, make in groups:
+            for a, tgt, rsq, rab in lst:
+                if kind == 0:
+                    if (up >> tgt[0]) & 1 or (dn >> tgt[1]) & 1:
+                        continue
+                elif kind == 1:
+                    cfg = up if alpha else dn
+                    if (cfg >> tgt[0]) & 1 or (cfg >> tgt[1]) & 1:
+                        continue
+                else:
+                    cfg = up if alpha else dn
+                    if (cfg >> tgt) & 1:
+                        continue
+                t = make(tgt)
+                if t in space:
+                    continue
+                if a * ac < eps:
+                    records.append((k, kind, alpha, i, j, rab, rsq, ac * rab, c * c * rsq))
+                    break
+                h = elem(w, t[0], t[1]) * c
+                if abs(h) > rel[k]:
+                    rel[k] = abs(h)
+                if not (kind == 2 and abs(h) < eps):
+                    if t in psi:
+                        psi[t][0] += h
+                    else:
+                        exc = (kind, alpha, i, j, tgt)
+                        psi[t] = [h, k, exc]
+    keys = list(psi)
+    words = _words([t[0] for t in keys], [t[1] for t in keys]) if keys else np.zeros((0, 4), dtype=np.uint64)
+    nums = np.array([psi[t][0] for t in keys])
+    e2 = 0.0
+    if keys:
+        for num, d in zip(nums, H.ham_diag(words)):
+            e2 += num * num / (e0 - d)
+    return words, nums, [(psi[t][1], psi[t][2]) for t in keys], rel, records, e2

@@ -80,6 +80,13 @@ __device__ __forceinline__ bool keep_single(double v, int rule) {
 // download skips (no stored element is ever exactly zero)
 __device__ __forceinline__ double single_or_zero(double v, int rule) { return keep_single(v, rule) ? v : 0.0; }
 
+// the rule of a pair whose larger caller index is m
+__device__ __forceinline__ int rule_of(const SingleRules& R, int32_t m) {
+    for (int k = 0; k + 1 < R.n; ++k)
+        if (m < R.end[k]) return R.rule[k];
+    return R.rule[R.n - 1];
+}
+
 template <class S>
 __device__ __forceinline__ void swap_if(bool c, S& a, S& b) {
     const S t = a;
@@ -290,7 +297,7 @@ __global__ void __launch_bounds__(kBlock)
 template <class S, bool FILL>
 __global__ void __launch_bounds__(kBlock)
     k_tile_connect(const SubBlock* __restrict__ sbs, int64_t n_sb, BuildView<S> B, const S* ub, DevInts T,
-                   int single_rule, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out) {
+                   SingleRules rules, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out) {
     __shared__ int32_t s_cur[kWaves][kWinPerSb];
     __shared__ int64_t s_off[kWaves][kWinPerSb];
     const int lane = threadIdx.x & 63;
@@ -405,7 +412,7 @@ __global__ void __launch_bounds__(kBlock)
                                 S u1 = other_row, u2 = ua_col;
                                 swap_if(j < i, u1, u2);
                                 swap_if(j < i, da, db);
-                                v = same ? single_or_zero(elem_single<S>(T, u1, u2, da), single_rule)
+                                v = same ? single_or_zero(elem_single<S>(T, u1, u2, da), rule_of(rules, max(i, j)))
                                          : elem_opp_double<S>(T, u1, u2, da, db);
                             }
                         } else {
@@ -413,7 +420,7 @@ __global__ void __launch_bounds__(kBlock)
                             keep = !same;
                             if (FILL && keep) {
                                 swap_if(j < i, da, db);
-                                v = single_or_zero(elem_single<S>(T, da, db, other_row), single_rule);
+                                v = single_or_zero(elem_single<S>(T, da, db, other_row), rule_of(rules, max(i, j)));
                             }
                         }
                     }
@@ -446,14 +453,14 @@ __global__ void __launch_bounds__(kBlock)
                 if (FILL && keep) {
                     swap_if(j < i, a, b);
                     v = dbl ? elem_same_double<S>(T, a, b)
-                            : single_or_zero(elem_single<S>(T, a, b, ub[sb.g_row]), single_rule);
+                            : single_or_zero(elem_single<S>(T, a, b, ub[sb.g_row]), rule_of(rules, max(i, j)));
                 }
             } else if (sb.kind == 0) {
                 keep = c == 2 || (c == 4 && dbl);
                 if (FILL && keep) {
                     const S up = B.ua[sb.g_row];
                     swap_if(j < i, a, b);
-                    v = c == 2 ? single_or_zero(elem_single<S>(T, a, b, up), single_rule)
+                    v = c == 2 ? single_or_zero(elem_single<S>(T, a, b, up), rule_of(rules, max(i, j)))
                                : elem_same_double<S>(T, a, b);
                 }
             } else {
@@ -462,7 +469,7 @@ __global__ void __launch_bounds__(kBlock)
                     S ua = B.ua[sb.g_row], uc = B.ua[sb.g_col];
                     swap_if(j < i, ua, uc);
                     swap_if(j < i, a, b);
-                    v = c == 0 ? single_or_zero(elem_single<S>(T, ua, uc, a), single_rule)
+                    v = c == 0 ? single_or_zero(elem_single<S>(T, ua, uc, a), rule_of(rules, max(i, j)))
                                : elem_opp_double<S>(T, ua, uc, a, b);
                 }
             }
@@ -1182,7 +1189,7 @@ hipError_t sort_grouped(hci_ctx* c, Scratch& tmp, S* k_in, S* k_out, int32_t* q_
 
 template <class S>
 int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t Ua, int32_t Ub,
-                int32_t n_pairs, const hci_build_opts& o, hci_csr* H) {
+                int32_t n_pairs, const hci_build_opts& o, const SingleRules& rules, hci_csr* H) {
     hipStream_t s = c->stream;
     const int64_t N = B.N;
     // beta-minus-one-electron keys, sorted within each alpha group
@@ -1382,7 +1389,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (n_sb > 0) {
         hipLaunchKernelGGL(k_win_init, grid_for(n_sb, 256), 256, 0, s, d_sb, n_sb, d_wc0, d_wnc);
         hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           o.single_rule, dbl, d_wc, none, nullptr);
+                           rules, dbl, d_wc, none, nullptr);
     }
     // window slots, offsets, the non-empty windows' new indices and the entry total
     hipLaunchKernelGGL(k_win_sizes, grid_for(n_wall + 1, 256), 256, 0, s, d_wc, n_wall, d_sz, d_flag);
@@ -1417,7 +1424,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMalloc(&H->d_win, (size_t)n_win * sizeof(Window) + 16));
     if (n_sb > 0)
         hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           o.single_rule, dbl, nullptr, fo, (unsigned long long*)d_nnz);
+                           rules, dbl, nullptr, fo, (unsigned long long*)d_nnz);
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
                        H->d_win);
     if (n_win > 0) {
@@ -1488,8 +1495,8 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
 
 template <class S>
 int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int64_t N,
-               const hci_csr* prev, int64_t n_stored, const hci_build_opts& o, hci_csr** out,
-               hci_build_stats* stats) {
+               const hci_csr* prev, int64_t n_stored, const hci_build_opts& o, const SingleRules& rules,
+               hci_csr** out, hci_build_stats* stats) {
     hipStream_t s = c->stream;
     Scratch tmp;
     DevBufs db;
@@ -1665,7 +1672,7 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
     const DevInts T = c->ints();
 
     if (o.storage == HCI_STORE_TILED) {
-        int trc = build_tiled<S>(c, tmp, B, ub, Ua, Ub, n_pairs, o, H);
+        int trc = build_tiled<S>(c, tmp, B, ub, Ua, Ub, n_pairs, o, rules, H);
         if (trc) return fail_free(trc);
         BTH(hipMalloc(&H->d_diag, (size_t)N * 8 + 16));
         hipLaunchKernelGGL(k_gather_f64, grid_for(N, 256), 256, 0, s, sa_idx, diag_dev, N, H->d_diag);
@@ -1825,8 +1832,6 @@ extern "C" int hci_build_h(hci_ctx* c, const uint64_t* dets, const double* diag,
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: bad single_rule");
     if (opts->storage != HCI_STORE_FULL && opts->storage != HCI_STORE_UPPER && opts->storage != HCI_STORE_TILED)
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: bad storage");
-    if (opts->storage == HCI_STORE_TILED && prev)
-        return hci_fail(c, HCI_ERR_ARG, "hci_build_h: HCI_STORE_TILED builds are not incremental");
     if (prev) {
         if (prev->ctx != c) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: prev from another context");
         if (n_stored != prev->n || n < prev->n)
@@ -1836,6 +1841,25 @@ extern "C" int hci_build_h(hci_ctx* c, const uint64_t* dets, const double* diag,
     } else if (n_stored != 0) {
         return hci_fail(c, HCI_ERR_ARG, "hci_build_h: n_stored must be 0 without prev");
     }
+    // single-storage rule of every pair: the previous builds' segments, then
+    // this build's rule up to n (adjacent equal rules merged)
+    std::vector<std::pair<int64_t, int>> segs;
+    if (prev) segs = prev->rule_segs;
+    if (!segs.empty() && segs.back().second == opts->single_rule)
+        segs.back().first = n;
+    else
+        segs.emplace_back(n, opts->single_rule);
+    if ((int)segs.size() > kMaxRuleSegs)
+        return hci_fail(c, HCI_ERR_ARG, "hci_build_h: more than 8 alternations of the single rule across builds");
+    SingleRules rules{};
+    rules.n = (int32_t)segs.size();
+    for (size_t k = 0; k < segs.size(); ++k) {
+        rules.end[k] = (int32_t)segs[k].first;
+        rules.rule[k] = segs[k].second;
+    }
+    // a tiled matrix is rebuilt whole: the pairs among the previous
+    // determinants are found again and keep the rule they were stored under
+    const bool tiled_inc = prev && opts->storage == HCI_STORE_TILED;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(c, hipSetDevice(c->device));
     if (opts->storage != HCI_STORE_TILED) recycled_flush(c);
@@ -1851,9 +1875,12 @@ extern "C" int hci_build_h(hci_ctx* c, const uint64_t* dets, const double* diag,
         d_dets = tmp_dets;
         d_diag = tmp_diag;
     }
+    const hci_csr* pv = tiled_inc ? nullptr : prev;
+    const int64_t ns = tiled_inc ? 0 : n_stored;
     int rc = c->n_orb <= 64
-                 ? build_impl<uint64_t>(c, d_dets, d_diag, n, prev, n_stored, *opts, out, stats)
-                 : build_impl<u128>(c, d_dets, d_diag, n, prev, n_stored, *opts, out, stats);
+                 ? build_impl<uint64_t>(c, d_dets, d_diag, n, pv, ns, *opts, rules, out, stats)
+                 : build_impl<u128>(c, d_dets, d_diag, n, pv, ns, *opts, rules, out, stats);
+    if (rc == HCI_OK && *out) (*out)->rule_segs = segs;
     hipStreamSynchronize(c->stream);
     hipFree(tmp_dets);
     hipFree(tmp_diag);

@@ -63,6 +63,17 @@ constexpr int kSmallWin = 8;
 struct TileTask {
     int32_t r0, nr, w_begin, w_end, w_split;
 };
+// Which single-storage rule (HCI_SINGLES_*) a pair follows: the rule of the
+// first segment whose end exceeds the pair's larger caller index.  A fresh
+// build has one segment; an incremental build appends (n, its rule) to the
+// previous matrix's segments, as the reference's builds add the pairs with a
+// new index under the rule of that build (src/var/davidson.rs:55-61).
+constexpr int kMaxRuleSegs = 8;
+struct SingleRules {
+    int32_t n;
+    int32_t end[kMaxRuleSegs];
+    int32_t rule[kMaxRuleSegs];
+};
 
 struct hci_ctx {
     int device = 0;
@@ -157,6 +168,8 @@ struct hci_csr {
     double* d_xb = nullptr;        // x in beta-major order
     float* d_x32 = nullptr;        // FP32 copies of x for FP16-value products (HCI_PREC_F16)
     float* d_xb32 = nullptr;
+    // single-storage rule segments of the pairs (see SingleRules)
+    std::vector<std::pair<int64_t, int>> rule_segs;
 };
 
 // recycled entry arrays (ctx.hip): recycled_take hands out a kept buffer of at

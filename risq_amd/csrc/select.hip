@@ -52,6 +52,8 @@ struct SelectLists {
     // sum of the squared magnitudes after each entry of its list
     // (sum_remaining_h_squared, src/excite/init.rs:282-309)
     double *d_or = nullptr, *d_sr = nullptr, *d_gr = nullptr;
+    // sum of the magnitudes after each entry (sum_remaining_abs_h)
+    double *d_oa = nullptr, *d_sa = nullptr, *d_ga = nullptr;
     uint16_t *d_ot = nullptr, *d_st = nullptr, *d_gt = nullptr;  // targets r | s << 8
     int32_t *d_vpos = nullptr, *d_spos = nullptr;
     ~SelectLists() {
@@ -61,6 +63,9 @@ struct SelectLists {
         hipFree(d_or);
         hipFree(d_sr);
         hipFree(d_gr);
+        hipFree(d_oa);
+        hipFree(d_sa);
+        hipFree(d_ga);
         hipFree(d_ot);
         hipFree(d_st);
         hipFree(d_gt);
@@ -88,11 +93,13 @@ struct DevSel {
     const uint16_t *ot, *st, *gt;
     const int32_t *vpos, *spos;
     const double *orem, *srem, *grem;
+    const double *oabs, *sabs, *gabs;
 };
 
 DevSel dev_sel(const SelectLists& L) {
     return DevSel{L.nv,   L.Lo,   L.Ls,   L.Lg,     L.max_opp, L.max_same, L.max_sing, L.d_oh, L.d_sh,
-                  L.d_gh, L.d_ot, L.d_st, L.d_gt,   L.d_vpos,  L.d_spos,   L.d_or,     L.d_sr, L.d_gr};
+                  L.d_gh, L.d_ot, L.d_st, L.d_gt,   L.d_vpos,  L.d_spos,   L.d_or,     L.d_sr, L.d_gr,
+                  L.d_oa, L.d_sa, L.d_ga};
 }
 
 struct NoBreak {
@@ -605,6 +612,236 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+
+// membership of (up, dn) in the sorted space (didx: the space's indices in
+// cmp_det order)
+template <class S>
+__device__ __forceinline__ bool in_space(S up, S dn, const uint64_t* space, const int64_t* didx, int64_t n) {
+    uint64_t a[4];
+    store_string<S>(a, up);
+    store_string<S>(a + 2, dn);
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cmp_det(space + 4 * didx[mid], a) < 0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && cmp_det(space + 4 * didx[lo], a) == 0;
+}
+
+// The deterministic stage of new_semistoch_enpt2_dtm_diag_singles
+// (src/semistoch/mod.rs:262-318) for one determinant with coefficient c:
+// every list of its valence pairs and electrons, in the order of
+// valence_elecs_and_epairs, read through excites_from_det_and_orbs (targets
+// already occupied or inside the space are skipped first,
+// src/excite/iterator.rs:113-130).  The first remaining entry with
+// |stored magnitude| * |c| < eps ends the list: sample(kind, alpha, p, q,
+// sum_remaining_abs_h, sum_remaining_h_squared) of that entry.  Each entry
+// before it: emit(up', dn', H_ai c, code, in_psi), in_psi false for a single
+// with |H_ai c| < eps.  Targets inside the space above the cut are emitted too
+// (the caller's sorted pass drops them); only the cut test needs membership.
+template <class S, class Emit, class Sample>
+__device__ void walk_sd(const DevSel& L, const DevInts& T, S vmask, S up, S dn, double c, double eps,
+                        const uint64_t* space, const int64_t* didx, int64_t n, Emit emit, Sample sample) {
+    const double ac = fabs(c);
+    for (S a = vmask & up; a; a &= a - (S)1) {
+        const int i = ctz(a);
+        for (S b = vmask & dn; b; b &= b - (S)1) {
+            const int j = ctz(b);
+            const int64_t base = ((int64_t)L.vpos[i] * L.nv + L.vpos[j]) * L.Lo;
+            for (int64_t e = 0; e < L.Lo; ++e) {
+                const uint16_t t = L.ot[base + e];
+                const int r = t & 0xFF, s = t >> 8;
+                if (((up >> r) & (S)1) || ((dn >> s) & (S)1)) continue;
+                const S nu = (up ^ bit<S>(i)) | bit<S>(r), nd = (dn ^ bit<S>(j)) | bit<S>(s);
+                if (L.oh[base + e] * ac < eps) {
+                    if (in_space<S>(nu, nd, space, didx, n)) continue;
+                    sample(0, -1, i, j, L.oabs[base + e], L.orem[base + e]);
+                    break;
+                }
+                emit(nu, nd, elem_opp_double<S>(T, up, nu, dn, nd) * c, exc_code(0, 0, i, j, r, s), true);
+            }
+        }
+    }
+    if (L.Ls > 0) {
+        for (int spin = 0; spin < 2; ++spin) {
+            const S cfg = spin == 0 ? up : dn;
+            for (S a = vmask & cfg; a; a &= a - (S)1) {
+                const int i = ctz(a);
+                for (S b = a & (a - (S)1); b; b &= b - (S)1) {
+                    const int j = ctz(b);
+                    const int64_t base = (int64_t)L.spos[L.vpos[i] * L.nv + L.vpos[j]] * L.Ls;
+                    for (int64_t e = 0; e < L.Ls; ++e) {
+                        const uint16_t t = L.st[base + e];
+                        const int r = t & 0xFF, s = t >> 8;
+                        if (((cfg >> r) & (S)1) || ((cfg >> s) & (S)1)) continue;
+                        const S nw = (cfg ^ bit<S>(i) ^ bit<S>(j)) | bit<S>(r) | bit<S>(s);
+                        const S nu = spin == 0 ? nw : up, nd = spin == 0 ? dn : nw;
+                        if (L.sh[base + e] * ac < eps) {
+                            if (in_space<S>(nu, nd, space, didx, n)) continue;
+                            sample(1, spin == 0, i, j, L.sabs[base + e], L.srem[base + e]);
+                            break;
+                        }
+                        emit(nu, nd, elem_same_double<S>(T, cfg, nw) * c, exc_code(1, spin == 0, i, j, r, s), true);
+                    }
+                }
+            }
+        }
+    }
+    for (int spin = 0; spin < 2; ++spin) {
+        const S cfg = spin == 0 ? up : dn, other = spin == 0 ? dn : up;
+        for (S a = vmask & cfg; a; a &= a - (S)1) {
+            const int i = ctz(a);
+            const int64_t base = (int64_t)L.vpos[i] * L.Lg;
+            for (int64_t e = 0; e < L.Lg; ++e) {
+                const int r = L.gt[base + e] & 0xFF;
+                if ((cfg >> r) & (S)1) continue;
+                const S nw = (cfg ^ bit<S>(i)) | bit<S>(r);
+                const S nu = spin == 0 ? nw : up, nd = spin == 0 ? dn : nw;
+                if (L.gh[base + e] * ac < eps) {
+                    if (in_space<S>(nu, nd, space, didx, n)) continue;
+                    sample(2, spin == 0, i, -1, L.gabs[base + e], L.grem[base + e]);
+                    break;
+                }
+                const double h = elem_single<S>(T, cfg, nw, other) * c;
+                emit(nu, nd, h, exc_code(2, spin == 0, i, 0, r, 0), fabs(h) >= eps);
+            }
+        }
+    }
+}
+
+// candidate code bit 31: the candidate joins h_psi
+constexpr uint32_t kInPsi = 1u << 31;
+
+template <class S, bool FILL>
+__global__ void k_sd_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const uint64_t* __restrict__ dets,
+                          const double* __restrict__ coeffs, int64_t n, double eps, const int64_t* __restrict__ didx,
+                          int64_t* counts, int64_t* scounts, const int64_t* __restrict__ offs,
+                          const int64_t* __restrict__ soffs, uint64_t* words, double* vals, int64_t* src,
+                          uint32_t* codes, hci_det_orb_sample* samples) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const S up = load_string<S>(dets + 4 * k), dn = load_string<S>(dets + 4 * k + 2);
+    S vmask;
+    if constexpr (sizeof(S) > 8)
+        vmask = ((S)vm1 << 64) | (S)vm0;
+    else
+        vmask = (S)vm0;
+    const double c = coeffs[k];
+    int64_t cnt = 0, scnt = 0;
+    const int64_t o = FILL ? offs[k] : 0, so = FILL && samples ? soffs[k] : 0;
+    walk_sd<S>(
+        L, T, vmask, up, dn, c, eps, dets, didx, n,
+        [&](S nu, S nd, double h, uint32_t code, bool psi) {
+            if (FILL) {
+                store_string<S>(words + 4 * (o + cnt), nu);
+                store_string<S>(words + 4 * (o + cnt) + 2, nd);
+                vals[o + cnt] = h;
+                src[o + cnt] = k;
+                codes[o + cnt] = code | (psi ? kInPsi : 0u);
+            }
+            ++cnt;
+        },
+        [&](int kind, int alpha, int p, int q, double rabs, double rsq) {
+            if (FILL && samples) {
+                hci_det_orb_sample r;
+                r.det = k;
+                r.kind = kind;
+                r.is_alpha = alpha;
+                r.p = p;
+                r.q = q;
+                r.sum_abs_h = rabs;
+                r.sum_h_squared = rsq;
+                r.sum_abs_hc = fabs(c) * rabs;
+                r.sum_hc_squared = c * c * rsq;
+                samples[so + scnt] = r;
+            }
+            ++scnt;
+        });
+    if (!FILL) {
+        counts[k] = cnt;
+        scounts[k] = scnt;
+    }
+}
+
+// One thread per run of equal targets (sorted, stable: discovery order within
+// the run).  A target outside the space: its h_psi candidates summed in
+// order, the first of them recorded (it creates the entry), and every
+// candidate's |H_ai c_i| raised into rel_probs of its source; keep = outside
+// the space and at least one h_psi candidate.
+__global__ void k_sd_run(const uint64_t* w, const double* v, const int64_t* src, const uint32_t* codes,
+                         const int64_t* idx, const int64_t* start, int64_t runs, const uint64_t* space,
+                         const int64_t* sidx, int64_t n, double* num, int64_t* first, int64_t* keep, double* rel) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= runs) return;
+    const int64_t a = start[u], b = start[u + 1];
+    const uint64_t* d = w + 4 * idx[a];
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cmp_det(space + 4 * sidx[mid], d) < 0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const bool ext = !(lo < n && cmp_det(space + 4 * sidx[lo], d) == 0);
+    double sum = 0.0;
+    int64_t f = -1;
+    if (ext) {
+        for (int64_t i = a; i < b; ++i) {
+            const int64_t ci = idx[i];
+            const double h = v[ci];
+            atomicMax((unsigned long long*)(rel + src[ci]), (unsigned long long)__double_as_longlong(fabs(h)));
+            if (codes[ci] & kInPsi) {
+                if (f < 0) f = ci;
+                sum += h;
+            }
+        }
+    }
+    keep[u] = (ext && f >= 0) ? 1 : 0;
+    num[u] = sum;
+    first[u] = f;
+}
+
+__global__ void k_sd_first(const int64_t* first_run, const int64_t* keep, const int64_t* kpos, int64_t runs,
+                           int64_t* first, int64_t* run_id) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= runs || !keep[u]) return;
+    first[kpos[u]] = first_run[u];
+    run_id[kpos[u]] = u;
+}
+
+// e2 terms in first-discovery order: num^2 / (e0 - H_aa), H_aa from the first
+// discoverer's diagonal (Det::new_diag); fixed-order partial sums per block
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_sd_terms(DevInts T, const uint64_t* __restrict__ dets, const double* __restrict__ diag,
+               const int64_t* __restrict__ src, const uint32_t* __restrict__ codes, const int64_t* __restrict__ first,
+               const int64_t* __restrict__ run_id, const double* __restrict__ num, int64_t k, double e0,
+               double* partial) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const int64_t per = (k + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * per, hi = min(k, lo + per);
+    // each block sums a contiguous range, thread by thread in order of its stride
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const int64_t f = first[i], j = src[f];
+        const double d = new_diag<S>(T, load_string<S>(dets + 4 * j), load_string<S>(dets + 4 * j + 2), diag[j],
+                                     codes[f] & ~kInPsi);
+        const double x = num[run_id[i]];
+        acc += x * x / (e0 - d);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
 struct Tmp {
     void* p = nullptr;
     size_t bytes = 0;
@@ -802,6 +1039,19 @@ int hci_excite_init(hci_ctx* c, int n_core, int n_up, int n_dn, double eps_var, 
     suffix(oh, L->Lo, orem);
     suffix(sh, L->Ls, srem);
     suffix(gh, L->Lg, grem);
+    // sum_remaining_abs_h: rem[e] = rem[e + 1] + a[e + 1] (same file, :288-309)
+    auto suffix_abs = [](const std::vector<double>& a, int64_t len, std::vector<double>& rem) {
+        rem.assign(a.size(), 0.0);
+        if (len <= 0) return;
+        for (size_t base = 0; base < a.size(); base += (size_t)len) {
+            rem[base + len - 1] = 0.0;
+            for (int64_t e = len - 2; e >= 0; --e) rem[base + e] = rem[base + e + 1] + a[base + e + 1];
+        }
+    };
+    std::vector<double> oabs, sabs, gabs;
+    suffix_abs(oh, L->Lo, oabs);
+    suffix_abs(sh, L->Ls, sabs);
+    suffix_abs(gh, L->Lg, gabs);
     // initial epsilon from Hartree-Fock (src/wf/eps.rs:44-171, z_sym = 1)
     {
         auto occ = [](int p, int ne) { return p < ne; };  // Hartree-Fock: the lowest orbitals by index
@@ -855,7 +1105,8 @@ int hci_excite_init(hci_ctx* c, int n_core, int n_up, int n_dn, double eps_var, 
     if ((rc = up(&L->d_oh, oh)) || (rc = up(&L->d_ot, ot)) || (rc = up(&L->d_sh, sh)) || (rc = up(&L->d_st, st)) ||
         (rc = up(&L->d_gh, gh)) || (rc = up(&L->d_gt, gt)) || (rc = up(&L->d_vpos, vpos)) ||
         (rc = up(&L->d_spos, spos)) || (rc = up(&L->d_or, orem)) || (rc = up(&L->d_sr, srem)) ||
-        (rc = up(&L->d_gr, grem)))
+        (rc = up(&L->d_gr, grem)) || (rc = up(&L->d_oa, oabs)) || (rc = up(&L->d_sa, sabs)) ||
+        (rc = up(&L->d_ga, gabs)))
         return rc;
     if (eps_start) *eps_start = L->eps_start;
     return HCI_OK;
@@ -1257,6 +1508,160 @@ int hci_matmul_external(hci_ctx* c, const uint64_t* dets, const double* coeffs, 
     HIP_TRY(c, hipMemcpyAsync(out_coeffs, ov, (size_t)k * 8, out_kind, s));
     if (out_diag) HIP_TRY(c, hipMemcpyAsync(out_diag, od, (size_t)k * 8, out_kind, s));
     HIP_TRY(c, hipStreamSynchronize(s));
+    return HCI_OK;
+}
+
+
+int hci_semistoch_dtm(hci_ctx* c, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
+                      double e0, double eps, double* e2, int64_t* n_ext, double* rel_probs,
+                      hci_det_orb_sample* samples, int64_t cap, int64_t* n_samples, int mem) {
+    if (!c) return HCI_ERR_ARG;
+    if (!c->sel) return hci_fail(c, HCI_ERR_STATE, "hci_semistoch_dtm: call hci_excite_init first");
+    if (n < 1 || !dets || !coeffs || !diag || !e2 || !(eps > 0.0) || cap < 0 || (cap > 0 && !samples))
+        return hci_fail(c, HCI_ERR_ARG, "hci_semistoch_dtm: bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const SelectLists& L = *c->sel;
+    const DevSel D = dev_sel(L);
+    const DevInts T = c->ints();
+    const bool wide = c->n_orb > 64;
+    const int nw = wide ? 4 : 2;
+    const hipMemcpyKind out_kind = mem == HCI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    Bufs b;
+    Tmp tmp;
+    const uint64_t* d_dets = dets;
+    const double* d_coef = coeffs;
+    const double* d_diag = diag;
+    if (mem == HCI_MEM_HOST) {
+        uint64_t* w = nullptr;
+        double *q = nullptr, *g = nullptr;
+        HIP_TRY(c, b.alloc(&w, (size_t)n * 4));
+        HIP_TRY(c, b.alloc(&q, (size_t)n));
+        HIP_TRY(c, b.alloc(&g, (size_t)n));
+        HIP_TRY(c, hipMemcpyAsync(w, dets, (size_t)n * 32, hipMemcpyHostToDevice, s));
+        HIP_TRY(c, hipMemcpyAsync(q, coeffs, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(c, hipMemcpyAsync(g, diag, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        d_dets = w;
+        d_coef = q;
+        d_diag = g;
+    }
+    auto scan = [&](const int64_t* in, int64_t* o, int64_t len) -> hipError_t {
+        size_t need = 0;
+        hipError_t e = rocprim::exclusive_scan(nullptr, need, in, o, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
+        if (e != hipSuccess) return e;
+        if ((e = tmp.ensure(need)) != hipSuccess) return e;
+        return rocprim::exclusive_scan(tmp.p, need, in, o, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
+    };
+    int64_t *cnt = nullptr, *off = nullptr, *scnt = nullptr, *soff = nullptr, *didx = nullptr;
+    double* rel = nullptr;
+    HIP_TRY(c, b.alloc(&cnt, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&off, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&scnt, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&soff, (size_t)n + 1));
+    HIP_TRY(c, b.alloc(&didx, (size_t)n));
+    HIP_TRY(c, b.alloc(&rel, (size_t)n));
+    HIP_TRY(c, hipMemsetAsync(rel, 0, (size_t)n * 8, s));
+    HIP_TRY(c, sort_dets(tmp, b, d_dets, n, nw, didx, s));
+#define HCI_SD_WALK(FILL, ...)                                                                                   \
+    do {                                                                                                          \
+        if (wide)                                                                                                 \
+            hipLaunchKernelGGL((k_sd_walk<u128, FILL>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1], \
+                               d_dets, d_coef, n, eps, didx, __VA_ARGS__);                                        \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_sd_walk<uint64_t, FILL>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0],        \
+                               L.vmask[1], d_dets, d_coef, n, eps, didx, __VA_ARGS__);                            \
+    } while (0)
+    HCI_SD_WALK(false, cnt, scnt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
+    HIP_TRY(c, hipMemsetAsync(scnt + n, 0, 8, s));
+    HIP_TRY(c, scan(cnt, off, n + 1));
+    HIP_TRY(c, scan(scnt, soff, n + 1));
+    int64_t m = 0, ns = 0;
+    HIP_TRY(c, hipMemcpyAsync(&m, off + n, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&ns, soff + n, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (n_samples) *n_samples = ns;
+    if (samples && ns > cap)
+        return hci_fail(c, HCI_ERR_ARG, "hci_semistoch_dtm: sample capacity too small (n_samples holds the need)");
+    uint64_t* cw = nullptr;
+    double* cv = nullptr;
+    int64_t* csrc = nullptr;
+    uint32_t* ccode = nullptr;
+    hci_det_orb_sample* d_smp = nullptr;
+    HIP_TRY(c, b.alloc(&cw, (size_t)m * 4 + 4));
+    HIP_TRY(c, b.alloc(&cv, (size_t)m + 1));
+    HIP_TRY(c, b.alloc(&csrc, (size_t)m + 1));
+    HIP_TRY(c, b.alloc(&ccode, (size_t)m + 1));
+    if (samples && ns > 0) {
+        if (mem == HCI_MEM_DEVICE)
+            d_smp = samples;
+        else
+            HIP_TRY(c, b.alloc(&d_smp, (size_t)ns));
+    }
+    HCI_SD_WALK(true, nullptr, nullptr, off, soff, cw, cv, csrc, ccode, d_smp);
+#undef HCI_SD_WALK
+    HIP_TRY(c, hipGetLastError());
+    double e_sum = 0.0;
+    int64_t k = 0;
+    if (m > 0) {
+        int64_t *cidx = nullptr, *head = nullptr, *hpos = nullptr, *start = nullptr, *keep = nullptr, *kpos = nullptr,
+                *frun = nullptr;
+        double* rv = nullptr;
+        HIP_TRY(c, b.alloc(&cidx, (size_t)m));
+        HIP_TRY(c, sort_dets(tmp, b, cw, m, nw, cidx, s));
+        HIP_TRY(c, b.alloc(&head, (size_t)m + 1));
+        HIP_TRY(c, b.alloc(&hpos, (size_t)m + 1));
+        hipLaunchKernelGGL(k_run_heads, grid_for(m, 256), 256, 0, s, cw, cidx, m, head);
+        HIP_TRY(c, hipMemsetAsync(head + m, 0, 8, s));
+        HIP_TRY(c, scan(head, hpos, m + 1));
+        int64_t runs = 0;
+        HIP_TRY(c, hipMemcpyAsync(&runs, hpos + m, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        HIP_TRY(c, b.alloc(&start, (size_t)runs + 1));
+        HIP_TRY(c, b.alloc(&rv, (size_t)runs));
+        HIP_TRY(c, b.alloc(&frun, (size_t)runs));
+        HIP_TRY(c, b.alloc(&keep, (size_t)runs + 1));
+        HIP_TRY(c, b.alloc(&kpos, (size_t)runs + 1));
+        hipLaunchKernelGGL(k_run_starts, grid_for(m + 1, 256), 256, 0, s, head, hpos, m, start);
+        hipLaunchKernelGGL(k_sd_run, grid_for(runs, 256), 256, 0, s, cw, cv, csrc, ccode, cidx, start, runs, d_dets,
+                           didx, n, rv, frun, keep, rel);
+        HIP_TRY(c, hipMemsetAsync(keep + runs, 0, 8, s));
+        HIP_TRY(c, scan(keep, kpos, runs + 1));
+        HIP_TRY(c, hipMemcpyAsync(&k, kpos + runs, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        if (k > 0) {
+            // h_psi in first-discovery order (the order Wf::push gives it)
+            int64_t *first = nullptr, *rid = nullptr, *first2 = nullptr, *rid2 = nullptr;
+            HIP_TRY(c, b.alloc(&first, (size_t)k));
+            HIP_TRY(c, b.alloc(&rid, (size_t)k));
+            HIP_TRY(c, b.alloc(&first2, (size_t)k));
+            HIP_TRY(c, b.alloc(&rid2, (size_t)k));
+            hipLaunchKernelGGL(k_sd_first, grid_for(runs, 256), 256, 0, s, frun, keep, kpos, runs, first, rid);
+            size_t need = 0;
+            HIP_TRY(c, rocprim::radix_sort_pairs(nullptr, need, first, first2, rid, rid2, (size_t)k, 0, 64, s));
+            HIP_TRY(c, tmp.ensure(need));
+            HIP_TRY(c, rocprim::radix_sort_pairs(tmp.p, need, first, first2, rid, rid2, (size_t)k, 0, 64, s));
+            double* part = nullptr;
+            HIP_TRY(c, b.alloc(&part, (size_t)kPtBlocks));
+            if (wide)
+                hipLaunchKernelGGL(k_sd_terms<u128>, kPtBlocks, 256, 0, s, T, d_dets, d_diag, csrc, ccode, first2,
+                                   rid2, rv, k, e0, part);
+            else
+                hipLaunchKernelGGL(k_sd_terms<uint64_t>, kPtBlocks, 256, 0, s, T, d_dets, d_diag, csrc, ccode, first2,
+                                   rid2, rv, k, e0, part);
+            HIP_TRY(c, hipGetLastError());
+            std::vector<double> hp(kPtBlocks);
+            HIP_TRY(c, hipMemcpyAsync(hp.data(), part, (size_t)kPtBlocks * 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+            for (double x : hp) e_sum += x;
+        }
+    }
+    if (rel_probs) HIP_TRY(c, hipMemcpyAsync(rel_probs, rel, (size_t)n * 8, out_kind, s));
+    if (samples && ns > 0 && mem == HCI_MEM_HOST)
+        HIP_TRY(c, hipMemcpyAsync(samples, d_smp, (size_t)ns * sizeof(hci_det_orb_sample), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    *e2 = e_sum;
+    if (n_ext) *n_ext = k;
     return HCI_OK;
 }
 

@@ -131,7 +131,14 @@ SIGNATURES = {
                               C.POINTER(C.c_int64), C.c_int]),
     "hci_matmul_external": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, _P, _P, _P, C.c_int64,
                                       C.POINTER(C.c_int64), _P, C.c_int]),
+    "hci_semistoch_dtm": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int64), _P, _P, C.c_int64, C.POINTER(C.c_int64), C.c_int]),
 }
+
+# hci_det_orb_sample (include/hci_gpu.h), the reference's DetOrbSample
+DET_ORB_SAMPLE = np.dtype([("det", np.int64), ("kind", np.int32), ("is_alpha", np.int32), ("p", np.int32),
+                           ("q", np.int32), ("sum_abs_h", np.float64), ("sum_h_squared", np.float64),
+                           ("sum_abs_hc", np.float64), ("sum_hc_squared", np.float64)])
 
 _lib = None
 
@@ -324,6 +331,32 @@ class HciContext:
                 return out[:k], oc[:k], od[:k], rem
             if cnt.value > cap:
                 cap = cnt.value
+                continue
+            self._check(rc)
+
+    def semistoch_dtm(self, dets: np.ndarray, coeffs: np.ndarray, diag: np.ndarray, e0: float, eps: float):
+        """The deterministic stage of new_semistoch_enpt2_dtm_diag_singles
+        (src/semistoch/mod.rs:262-318): (e2 of h_psi, its size, rel_probs per
+        determinant, the DetOrbSample records as a DET_ORB_SAMPLE array).
+        Needs excite_init on the same integrals."""
+        dets = np.ascontiguousarray(dets, dtype=np.uint64)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        n = dets.shape[0]
+        e2 = C.c_double()
+        n_ext = C.c_int64()
+        ns = C.c_int64()
+        rel = np.empty(n)
+        cap = max(1024, 64 * n)
+        while True:
+            smp = np.empty(cap, dtype=DET_ORB_SAMPLE)
+            rc = self.lib.hci_semistoch_dtm(self.handle, _ptr(dets), _ptr(coeffs), _ptr(diag), n, e0, eps,
+                                            C.byref(e2), C.byref(n_ext), _ptr(rel), _ptr(smp), cap, C.byref(ns),
+                                            MEM_HOST)
+            if rc == HCI_OK:
+                return e2.value, n_ext.value, rel, smp[:ns.value]
+            if ns.value > cap:
+                cap = ns.value
                 continue
             self._check(rc)
 
@@ -526,10 +559,13 @@ def gen_sparse_ham_fast_lookup(ctx: HciContext, dets: np.ndarray, diag: np.ndarr
 def sparse_optimize(ctx: HciContext, dets: np.ndarray, diag: np.ndarray,
                     prev: SparseMatUpperTri | None = None, n_stored: int = 0,
                     init: np.ndarray | None = None, coeff_eps: float = 1e-4,
-                    energy_eps: float = 1e-8, **dav) -> tuple[SparseMatUpperTri, DavidsonResult]:
+                    energy_eps: float = 1e-8, storage: int = STORE_TILED,
+                    **dav) -> tuple[SparseMatUpperTri, DavidsonResult]:
     """Build H then run Davidson (reference src/var/davidson.rs:43-113).  The
-    n <= 100 small-system path of the reference stores singles iff H != 0."""
+    n <= 100 small-system path of the reference stores singles iff H != 0.
+    storage: tiled (the fast product; an incremental build rebuilds it whole)
+    or FULL (keeps the previous entries, bitwise reproducible products)."""
     rule = SINGLES_NONZERO if dets.shape[0] <= 100 else SINGLES_ABS_GT_1E12
-    h = ctx.build_h(dets, diag, prev=prev, n_stored=n_stored, single_rule=rule)
+    h = ctx.build_h(dets, diag, prev=prev, n_stored=n_stored, single_rule=rule, storage=storage)
     res = ctx.davidson(h, init=init, tol=coeff_eps, etol=energy_eps, **dav)
     return h, res

@@ -374,12 +374,17 @@ class GpuSolver:
     """Selection, elements, diagonals, incremental H builds and Davidson on the
     GPU library (reference semantics: FP64 values, Davidson tol 1e-4 / etol
     1e-8, max 50 iterations, restart to the Ritz vector).  With
-    library_select=False the selection runs in this module (find_new_dets)."""
+    library_select=False the selection runs in this module (find_new_dets).
+    storage: tiled by default (rebuilt whole at each iteration, keeping the
+    rule each pair was stored under), or FULL (entries of the previous
+    iteration kept and renumbered)."""
 
-    def __init__(self, ctx, ints: Integrals, library_select: bool = True):
+    def __init__(self, ctx, ints: Integrals, library_select: bool = True, storage: int | None = None):
+        from .hci import STORE_TILED
         self.ctx = ctx
         ctx.set_integrals(ints)
         self.h = None
+        self.storage = STORE_TILED if storage is None else storage
         if library_select:
             self.select_init = lambda n_core, n_up, n_dn, eps_var: ctx.excite_init(n_core, n_up, n_dn, eps_var)
             self.select = lambda words, coeffs, eps, singles, diag: ctx.select(words, coeffs, eps, singles,
@@ -394,7 +399,8 @@ class GpuSolver:
     def optimize(self, words, diag, init):
         from .hci import sparse_optimize
         n_stored = self.h.n if self.h is not None else 0
-        h, res = sparse_optimize(self.ctx, words, diag, prev=self.h, n_stored=n_stored, init=init)
+        h, res = sparse_optimize(self.ctx, words, diag, prev=self.h, n_stored=n_stored, init=init,
+                                 storage=self.storage)
         if self.h is not None:
             self.h.free()
         self.h = h

@@ -415,13 +415,53 @@ def test_tiled_storage_large_groups(gpu_ctx, oracle_lib, prec):
     assert abs(ra.energy - rb.energy) < 1e-8
 
 
-def test_tiled_rejects_incremental(gpu_ctx):
-    ints, dets = mol_space(*MOL0)
+@pytest.mark.parametrize("prec", [64, 32])
+def test_tiled_incremental_build(gpu_ctx, oracle_lib, prec):
+    """The HCI loop's chain of builds on tiled storage: an n <= 100 build
+    (singles iff H != 0), then two incremental builds (|H| > 1e-12 for pairs
+    with a new index).  Downloads equal the FULL-storage chain's and the
+    oracle's (its n <= 100 build, then its fast-lookup builds with n_stored);
+    the tiny singles of the first build survive, none is added later; the
+    product matches the oracle's."""
+    from helpers import spin_flip_closed, tiny_single_molecular
+    ints = tiny_single_molecular()
+    dets = spin_flip_closed(full_space(8, 3, 3))
+    n1, n2, n3 = 99, 1500, dets.shape[0]
     gpu_ctx.set_integrals(ints)
-    diag = gpu_ctx.ham_diag(dets)
-    h1 = gpu_ctx.build_h(dets[:100], diag[:100], storage=STORE_TILED)
-    with pytest.raises(HciError, match="not incremental"):
-        gpu_ctx.build_h(dets, diag, prev=h1, n_stored=100, storage=STORE_TILED)
+    H = oracle_lib.OracleHam(ints)
+    diag = H.ham_diag(dets)
+    S = oracle_lib.OracleSparse()
+    S.build_small(H, dets[:n1], diag[:n1])
+    S.build(H, dets[:n2], diag[:n2], n_stored=n1)
+    S.build(H, dets, diag, n_stored=n2)
+    o = S.export()
+    chain = {}
+    for storage in (STORE_FULL, STORE_TILED):
+        h1 = gpu_ctx.build_h(dets[:n1], diag[:n1], single_rule=SINGLES_NONZERO, storage=storage, value_prec=prec)
+        h2 = gpu_ctx.build_h(dets[:n2], diag[:n2], prev=h1, n_stored=n1, storage=storage, value_prec=prec)
+        h1.free()
+        h3 = gpu_ctx.build_h(dets[:n3], diag[:n3], prev=h2, n_stored=n2, storage=storage, value_prec=prec)
+        h2.free()
+        chain[storage] = h3
+    t = chain[STORE_TILED].download()
+    for u, v in zip(chain[STORE_FULL].download(), t):
+        assert np.array_equal(u, v)
+    assert np.array_equal(t[0], o[0]) and np.array_equal(t[1], o[1])
+    if prec == 64:
+        assert np.array_equal(t[2].view(np.uint64), o[2].view(np.uint64))
+    else:
+        assert np.max(np.abs(t[2] - o[2]) / np.abs(o[2])) < 1e-13
+    rows = np.repeat(np.arange(n3), np.diff(t[0]))
+    tiny = np.abs(t[2]) <= 1e-12
+    assert tiny.any() and np.all(rows[tiny] < n1) and np.all(t[1][tiny] < n1)
+    x = np.random.default_rng(14).standard_normal(n3)
+    ref = oracle_lib.spmv_csr(*o, x)
+    y = chain[STORE_TILED].matrix_vector_prod(x)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
+    with pytest.raises(HciError, match="another storage kind"):
+        gpu_ctx.build_h(dets, diag, prev=chain[STORE_FULL], n_stored=n3, storage=STORE_TILED)
+    for h in chain.values():
+        h.free()
 
 
 def test_recycled_entry_arrays(gpu_ctx, oracle_lib):

@@ -183,3 +183,72 @@ def test_external_product_device_memory(gpu_ctx):
     assert np.array_equal(ov.to_array(np.float64, k), v)
     assert np.array_equal(od.to_array(np.float64, k), d)
     assert np.array_equal(orem.to_array(np.float64, dets.shape[0]), rem)
+
+
+def _check_semistoch(gpu_ctx, ints, dets, c, e0, eps, n_core, n_up, n_dn):
+    """hci_semistoch_dtm (the deterministic stage of the reference's default
+    PT2) against pt2_oracle.semistoch_dtm: the same h_psi size, e2 with the
+    first discoverers' new_diag within 1e-12 relative, rel_probs and every
+    sample record equal."""
+    from risq_amd.hci_select import new_diag
+    diag = gpu_ctx.ham_diag(dets)
+    gpu_ctx.excite_init(n_core, n_up, n_dn, 1e-3)
+    e2, n_ext, rel, smp = gpu_ctx.semistoch_dtm(dets, c, diag, e0, eps)
+    ow, on, firsts, orel, orec, _ = pt2_oracle.semistoch_dtm(ints, dets, c, e0, eps, n_core, n_up, n_dn)
+    print(f"eps={eps:g} n_core={n_core}: h_psi {n_ext} (oracle {ow.shape[0]}), records {smp.size} "
+          f"(oracle {len(orec)}), e2={e2:.12f}")
+    assert n_ext == ow.shape[0] > 0
+    names = {0: "opp", 1: "same", 2: "sing"}
+    o2 = 0.0
+    for num, (k, (kind, alpha, i, j, tgt)) in zip(on, firsts):
+        parent = (int(dets[k][0]) | int(dets[k][1]) << 64, int(dets[k][2]) | int(dets[k][3]) << 64)
+        if kind == 0:
+            exc = ("opp", (i, j), tgt)
+        elif kind == 1:
+            exc = ("same", (i, j), tgt, alpha == 1)
+        else:
+            exc = ("sing", i, tgt, alpha == 1)
+        o2 += num * num / (e0 - new_diag(ints, parent, float(diag[k]), exc))
+    assert names
+    assert abs(e2 - o2) < 1e-12 * abs(o2)
+    assert np.array_equal(rel, orel)
+    assert smp.size == len(orec) > 0
+    orec = np.array(orec, dtype=object)
+    for f, col in (("det", 0), ("kind", 1), ("is_alpha", 2), ("p", 3), ("q", 4)):
+        assert np.array_equal(smp[f], orec[:, col].astype(np.int64)), f
+    for f, col in (("sum_abs_h", 5), ("sum_h_squared", 6), ("sum_abs_hc", 7), ("sum_hc_squared", 8)):
+        want = orec[:, col].astype(np.float64)
+        assert np.all(np.abs(smp[f] - want) <= 1e-14 * np.abs(want)), f
+
+
+@pytest.mark.parametrize("eps", [1e-5, 1e-6])
+@pytest.mark.parametrize("n_core", [0, 1])
+def test_semistoch_dtm_matches_oracle(gpu_ctx, eps, n_core):
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    if n_core:
+        dets = dets[(dets[:, 0] & dets[:, 2] & np.uint64(1)) == 1][:300]
+    else:
+        dets = dets[:300]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    _check_semistoch(gpu_ctx, ints, dets, c, e0, eps, n_core, 4, 4)
+
+
+def test_semistoch_dtm_two_word_strings(gpu_ctx):
+    ints, dets = mol_space(70, 8, 5, 3, 3, 1500)
+    dets = dets[:12]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    _check_semistoch(gpu_ctx, ints, dets, c, e0, 1e-5, 0, 3, 3)
+
+
+def test_semistoch_dtm_h_psi_equals_dtm_pt(gpu_ctx):
+    """The entries read before the cuts are dtm_pt's candidates, so h_psi is
+    dtm_pt's external space: the same count and e2 up to the diagonals
+    (first discoverer's new_diag against ham_diag)."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    dets = dets[:2000]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    gpu_ctx.excite_init(0, 4, 4, 1e-3)
+    e2, n_ext, rel, smp = gpu_ctx.semistoch_dtm(dets, c, gpu_ctx.ham_diag(dets), e0, 1e-6)
+    p2, pn = gpu_ctx.pt2_dtm(dets, c, e0, 1e-6)
+    assert n_ext == pn
+    assert abs(e2 - p2) < 1e-11 * abs(p2)
