@@ -412,6 +412,12 @@ def main() -> int:
         if args.config == 1:
             compare("fp32_learned_path", PREC_F32, 1, args.keep, PRECOND_MLP, args.max_sub,
                     "FP32 values (FP32 -> FP64 refinement, no FP16 phase) with the learned correction")
+            if os.path.exists(precond_train.SMALL_RECIPE_PATH):
+                precond_train.load(ctx, precond_train.SMALL_RECIPE_PATH)
+                compare("learned_small_recipe_network", fast_prec, fast_mixed, args.keep, PRECOND_MLP, args.max_sub,
+                        "the timed path with the network trained only on the small systems "
+                        "(n_orb 20-28, 4-5 electrons per spin)")
+                precond_train.load(ctx)
     if args.config == 1 and n == 2_000_000 and not args.no_hci_space:
         out["hci_selected_space"] = hci_space_run(ctx, ints, failures)
     if rank == 0 and not args.no_cpu:

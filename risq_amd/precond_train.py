@@ -13,7 +13,7 @@ the exact correction (hci_mlp_collect).  The network is trained on the pooled ro
 the library's BF16-MFMA kernels and saved as an .npz (parameters + feature
 standardisation).
 
-    python -m risq_amd.precond_train [out.npz]
+    python -m risq_amd.precond_train [out.npz] [--recipe small]
 """
 from __future__ import annotations
 
@@ -28,6 +28,8 @@ from .fcidump import synthetic_molecular
 from .hci import HciContext
 
 DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "precond_mlp.npz")
+# trained on the small systems only (n_orb 20-28, 4-5 electrons per spin)
+SMALL_RECIPE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "precond_mlp_small.npz")
 
 # (n_orb, electrons per spin, determinants, seed)
 TRAINING_SYSTEMS = [(20, 4, 30000, 100), (22, 5, 30000, 101), (24, 4, 40000, 102),
@@ -72,11 +74,18 @@ def load(ctx: HciContext, path: str = DEFAULT_PATH) -> None:
 
 
 def main(argv=None) -> int:
-    argv = sys.argv[1:] if argv is None else argv
-    path = argv[0] if argv else DEFAULT_PATH
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("out", nargs="?", default=None)
+    ap.add_argument("--recipe", choices=["all", "small"], default="all",
+                    help="small: only the n_orb 20-28, 8-10 electron systems (the recipe of the learned-"
+                         "preconditioner workload); all: those plus four n_orb-40 spaces")
+    args = ap.parse_args(sys.argv[1:] if argv is None else argv)
+    path = args.out or (SMALL_RECIPE_PATH if args.recipe == "small" else DEFAULT_PATH)
+    systems = [s for s in TRAINING_SYSTEMS if s[0] <= 28] if args.recipe == "small" else TRAINING_SYSTEMS
     ctx = HciContext(0)
     t = time.time()
-    X, m, y = collect(ctx)
+    X, m, y = collect(ctx, systems)
     print(f"collected {X.shape[0]} rows in {time.time() - t:.1f}s", flush=True)
     st = train(ctx, X, m, y)
     print(f"trained: loss {st['first_epoch_loss']:.4f} -> {st['last_epoch_loss']:.4f} in {st['seconds']:.1f}s",

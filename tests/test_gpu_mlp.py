@@ -154,3 +154,37 @@ def test_mlp_requires_a_network():
     with pytest.raises(HciError, match="no network"):
         ctx.davidson(h, precond=PRECOND_MLP)
     ctx.close()
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("system", ["held_out_small", "n_orb40_prefix"])
+def test_small_recipe_network(gpu_ctx, system):
+    """The network trained only on the small recipe (n_orb 20-28, 4-5 electrons
+    per spin; risq_amd/precond_mlp_small.npz): on a held-out system of that
+    recipe it saves at least 10% of the iterations; carried over to the n_orb 40
+    workload's prefix it keeps the answer (its iteration count is printed: it
+    does not transfer, see the committed network trained with n_orb 40 spaces
+    too, which gains there)."""
+    from risq_amd import precond_train
+    from risq_amd.dets import energy_truncated_space
+    from risq_amd.fcidump import synthetic_molecular
+    from risq_amd.hci import PREC_F32, STORE_TILED
+    precond_train.load(gpu_ctx, precond_train.SMALL_RECIPE_PATH)
+    if system == "held_out_small":
+        ints = synthetic_molecular(24, 48, 110)
+        eps = np.array([ints.h1(p, p) for p in range(24)])
+        dets = energy_truncated_space(eps, 4, 4, 40000)
+    else:
+        ints, dets = mol_space(40, 80, 1, 6, 6, 100000)
+    gpu_ctx.set_integrals(ints)
+    h = gpu_ctx.build_h(dets, gpu_ctx.ham_diag(dets), value_prec=PREC_F32, storage=STORE_TILED)
+    kw = dict(tol=1e-4, etol=1e-8, max_iters=600, mixed=True, restart_keep=2)
+    dpr = gpu_ctx.davidson(h, precond=PRECOND_DPR, **kw)
+    mlp = gpu_ctx.davidson(h, precond=PRECOND_MLP, **kw)
+    print(f"{system}: diagonal {dpr.stats['iters']} iterations, small-recipe network {mlp.stats['iters']} "
+          f"({mlp.stats['mlp_applied']} network steps)")
+    assert dpr.converged and mlp.converged and abs(mlp.energy - dpr.energy) < 1e-6
+    if system == "held_out_small":
+        assert mlp.stats["iters"] <= 0.9 * dpr.stats["iters"]
+    h.free()
+    precond_train.load(gpu_ctx)
