@@ -196,6 +196,10 @@ int hci_csr_row(const hci_csr* h, int64_t row, int32_t* cols, double* vals, int6
  * widened to double: hi+lo for HCI_PREC_F32), diag[n]; any may be NULL */
 int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* val,
                      double* diag);
+/* MatrixOperations::set_diagonal (src/var/eigenvalues/matrix_operations.rs:14-27):
+ * replaces the stored diagonal with diag[n] (caller order).  The diagonal is
+ * read back with hci_csr_download, nrows = ncols = n with hci_csr_info. */
+int hci_csr_set_diagonal(hci_csr* h, const double* diag, int mem);
 void hci_csr_free(hci_csr* h);
 /* Frees h like hci_csr_free but keeps an HCI_STORE_TILED matrix's entry arrays
  * in its context for the next build there (device memory freed and allocated

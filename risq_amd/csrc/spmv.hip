@@ -639,3 +639,22 @@ extern "C" int hci_sigma_noalloc(const hci_csr* h, int ncol, const double* x, do
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return HCI_OK;
 }
+
+// MatrixOperations::set_diagonal (src/var/eigenvalues/matrix_operations.rs:14-27):
+// the caller-order diagonal replaces the matrix's
+extern "C" int hci_csr_set_diagonal(hci_csr* h, const double* diag, int mem) {
+    if (!h) return HCI_ERR_ARG;
+    hci_ctx* c = h->ctx;
+    if (!diag) return hci_fail(c, HCI_ERR_ARG, "hci_csr_set_diagonal: null diagonal");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const double* src = diag;
+    if (mem == HCI_MEM_HOST) {
+        if (int rc = work_reserve(c, (size_t)h->n * 8 + 256, 256)) return rc;
+        HIP_TRY(c, hipMemcpyAsync(c->work_dev, diag, (size_t)h->n * 8, hipMemcpyHostToDevice, c->stream));
+        src = (const double*)c->work_dev;
+    }
+    permute_to_internal(h, src, h->d_diag);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HCI_OK;
+}

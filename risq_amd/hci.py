@@ -106,6 +106,7 @@ SIGNATURES = {
     "hci_csr_recycle": (C.c_int, [_P]),
     "hci_sigma": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.c_int]),
     "hci_sigma_noalloc": (C.c_int, [_P, C.c_int, _P, _P, C.c_int]),
+    "hci_csr_set_diagonal": (C.c_int, [_P, _P, C.c_int]),
     "hci_dav_default_opts": (None, [C.POINTER(DavOpts)]),
     "hci_davidson": (C.c_int, [_P, _P, _P, C.POINTER(DavOpts), C.POINTER(C.c_double), _P,
                                C.POINTER(DavStats)]),
@@ -536,6 +537,15 @@ class SparseMatUpperTri:
         vals = np.empty(max(cnt.value, 1))
         self.ctx._check(self.ctx.lib.hci_csr_row(self.handle, i, _ptr(cols), _ptr(vals), cnt.value, C.byref(cnt)))
         return cols[:cnt.value], vals[:cnt.value]
+
+    def set_diagonal(self, diag: np.ndarray) -> None:
+        """MatrixOperations::set_diagonal (src/var/eigenvalues/matrix_operations.rs:14-27)."""
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        assert diag.shape == (self.n,)
+        self.ctx._check(self.ctx.lib.hci_csr_set_diagonal(self.handle, _ptr(diag), MEM_HOST))
+
+    def diagonal(self) -> np.ndarray:
+        return self.download()[3]
 
     def matrix_vector_prod(self, x: np.ndarray, mode: int = SIGMA_FULL) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64)

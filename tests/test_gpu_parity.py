@@ -316,6 +316,28 @@ def test_mixed_floor_hands_over_to_refinement(gpu_ctx, oracle_lib):
     assert abs(r.energy - ref) < 1e-10, (r.energy, ref)
 
 
+@pytest.mark.parametrize("storage", [STORE_FULL, STORE_UPPER, STORE_TILED])
+def test_set_diagonal(gpu_ctx, oracle_lib, storage):
+    """MatrixOperations::set_diagonal: the new caller-order diagonal reads back
+    unchanged, and the product and Davidson use it (against the oracle's matrix
+    with the same diagonal)."""
+    ints, dets = mol_space(*MOL0)
+    gpu_ctx.set_integrals(ints)
+    diag = oracle_lib.OracleHam(ints).ham_diag(dets)
+    h = gpu_ctx.build_h(dets, diag, storage=storage)
+    shifted = diag + np.random.default_rng(13).uniform(0.0, 0.5, diag.size)
+    h.set_diagonal(shifted)
+    assert np.array_equal(h.diagonal(), shifted)
+    S = _oracle_build(oracle_lib, ints, dets, shifted)
+    x = np.random.default_rng(15).standard_normal(dets.shape[0])
+    ref = S.spmv(x)
+    assert np.max(np.abs(h.matrix_vector_prod(x) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    e_ref = np.linalg.eigvalsh(csr_to_dense(*S.export()))[0]
+    r = gpu_ctx.davidson(h, tol=1e-6, etol=1e-10, max_iters=400)
+    assert r.converged and abs(r.energy - e_ref) < 1e-6
+    h.free()
+
+
 def test_full_ci_tiny(gpu_ctx, oracle_lib):
     """Full CI of a 6-orbital system: every determinant, dense eigh reference."""
     from risq_amd.fcidump import synthetic_molecular
