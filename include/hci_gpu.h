@@ -78,7 +78,11 @@ extern "C" {
                              rounding).  An incremental build rebuilds the whole
                              tiled matrix; pairs among the previous
                              determinants keep the single rule they were stored
-                             under.  No hci_csr_row. */
+                             under.  Without FP16 copies, a dense window (at
+                             least 2048 entries) whose rows and columns are two
+                             tiles' whole row ranges is stored once for both
+                             triangles (its transposed sums go through
+                             per-window partial sums).  No hci_csr_row. */
 
 /* which singles are stored (doubles are always stored iff H != 0,
  * src/var/off_diag.rs:33) */
@@ -110,6 +114,7 @@ typedef struct hci_build_stats {
     int64_t n_beta;         /* unique beta strings */
     int64_t alpha_pairs;    /* single-connected alpha-string pairs (ordered) */
     double seconds;         /* wall time of the call (device work included) */
+    int64_t mirror_pairs;   /* HCI_STORE_TILED: dense window pairs stored as one half */
 } hci_build_stats;
 
 #define HCI_DAV_MAX_SUB 32

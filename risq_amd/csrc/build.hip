@@ -297,7 +297,8 @@ __global__ void __launch_bounds__(kBlock)
 template <class S, bool FILL>
 __global__ void __launch_bounds__(kBlock)
     k_tile_connect(const SubBlock* __restrict__ sbs, int64_t n_sb, BuildView<S> B, const S* ub, DevInts T,
-                   SingleRules rules, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out) {
+                   SingleRules rules, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out,
+                   const int8_t* __restrict__ pairf) {
     __shared__ int32_t s_cur[kWaves][kWinPerSb];
     __shared__ int64_t s_off[kWaves][kWinPerSb];
     const int lane = threadIdx.x & 63;
@@ -313,11 +314,16 @@ __global__ void __launch_bounds__(kBlock)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    unsigned long long nz = 0;  // FILL: nonzero entries written
+    unsigned long long nz = 0;  // FILL: nonzero entries written (pairs of a stored mirror half count twice)
+    // FILL: a window whose mirror holds its entries is not written
+    auto dropped = [&](int32_t cl) { return FILL && pairf && pairf[sb.win_begin + cl / kWinCols] == 2; };
     auto emit = [&](bool keep, int32_t rl, int32_t cl, double v) {
+        int8_t f = 0;
+        if (FILL && pairf && keep) f = pairf[sb.win_begin + cl / kWinCols];
+        keep = keep && f != 2;
         const uint64_t mk = __ballot(keep);
         if (!mk) return;
-        if (FILL) nz += __popcll(__ballot(keep && v != 0.0));
+        if (FILL) nz += __popcll(__ballot(keep && v != 0.0)) + __popcll(__ballot(keep && v != 0.0 && f == 1));
         const int slice = cl / kWinCols;
         const int s0 = __shfl(slice, (int)__builtin_ctzll(mk));
         const uint64_t ms = __ballot(keep && slice == s0);
@@ -407,7 +413,7 @@ __global__ void __launch_bounds__(kBlock)
                         const bool same = (da == db);
                         if (sb.kind == 3) {
                             // an equal beta string matches every key: count it once
-                            keep = same ? kk == 0 : dbl != 0;
+                            keep = (same ? kk == 0 : dbl != 0) && !dropped(cl);
                             if (FILL && keep) {
                                 S u1 = other_row, u2 = ua_col;
                                 swap_if(j < i, u1, u2);
@@ -417,7 +423,7 @@ __global__ void __launch_bounds__(kBlock)
                             }
                         } else {
                             // strings are unique within a group: equal only on the diagonal
-                            keep = !same;
+                            keep = !same && !dropped(cl);
                             if (FILL && keep) {
                                 swap_if(j < i, da, db);
                                 v = single_or_zero(elem_single<S>(T, da, db, other_row), rule_of(rules, max(i, j)));
@@ -480,10 +486,10 @@ __global__ void __launch_bounds__(kBlock)
         // visited in order: the running count of the current window is a
         // wave-uniform register (no LDS cursor), stored when its last column
         // is done
-        auto emit_col = [&](bool keep, int32_t rl, int32_t cl, double v, int32_t& run, int64_t base) {
+        auto emit_col = [&](bool keep, int32_t rl, int32_t cl, double v, int32_t& run, int64_t base, int wgt) {
             const uint64_t mk = __ballot(keep);
             if (FILL) {
-                nz += __popcll(__ballot(keep && v != 0.0));
+                nz += (unsigned long long)wgt * __popcll(__ballot(keep && v != 0.0));
                 if (keep)
                     put16(out, base + run + __popcll(mk & lt_mask), ((uint32_t)(rl + sb.rb) << 8) | (uint32_t)(cl % kWinCols),
                           v);
@@ -505,6 +511,9 @@ __global__ void __launch_bounds__(kBlock)
         }
         for (int32_t w0 = 0; w0 < sb.nc; w0 += kWinCols) {
             const int slice = w0 / kWinCols;
+            const int8_t f = (FILL && pairf) ? pairf[sb.win_begin + slice] : 0;
+            if (f == 2) continue;  // the mirror window holds these entries
+            const int wgt = f == 1 ? 2 : 1;
             const int64_t base = FILL ? out.woff[sb.win_begin + slice] : 0;
             int32_t run = 0;
             const int32_t w1 = min(sb.nc, w0 + kWinCols);
@@ -518,7 +527,7 @@ __global__ void __launch_bounds__(kBlock)
                     bool keep;
                     double v;
                     pair_val(lane + 64 * k < sb.nr, rs[k], ri[k], b, j, keep, v);
-                    emit_col(keep, lane + 64 * k, cl, v, run, base);
+                    emit_col(keep, lane + 64 * k, cl, v, run, base, wgt);
                 }
             }
             if (!FILL && lane == 0) win_cnt[sb.win_begin + slice] = run;
@@ -537,7 +546,7 @@ __global__ void __launch_bounds__(kBlock)
         }
         for (int32_t cb = 0; cb < sb.nc; cb += per) {
             const int32_t cl = cb + co;
-            const bool ok = lane_ok && cl < sb.nc;
+            const bool ok = lane_ok && cl < sb.nc && !dropped(cl);
             S b = 0;
             int32_t j = 0;
             if (ok) {
@@ -575,14 +584,121 @@ __global__ void k_win_sizes(const int64_t* wc, int64_t n_win, int64_t* sz, int32
 }
 
 __global__ void k_win_compact(const int64_t* wc, const int64_t* woff, const int32_t* nidx, const int32_t* wc0,
-                              const int32_t* wnc, int64_t n_win, Window* out) {
+                              const int32_t* wnc, int64_t n_win, const int8_t* pairf, const int32_t* pslot,
+                              Window* out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_win || wc[k] == 0) return;
     Window w;
-    w.off = woff[k];
+    w.off_sp = woff[k];
+    if (pairf && pairf[k] == 1) w.off_sp |= (int64_t)(pslot[k] + 1) << 40;
     w.c0 = wc0[k];
     w.cnt_nc = (uint32_t)wc[k] | ((uint32_t)wnc[k] << 20);
     out[nidx[k]] = w;
+}
+
+// ---- mirror pairs ---------------------------------------------------------------
+
+// row range and part of every window (the tile that holds it); one wave per tile
+constexpr int kPairWaves = 4;
+__global__ void __launch_bounds__(64 * kPairWaves)
+    k_win_rows(const TileTask* __restrict__ tasks, int64_t n_task, int8_t part, int32_t* wr0, int32_t* wnr,
+               int8_t* wpart) {
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * kPairWaves + (threadIdx.x >> 6);
+    if (t >= n_task) return;
+    const TileTask T = tasks[t];
+    for (int32_t k = T.w_begin + lane; k < T.w_end; k += 64) {
+        wr0[k] = T.r0;
+        wnr[k] = T.nr;
+        wpart[k] = part;
+    }
+}
+
+// key (part, first row, first column) of every window dense enough to pair
+__global__ void k_mirror_keys(const int64_t* wc, const int32_t* wc0, const int32_t* wr0, const int8_t* wpart,
+                              int64_t n, uint64_t* key, int32_t* idx) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    key[k] = wc[k] >= kPairMinEntries
+                 ? ((uint64_t)wpart[k] << 62) | ((uint64_t)(uint32_t)wr0[k] << 31) | (uint64_t)(uint32_t)wc0[k]
+                 : ~0ull;
+    idx[k] = (int32_t)k;
+}
+
+// A dense window (rows: one tile's range R, columns C) is paired with the
+// window whose rows are C and whose columns are R, when both are the only
+// windows with their keys and their sizes and counts agree; the half above
+// the diagonal (R before C) is kept (flag 1), the other dropped (flag 2).
+__global__ void k_mirror_match(const uint64_t* __restrict__ skey, const int32_t* __restrict__ sidx, int64_t n,
+                               const int64_t* wc, const int32_t* wc0, const int32_t* wnc, const int32_t* wr0,
+                               const int32_t* wnr, int8_t* pairf) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || skey[i] == ~0ull) return;
+    const uint64_t key = skey[i];
+    if ((i > 0 && skey[i - 1] == key) || (i + 1 < n && skey[i + 1] == key)) return;
+    const int32_t k = sidx[i];
+    const int32_t R0 = wr0[k], C0 = wc0[k];
+    if (R0 == C0) return;  // the diagonal block pairs with itself
+    const uint64_t target = (key >> 62 << 62) | ((uint64_t)(uint32_t)C0 << 31) | (uint64_t)(uint32_t)R0;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (skey[mid] < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo >= n || skey[lo] != target) return;
+    if (lo + 1 < n && skey[lo + 1] == target) return;
+    const int32_t k2 = sidx[lo];
+    if (wnc[k2] != wnr[k] || wnc[k] != wnr[k2] || wc[k2] != wc[k]) return;
+    pairf[k] = R0 < C0 ? 1 : 2;
+}
+
+// dropped windows hold no entries; kept ones are numbered (flag0[k] = 1)
+__global__ void k_mirror_drop(const int8_t* pairf, int64_t n, int64_t* wc, int32_t* kept) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    const int8_t f = k < n ? pairf[k] : 0;
+    if (f == 2) wc[k] = 0;
+    kept[k] = f == 1 ? 1 : 0;
+}
+
+// receiving key of each kept window's slot: the tile whose rows are its
+// columns (part, first row, rows)
+__global__ void k_mirror_recv(const int8_t* pairf, const int32_t* pslot, const int32_t* wc0, const int32_t* wnc,
+                              const int8_t* wpart, int64_t n, uint64_t* rkey, int32_t* rslot) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || pairf[k] != 1) return;
+    const int32_t sl = pslot[k];
+    rkey[sl] = ((uint64_t)wpart[k] << 62) | ((uint64_t)(uint32_t)wc0[k] << 31) | (uint64_t)(uint32_t)wnc[k];
+    rslot[sl] = sl;
+}
+
+__global__ void k_mirror_heads(const uint64_t* rkey, int64_t n, int32_t* head) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    head[i] = i < n && (i == 0 || rkey[i] != rkey[i - 1]) ? 1 : 0;
+}
+
+__global__ void k_mirror_segs(const uint64_t* rkey, const int32_t* head, const int32_t* hpos, int64_t n,
+                              PairSeg* seg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !head[i]) return;
+    const int32_t g = hpos[i];
+    PairSeg p;
+    p.part = (int32_t)(rkey[i] >> 62);
+    p.r0 = (int32_t)((rkey[i] >> 31) & 0x7FFFFFFFull);
+    p.nr = (int32_t)(rkey[i] & 0x7FFFFFFFull);
+    p.s_begin = (int32_t)i;
+    p.s_end = (int32_t)n;
+    seg[g] = p;
+}
+
+// each segment ends where the next begins (the last at n, set above)
+__global__ void k_mirror_seg_end(PairSeg* seg, int32_t n_seg) {
+    const int32_t g = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g + 1 < n_seg) seg[g].s_end = seg[g + 1].s_begin;
 }
 
 // In place, one wavefront per window: every 512-entry piece (the last one
@@ -621,14 +737,14 @@ __global__ void __launch_bounds__(64 * kIlWaves)
     const Window W = wins[k];
     const int cnt = W.cnt();
     if (cnt <= kWinAlign) {  // one group: its slots are its positions
-        if (!F64 && h16 && lane < cnt) h16[W.off + lane] = fp16_of(hi[W.off + lane], lo[W.off + lane]);
+        if (!F64 && h16 && lane < cnt) h16[W.o() + lane] = fp16_of(hi[W.o() + lane], lo[W.o() + lane]);
         continue;
     }
     for (int base = 0; base < cnt; base += kWinChunk) {
         const int n = min(kWinChunk, cnt - base);
         // fill order in: lane l takes entries 8 l .. 8 l + 7 (slots up to a
         // multiple of 8 past the last entry belong to the window)
-        const int64_t e0 = W.off + base + 8 * lane;
+        const int64_t e0 = W.o() + base + 8 * lane;
         if (8 * lane < n) {
             *(il_u16x8*)(si + lp) = *(const il_u16x8*)(idx + e0);
             if constexpr (F64) {
@@ -647,8 +763,13 @@ __global__ void __launch_bounds__(64 * kIlWaves)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // slot order out: lane l writes slots 8 l .. 8 l + 7, which hold
         // entries 64 q + l of a full piece, or l + q R of the last one (slots
-        // past its entries are written as zeros)
-        const bool full = n == kWinChunk;
+        // past its entries are written as zeros).  The stored half of a
+        // mirror pair keeps lane l's group as fill-order entries 8 l .. 8 l + 7,
+        // rotated by l (slot 8 l + q holds entry 8 l + (q + l) % 8): runs of
+        // one column stay mostly together for the product's transposed sums,
+        // and neighbouring lanes, often in neighbouring columns with similar
+        // rows, do not reach the same row in the same step.
+        const bool full = n == kWinChunk, fill_order = W.sp() >= 0;
         const int R = (n + kWinAlign - 1) / kWinAlign;
         if (full || lane < R) {
             il_u16x8 ui, uh;
@@ -656,7 +777,7 @@ __global__ void __launch_bounds__(64 * kIlWaves)
             il_f64x2 d2[4];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int j = full ? q * 64 + lane : lane + q * R;
+                const int j = fill_order ? 8 * lane + ((q + lane) & 7) : full ? q * 64 + lane : lane + q * R;
                 const bool ok = j < n;
                 const int pj = j + ((j >> 6) << 3);
                 ui[q] = ok ? si[pj] : (unsigned short)0;
@@ -669,7 +790,7 @@ __global__ void __launch_bounds__(64 * kIlWaves)
                     uh[q] = fp16_of(hq, lq);
                 }
             }
-            const int64_t e = W.off + base + 8 * lane;
+            const int64_t e = W.o() + base + 8 * lane;
             *(il_u16x8*)(idx + e) = ui;
             if constexpr (F64) {
 #pragma unroll
@@ -1389,7 +1510,96 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (n_sb > 0) {
         hipLaunchKernelGGL(k_win_init, grid_for(n_sb, 256), 256, 0, s, d_sb, n_sb, d_wc0, d_wnc);
         hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           rules, dbl, d_wc, none, nullptr);
+                           rules, dbl, d_wc, none, nullptr, nullptr);
+    }
+    // mirror pairs: dense windows whose mirror is another tile's window; the
+    // kept halves are numbered (their partial-sum slots) and grouped by the
+    // tile that receives their transposed sums.  Not with FP16 copies: the
+    // FP16 product is bound by its LDS row atomics rather than its bytes, and
+    // the pairs' extra LDS work made it slower (3.43-3.58 ms against 3.17 ms
+    // per product on the 2e6-determinant workload)
+    int8_t* d_pairf = nullptr;
+    int32_t* d_pslot = nullptr;
+    if (n_wall > 0 && n_sb > 0 && o.value_prec != HCI_PREC_F16) {
+        int32_t *d_wr0 = nullptr, *d_wnr = nullptr, *d_mi = nullptr, *d_mi2 = nullptr, *d_kept = nullptr;
+        int8_t* d_wpart = nullptr;
+        uint64_t *d_mk = nullptr, *d_mk2 = nullptr;
+        HIP_TRY(c, wb.alloc(&d_pairf, (size_t)n_wall + 1));
+        HIP_TRY(c, wb.alloc(&d_pslot, (size_t)n_wall + 1));
+        HIP_TRY(c, wb.alloc(&d_kept, (size_t)n_wall + 1));
+        HIP_TRY(c, wb.alloc(&d_wr0, (size_t)n_wall));
+        HIP_TRY(c, wb.alloc(&d_wnr, (size_t)n_wall));
+        HIP_TRY(c, wb.alloc(&d_wpart, (size_t)n_wall));
+        HIP_TRY(c, wb.alloc(&d_mk, (size_t)n_wall));
+        HIP_TRY(c, wb.alloc(&d_mk2, (size_t)n_wall));
+        HIP_TRY(c, wb.alloc(&d_mi, (size_t)n_wall));
+        HIP_TRY(c, wb.alloc(&d_mi2, (size_t)n_wall));
+        HIP_TRY(c, hipMemsetAsync(d_pairf, 0, (size_t)n_wall + 1, s));
+        HIP_TRY(c, hipMemsetAsync(d_wr0, 0, (size_t)n_wall * 4, s));
+        HIP_TRY(c, hipMemsetAsync(d_wnr, 0, (size_t)n_wall * 4, s));
+        HIP_TRY(c, hipMemsetAsync(d_wpart, 0, (size_t)n_wall, s));
+        if (!ta.empty())
+            hipLaunchKernelGGL(k_win_rows, grid_for((int64_t)ta.size(), kPairWaves), 64 * kPairWaves, 0, s,
+                               H->d_task_a, (int64_t)ta.size(), (int8_t)0, d_wr0, d_wnr, d_wpart);
+        if (!tb.empty())
+            hipLaunchKernelGGL(k_win_rows, grid_for((int64_t)tb.size(), kPairWaves), 64 * kPairWaves, 0, s,
+                               H->d_task_b, (int64_t)tb.size(), (int8_t)1, d_wr0, d_wnr, d_wpart);
+        hipLaunchKernelGGL(k_mirror_keys, grid_for(n_wall, 256), 256, 0, s, d_wc, d_wc0, d_wr0, d_wpart, n_wall,
+                           d_mk, d_mi);
+        size_t need = 0;
+        HIP_TRY(c, rocprim::radix_sort_pairs(nullptr, need, d_mk, d_mk2, d_mi, d_mi2, (size_t)n_wall, 0, 64, s));
+        HIP_TRY(c, tmp.ensure(need));
+        HIP_TRY(c, rocprim::radix_sort_pairs(tmp.p, need, d_mk, d_mk2, d_mi, d_mi2, (size_t)n_wall, 0, 64, s));
+        hipLaunchKernelGGL(k_mirror_match, grid_for(n_wall, 256), 256, 0, s, d_mk2, d_mi2, n_wall, d_wc, d_wc0, d_wnc,
+                           d_wr0, d_wnr, d_pairf);
+        hipLaunchKernelGGL(k_mirror_drop, grid_for(n_wall + 1, 256), 256, 0, s, d_pairf, n_wall, d_wc, d_kept);
+        HIP_TRY(c, exclusive_scan(tmp, d_kept, d_pslot, (size_t)n_wall + 1, s));
+        int32_t n_pair = 0;
+        HIP_TRY(c, hipMemcpyAsync(&n_pair, d_pslot + n_wall, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        if (n_pair >= (1 << 24) - 1) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many mirror window pairs");
+        H->n_pair = n_pair;
+        if (n_pair > 0) {
+            uint64_t *d_rk = nullptr, *d_rk2 = nullptr;
+            int32_t *d_rs = nullptr, *d_head = nullptr, *d_hpos = nullptr;
+            HIP_TRY(c, wb.alloc(&d_rk, (size_t)n_pair));
+            HIP_TRY(c, wb.alloc(&d_rk2, (size_t)n_pair));
+            HIP_TRY(c, wb.alloc(&d_rs, (size_t)n_pair));
+            HIP_TRY(c, wb.alloc(&d_head, (size_t)n_pair + 1));
+            HIP_TRY(c, wb.alloc(&d_hpos, (size_t)n_pair + 1));
+            HIP_TRY(c, hipMalloc(&H->d_pair_slot, (size_t)n_pair * 4 + 16));
+            hipLaunchKernelGGL(k_mirror_recv, grid_for(n_wall, 256), 256, 0, s, d_pairf, d_pslot, d_wc0, d_wnc,
+                               d_wpart, n_wall, d_rk, d_rs);
+            need = 0;
+            HIP_TRY(c, rocprim::radix_sort_pairs(nullptr, need, d_rk, d_rk2, d_rs, H->d_pair_slot, (size_t)n_pair, 0,
+                                                 64, s));
+            HIP_TRY(c, tmp.ensure(need));
+            HIP_TRY(c, rocprim::radix_sort_pairs(tmp.p, need, d_rk, d_rk2, d_rs, H->d_pair_slot, (size_t)n_pair, 0,
+                                                 64, s));
+            hipLaunchKernelGGL(k_mirror_heads, grid_for((int64_t)n_pair + 1, 256), 256, 0, s, d_rk2, (int64_t)n_pair,
+                               d_head);
+            HIP_TRY(c, exclusive_scan(tmp, d_head, d_hpos, (size_t)n_pair + 1, s));
+            int32_t n_seg = 0;
+            HIP_TRY(c, hipMemcpyAsync(&n_seg, d_hpos + n_pair, 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+            H->n_pair_seg = n_seg;
+            HIP_TRY(c, hipMalloc(&H->d_pair_seg, (size_t)n_seg * sizeof(PairSeg) + 16));
+            HIP_TRY(c, hipMalloc(&H->d_pair_part, (size_t)n_pair * kWinCols * 8 + 16));
+            hipLaunchKernelGGL(k_mirror_segs, grid_for(n_pair, 256), 256, 0, s, d_rk2, d_head, d_hpos,
+                               (int64_t)n_pair, H->d_pair_seg);
+            hipLaunchKernelGGL(k_mirror_seg_end, grid_for(n_seg, 256), 256, 0, s, H->d_pair_seg, n_seg);
+            // the segments are in key order, part first: count the alpha-major ones
+            std::vector<PairSeg> hs((size_t)n_seg);
+            HIP_TRY(c, hipMemcpyAsync(hs.data(), H->d_pair_seg, (size_t)n_seg * sizeof(PairSeg),
+                                      hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+            int64_t n0 = 0;
+            while (n0 < n_seg && hs[(size_t)n0].part == 0) ++n0;
+            for (int64_t g = n0; g < n_seg; ++g)
+                if (hs[(size_t)g].part == 0) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: mirror segments out of order");
+            H->n_pair_seg0 = n0;
+        }
+        HIP_TRY(c, hipGetLastError());
     }
     // window slots, offsets, the non-empty windows' new indices and the entry total
     hipLaunchKernelGGL(k_win_sizes, grid_for(n_wall + 1, 256), 256, 0, s, d_wc, n_wall, d_sz, d_flag);
@@ -1424,9 +1634,9 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMalloc(&H->d_win, (size_t)n_win * sizeof(Window) + 16));
     if (n_sb > 0)
         hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           rules, dbl, nullptr, fo, (unsigned long long*)d_nnz);
+                           rules, dbl, nullptr, fo, (unsigned long long*)d_nnz, d_pairf);
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
-                       H->d_win);
+                       d_pairf, d_pslot, H->d_win);
     if (n_win > 0) {
         const unsigned ilg = std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u);
         if (H->d_val)
@@ -1684,6 +1894,7 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
             stats->n_alpha = Ua;
             stats->n_beta = Ub;
             stats->alpha_pairs = n_pairs;
+            stats->mirror_pairs = H->n_pair;
         }
         *out = H;
         return HCI_OK;
@@ -1806,6 +2017,7 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
         stats->n_alpha = Ua;
         stats->n_beta = Ub;
         stats->alpha_pairs = n_pairs;
+        stats->mirror_pairs = 0;
     }
     *out = H;
     return HCI_OK;

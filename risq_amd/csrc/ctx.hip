@@ -391,8 +391,13 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
                 const bool beta_major = (int64_t)t >= h->n_task_a;
                 for (int32_t k = tasks[t].w_begin; k < tasks[t].w_end; ++k) {
                     const Window& b = wins[(size_t)k];
+                    // the stored half of a mirror pair holds both triangles'
+                    // pairs, in (rotated) fill order
+                    const bool pair = b.sp() >= 0;
                     for (int64_t j = 0; j < b.cnt(); ++j) {
-                        const int64_t e = b.off + win_slot(j, b.cnt());
+                        // pair groups: slot 8 l + q of a chunk holds entry 8 l + (q + l) % 8
+                        const int64_t e =
+                            b.o() + (pair ? (j & ~7ll) + (((j & 7) - ((j >> 3) & 63)) & 7) : win_slot(j, b.cnt()));
                         const uint32_t u = (uint32_t)ic[(size_t)e];
                         int32_t p = tasks[t].r0 + (int32_t)(u >> 8), q = b.c0 + (int32_t)(u & 0xFFu);
                         if (beta_major) {
@@ -400,7 +405,11 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
                             q = bpos[q];
                         }
                         const int32_t i = perm[p], j2 = perm[q];
-                        if (i < j2 && iv[(size_t)e] != 0.0) f(i, j2, iv[(size_t)e]);
+                        if (iv[(size_t)e] == 0.0) continue;
+                        if (pair)
+                            f(std::min(i, j2), std::max(i, j2), iv[(size_t)e]);
+                        else if (i < j2)
+                            f(i, j2, iv[(size_t)e]);
                     }
                 }
             }
@@ -521,6 +530,9 @@ void hci_csr_free(hci_csr* h) {
     hipFree(h->d_chunk_row);
     hipFree(h->d_item_row);
     hipFree(h->d_rownorm);
+    hipFree(h->d_pair_part);
+    hipFree(h->d_pair_slot);
+    hipFree(h->d_pair_seg);
     hipFree(h->d_perm);
     hipFree(h->d_pos);
     hipFree(h->d_words);

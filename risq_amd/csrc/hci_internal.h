@@ -43,11 +43,30 @@ struct SubBlock {
     int32_t rb;               // r0 - the first row of its tile (tiles pack small groups)
 };
 struct Window {
-    int64_t off;      // first slot (multiple of kWinAlign)
+    int64_t off_sp;   // first slot (multiple of kWinAlign) | (pair slot + 1) << 40
     int32_t c0;       // first column, in the part's order
     uint32_t cnt_nc;  // entries | columns << 20
     __host__ __device__ int cnt() const { return (int)(cnt_nc & 0xFFFFFu); }
     __host__ __device__ int nc() const { return (int)(cnt_nc >> 20); }
+    __host__ __device__ int64_t o() const { return off_sp & ((1ll << 40) - 1); }
+    // the window's slot among the stored halves of mirror pairs, -1 if none
+    __host__ __device__ int32_t sp() const { return (int32_t)(off_sp >> 40) - 1; }
+};
+// Mirror pairs.  A dense window whose rows are one tile's whole row range and
+// whose columns are another tile's whole row range has a mirror (the same
+// entries transposed) in that other tile.  Only the half above the diagonal
+// block is stored; its product also forms the transposed sums (one per
+// column), which go to a partial-sum buffer, slot sp x kWinCols, and are added
+// into the receiving tile's rows by a reduction after the product.  Windows
+// with at least kPairMinEntries entries are paired (builds without FP16
+// copies).  The stored half keeps its entries in fill order, each lane's
+// group rotated by the lane (slot 8 l + q of a chunk holds entry
+// 8 l + (q + l) % 8, not win_slot's): a group is 8 consecutive entries.
+constexpr int kPairMinEntries = 2048;
+// receiving tile of a run of slots: rows [r0, r0 + nr) of part `part`
+// receive the slots' partial sums, slots s_begin .. s_end of the sorted list
+struct PairSeg {
+    int32_t part, r0, nr, s_begin, s_end;
 };
 constexpr int kWinChunk = 64 * kWinAlign;
 __host__ __device__ inline int64_t win_slot(int64_t j, int64_t cnt) {
@@ -168,6 +187,13 @@ struct hci_csr {
     double* d_xb = nullptr;        // x in beta-major order
     float* d_x32 = nullptr;        // FP32 copies of x for FP16-value products (HCI_PREC_F16)
     float* d_xb32 = nullptr;
+    // mirror pairs (TILED): partial sums (n_pair x kWinCols doubles), the
+    // slots in receiving order and the receiving segments (the alpha-major
+    // part's first: n_pair_seg0 of them)
+    double* d_pair_part = nullptr;
+    int32_t* d_pair_slot = nullptr;
+    PairSeg* d_pair_seg = nullptr;
+    int64_t n_pair = 0, n_pair_seg = 0, n_pair_seg0 = 0;
     // single-storage rule segments of the pairs (see SingleRules)
     std::vector<std::pair<int64_t, int>> rule_segs;
 };

@@ -253,10 +253,13 @@ struct Oct {
 };
 
 // x as the product reads it: FP64, or FP32 with FP16 values (MODE 3), whose
-// own rounding is 2^-11
+// own rounding is 2^-11.  The LDS accumulators stay FP64: ds_add_f64 runs
+// the FP16-value product at 3.6 ms on the 2e6-determinant workload, ds_add_f32
+// at 18.8 ms.
 template <int MODE>
 using XT = typename std::conditional<MODE == 3, float, double>::type;
 
+// H_pq x_q into row p's accumulator
 template <bool ABS, class X>
 __device__ __forceinline__ void add_entry(double* acc, const X* xw, uint32_t u, double v) {
     if (ABS)
@@ -265,20 +268,58 @@ __device__ __forceinline__ void add_entry(double* acc, const X* xw, uint32_t u, 
         atomicAdd(&acc[u >> 8], v * (double)xw[u & 0xFFu]);
 }
 
-template <int MODE, bool ABS>
+// A group of the stored half of a mirror pair (8 entries in fill order,
+// rotated, so columns come mostly in runs; slots past the window's entries
+// hold zeros): every entry goes into its row's accumulator, and H_pq x_p into
+// column q's transposed sum, added once per run of a column.  The x values of
+// four entries are read together before their adds.
+template <bool ABS, class O, class X>
+__device__ __forceinline__ void add_pair_group(double* acc, const X* xw, const X* xr, double* tw, const O& o) {
+    double run = 0.0;
+#pragma unroll
+    for (int h = 0; h < 8; h += 4) {
+        X xc[4], xp[4];
+        if (!ABS) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                xc[q] = xw[o.u[h + q] & 0xFFu];
+                xp[q] = xr[o.u[h + q] >> 8];
+            }
+        }
+#pragma unroll
+        for (int q = h; q < h + 4; ++q) {
+            const uint32_t u = o.u[q];
+            const double v = o.value(q);
+            atomicAdd(&acc[u >> 8], ABS ? fabs(v) : v * (double)xc[q - h]);
+            run += ABS ? fabs(v) : v * (double)xp[q - h];
+            const bool end = q == 7 || ((u ^ o.u[q + 1 < 8 ? q + 1 : q]) & 0xFFu) != 0;
+            if (end) atomicAdd(&tw[u & 0xFFu], run);
+            run = end ? 0.0 : run;
+        }
+    }
+}
+
+// PAIRS: the matrix has mirror pairs (their stored halves' code and LDS are
+// left out of the other instantiations, which keep their occupancy)
+template <int MODE, bool ABS, bool PAIRS>
 __global__ void __launch_bounds__(kTileThreads)
     k_spmv_win(const TileTask* __restrict__ tasks, const Window* __restrict__ wins,
                const uint16_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
                const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
-               double* __restrict__ y, const double* __restrict__ xdiag, const int32_t* __restrict__ ypos) {
+               double* __restrict__ y, const double* __restrict__ xdiag, const int32_t* __restrict__ ypos,
+               double* __restrict__ part) {
     __shared__ double acc[kTileRows];
     __shared__ XT<MODE> xs[kTileWaves][kWinCols];
+    __shared__ XT<MODE> xr[PAIRS ? kTileRows : 1];           // x of the tile's own rows (mirror pairs)
+    __shared__ double ts[kTileWaves][PAIRS ? kWinCols : 1];  // a wave's transposed sums (mirror pairs)
     __shared__ int next, next_small;
     const TileTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63;
     XT<MODE>* xw = xs[tid >> 6];
     const XT<MODE>* xx = (const XT<MODE>*)x;
     for (int r = tid; r < kTileRows; r += kTileThreads) acc[r] = 0.0;
+    if (PAIRS && !ABS)
+        for (int r = tid; r < t.nr; r += kTileThreads) xr[r] = xx[t.r0 + r];
     if (tid == 0) {
         next = 0;
         next_small = 0;
@@ -299,20 +340,27 @@ __global__ void __launch_bounds__(kTileThreads)
         Window Wn{};
         if (kn < n_w) Wn = wins[t.w_begin + kn];
         // full chunks: lane l's group of chunk c holds entries 512 c + 64 q + l;
-        // the tail of m entries: group g holds entries g, g + R, ..., g + 7R
+        // the tail of m entries: group g holds entries g, g + R, ..., g + 7R.
+        // The stored half of a mirror pair (sp >= 0) keeps fill order: lane
+        // l's group of chunk c holds entries 512 c + 8 l .. 512 c + 8 l + 7
+        // (the tail's groups too), at the same addresses.
         const int cnt = W.cnt(), nc = W.nc();
         const int n_full = cnt / kWinChunk;
         const int m = cnt - n_full * kWinChunk;
         const int R = (m + kWinAlign - 1) / kWinAlign;
-        const int64_t tail = W.off + (int64_t)n_full * kWinChunk;
-        const int64_t mine = W.off + (int64_t)kWinAlign * lane;
+        const int64_t tail = W.o() + (int64_t)n_full * kWinChunk;
+        const int64_t mine = W.o() + (int64_t)kWinAlign * lane;
+        const int32_t sp = PAIRS ? W.sp() : -1;
         // the first two groups' entries and the x slice are requested together
         Oct<MODE> o0{}, o1{};
         if (n_full > 0)
             o0.load(idx, val, hi, lo, mine);
         else if (lane < R)
             o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
-        if (n_full > 1) o1.load(idx, val, hi, lo, mine + kWinChunk);
+        if (n_full > 1)
+            o1.load(idx, val, hi, lo, mine + kWinChunk);
+        else if (sp >= 0 && n_full == 1 && lane < R)
+            o1.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
         if (!ABS) {
             XT<MODE> xv[kWinCols / 64];
 #pragma unroll
@@ -323,31 +371,61 @@ __global__ void __launch_bounds__(kTileThreads)
 #pragma unroll
             for (int j = 0; j < kWinCols / 64; ++j) xw[lane + 64 * j] = xv[j];
         }
+        // the stored half of a mirror pair also sums its transposed products
+        double* tw = ts[PAIRS ? tid >> 6 : 0];
+        if (sp >= 0) {
+#pragma unroll
+            for (int j = 0; j < kWinCols / 64; ++j) tw[lane + 64 * j] = 0.0;
+        }
         // the slice is written and read by this wave only: order its LDS
         // writes before the reads below
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        int c = 0;
-        for (; c + 1 < n_full; c += 2) {
+        if (sp < 0) {
+            int c = 0;
+            for (; c + 1 < n_full; c += 2) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+                for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
 #pragma unroll
-            for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o1.u[q], o1.value(q));
-            if (c + 2 < n_full) o0.load(idx, val, hi, lo, mine + (int64_t)(c + 2) * kWinChunk);
-            if (c + 3 < n_full) o1.load(idx, val, hi, lo, mine + (int64_t)(c + 3) * kWinChunk);
-        }
-        if (c < n_full) {  // one full chunk left, in o0 (odd count) ...
+                for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o1.u[q], o1.value(q));
+                if (c + 2 < n_full) o0.load(idx, val, hi, lo, mine + (int64_t)(c + 2) * kWinChunk);
+                if (c + 3 < n_full) o1.load(idx, val, hi, lo, mine + (int64_t)(c + 3) * kWinChunk);
+            }
+            if (c < n_full) {  // one full chunk left, in o0 (odd count) ...
 #pragma unroll
-            for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
-            if (lane < R) o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
-        } else if (n_full > 0 && lane < R) {  // ... or none: the tail is still to load
-            o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
-        }
-        if (lane < R) {
+                for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+                if (lane < R) o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+            } else if (n_full > 0 && lane < R) {  // ... or none: the tail is still to load
+                o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+            }
+            if (lane < R) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (lane + q * R < m) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+                for (int q = 0; q < 8; ++q)
+                    if (lane + q * R < m) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+            }
+        } else {
+            // chunks c and c + 1 in o0 and o1; the tail is chunk n_full, whose
+            // groups from lane R on hold no entries
+            const int n_grp = n_full + (m > 0 ? 1 : 0);
+            for (int c = 0; c < n_grp; c += 2) {
+                if (c < n_full || lane < R) add_pair_group<ABS>(acc, xw, xr, tw, o0);
+                if (c + 1 < n_full || (c + 1 == n_full && lane < R))
+                    add_pair_group<ABS>(acc, xw, xr, tw, o1);
+                if (c + 2 < n_full || (c + 2 == n_full && lane < R))
+                    o0.load(idx, val, hi, lo, mine + (int64_t)(c + 2) * kWinChunk);
+                if (c + 3 < n_full || (c + 3 == n_full && lane < R))
+                    o1.load(idx, val, hi, lo, mine + (int64_t)(c + 3) * kWinChunk);
+            }
+            // the transposed sums to the pair's slot, for the receiving tile
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int j = 0; j < kWinCols / 64; ++j) {
+                const int cc = lane + 64 * j;
+                if (cc < nc) part[(size_t)sp * kWinCols + cc] = tw[cc];
+            }
         }
         // reads of this slice precede the next window's staging
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -369,7 +447,7 @@ __global__ void __launch_bounds__(kTileThreads)
             const int R = (m + kWinAlign - 1) / kWinAlign;
             for (int g = 0; g < R; ++g) {
                 Oct<MODE> o;
-                o.load(idx, val, hi, lo, Ws.off + (int64_t)kWinAlign * g);
+                o.load(idx, val, hi, lo, Ws.o() + (int64_t)kWinAlign * g);
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                     if (g + q * R < m) {
@@ -390,6 +468,22 @@ __global__ void __launch_bounds__(kTileThreads)
             y[ypos[p]] += acc[r];
         else
             y[p] = (diag && !ABS) ? diag[p] * xdiag[p] + acc[r] : acc[r];  // xdiag: x in FP64
+    }
+}
+
+// The receiving side of the mirror pairs: one workgroup per receiving tile
+// adds, row by row, the transposed sums of the slots that target it (in slot
+// order) into y (beta-major tiles through ypos, to their alpha-major rows).
+// The tiles of one launch have disjoint rows; one launch per part.
+__global__ void __launch_bounds__(256)
+    k_pair_reduce(const PairSeg* __restrict__ segs, const int32_t* __restrict__ slots,
+                  const double* __restrict__ part, const int32_t* __restrict__ ypos, double* __restrict__ y) {
+    const PairSeg g = segs[blockIdx.x];
+    for (int r = threadIdx.x; r < g.nr; r += blockDim.x) {
+        double sum = 0.0;
+        for (int32_t k = g.s_begin; k < g.s_end; ++k) sum += part[(size_t)slots[k] * kWinCols + r];
+        const int64_t row = (int64_t)g.r0 + r;
+        y[g.part ? ypos[row] : row] += sum;
     }
 }
 
@@ -467,6 +561,30 @@ int spmv_prepare(hci_csr* h) {
 }
 
 namespace {
+// the two part launches of a TILED product: alpha-major tiles write y, then
+// beta-major tiles add their rows into their alpha-major rows (stream order)
+template <int MODE, bool PAIRS>
+void launch_win_parts(const hci_csr* h, hipStream_t s, const double* vals, const double* xa, double* xb,
+                      const double* x, double* y, bool absval) {
+    double* part = PAIRS ? h->d_pair_part : nullptr;
+    if (absval)
+        hipLaunchKernelGGL((k_spmv_win<MODE, true, PAIRS>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
+                           h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr, nullptr, part);
+    else
+        hipLaunchKernelGGL((k_spmv_win<MODE, false, PAIRS>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
+                           h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xa, h->d_diag, y, x, nullptr, part);
+    if (h->n_task_b > 0) {
+        if (absval)
+            hipLaunchKernelGGL((k_spmv_win<MODE, true, PAIRS>), (unsigned)h->n_task_b, kTileThreads, 0, s,
+                               h->d_task_b, h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr,
+                               h->d_bpos, part);
+        else
+            hipLaunchKernelGGL((k_spmv_win<MODE, false, PAIRS>), (unsigned)h->n_task_b, kTileThreads, 0, s,
+                               h->d_task_b, h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xb, nullptr, y, nullptr,
+                               h->d_bpos, part);
+    }
+}
+
 template <int MODE>
 void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
     hipStream_t s = h->ctx->stream;
@@ -481,27 +599,25 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
             xa = (const double*)h->d_x32;
             xb = (double*)h->d_xb32;
         }
-        // x gathered to beta-major order, the alpha-major part writes y,
-        // then the beta-major part adds each of its rows into its alpha-major
-        // row (stream order: the writes come first)
+        // x gathered to beta-major order for the beta-major part
         if (h->n_task_b > 0 && !absval)
             hipLaunchKernelGGL(k_gather_x<XT<MODE>>, grid_for(h->n, 256), 256, 0, s, h->d_bpos,
                                (const XT<MODE>*)xa, h->n, (XT<MODE>*)xb);
-        if (absval)
-            hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr, nullptr);
+        // the stored halves of mirror pairs (never with FP16 copies) take
+        // the instantiation with their code
+        if (MODE != 3 && h->n_pair > 0)
+            launch_win_parts<MODE, MODE != 3>(h, s, vals, xa, xb, x, y, absval);
         else
-            hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                               h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xa, h->d_diag, y, x, nullptr);
-        if (h->n_task_b > 0) {
-            if (absval)
-                hipLaunchKernelGGL((k_spmv_win<MODE, true>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr,
-                                   h->d_bpos);
-            else
-                hipLaunchKernelGGL((k_spmv_win<MODE, false>), (unsigned)h->n_task_b, kTileThreads, 0, s, h->d_task_b,
-                                   h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xb, nullptr, y, nullptr, h->d_bpos);
-        }
+            launch_win_parts<MODE, false>(h, s, vals, xa, xb, x, y, absval);
+        // the mirror pairs' transposed sums, after both parts have written y:
+        // the alpha-major receivers, then the beta-major ones (whose rows
+        // reach y through ypos and may meet an alpha-major receiver's row)
+        if (MODE != 3 && h->n_pair_seg0 > 0)
+            hipLaunchKernelGGL(k_pair_reduce, (unsigned)h->n_pair_seg0, 256, 0, s, h->d_pair_seg, h->d_pair_slot,
+                               h->d_pair_part, h->d_bpos, y);
+        if (MODE != 3 && h->n_pair_seg > h->n_pair_seg0)
+            hipLaunchKernelGGL(k_pair_reduce, (unsigned)(h->n_pair_seg - h->n_pair_seg0), 256, 0, s,
+                               h->d_pair_seg + h->n_pair_seg0, h->d_pair_slot, h->d_pair_part, h->d_bpos, y);
         return;
     }
     if constexpr (MODE == 3) return;  // FP16 copies exist in tiled storage only

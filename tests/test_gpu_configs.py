@@ -94,7 +94,8 @@ def test_workload_build_and_sigma_match_oracle(gpu_ctx, name, storage):
     assert np.array_equal(gpu_ctx.ham_diag(dets).view(np.uint64), diag.view(np.uint64))
     h = gpu_ctx.build_h(dets, diag, storage=storage)
     rp, col, val, dg = h.download()
-    print(f"{name}: n={dets.shape[0]} nnz_u={h.nnz} (oracle {S.nnz}) storage={storage}")
+    print(f"{name}: n={dets.shape[0]} nnz_u={h.nnz} (oracle {S.nnz}) storage={storage} "
+          f"mirror pairs {h.build_stats.get('mirror_pairs')}")
     assert h.nnz == S.nnz
     assert np.array_equal(rp, orp)
     assert np.array_equal(col, ocol)
