@@ -155,6 +155,7 @@ typedef struct hci_dav_stats {
     double sigma_ms;    /* summed device time of the sigma products (HIP events) */
     int n_sigma_f16;    /* FP16-value sigma products (mixed = 2) */
     int f16_iter;       /* iteration at which FP32 sigmas took over (-1: never) */
+    int orth_second;    /* corrections that needed the second Gram-Schmidt pass */
 } hci_dav_stats;
 
 /* context ---------------------------------------------------------------- */

@@ -1667,13 +1667,17 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         if (n_win > 0)
             HIP_TRY(c, hipMemcpyAsync(H->d_win, parted, (size_t)n_win * sizeof(Window), hipMemcpyDeviceToDevice, s));
     }
-    // tiles launch heaviest first (longest-processing-time order), so the
-    // largest alpha groups do not start last and stretch the product's tail
-    for (int part = 0; part < 2; ++part) {
-        std::vector<TileTask>& tv = part == 0 ? ta : tb;
-        TileTask* d_t = part == 0 ? H->d_task_a : H->d_task_b;
+    // alpha-major tiles launch heaviest first (longest-processing-time order),
+    // so the largest alpha groups do not start last and stretch the product's
+    // tail.  Beta-major tiles stay in position order: neighbouring beta groups
+    // add into neighbouring y entries of each alpha group, so tiles running
+    // together share y's cache lines (FP16 product on the 2e6-determinant
+    // workload 3.125 -> 3.054 ms, FP32 product on the 6x6 lattice 0.905 ->
+    // 0.878 ms)
+    if (ta.size() >= 2) {
+        std::vector<TileTask>& tv = ta;
+        TileTask* d_t = H->d_task_a;
         const int64_t nt = (int64_t)tv.size();
-        if (nt < 2) continue;
         int64_t* d_work = nullptr;
         HIP_TRY(c, wb.alloc(&d_work, (size_t)nt));
         hipLaunchKernelGGL(k_task_work, grid_for(nt, 256), 256, 0, s, d_t, nt, H->d_win, d_work);

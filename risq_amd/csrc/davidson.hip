@@ -107,12 +107,13 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(const double* partial, i
     if (threadIdx.x == 0) out[c] = sh[0];
 }
 
-// x = V y, wx = W y (= H x), r = wx - lam x, t = r / (lam - d); partial sums of
-// r.r and t.t; k <= KM
+// x = V y, r = W y - lam x (W = H V), t = r / (lam - d); partial sums of r.r
+// and t.t.  x and r are written only where given (most steps need t alone);
+// k <= KM
 template <int KM>
 __global__ void __launch_bounds__(kRedThreads)
     k_ritz(const double* __restrict__ V, const double* __restrict__ W, int64_t ld, int k, Coef y,
-           double lam, const double* __restrict__ d, int64_t n, double* x, double* wx, double* r, double* t,
+           double lam, const double* __restrict__ d, int64_t n, double* x, double* r, double* t,
            double* partial) {
     double acc[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -125,9 +126,8 @@ __global__ void __launch_bounds__(kRedThreads)
             }
         const double rv = wv - xv * lam;
         const double tv = rv / (lam - d[i]);
-        x[i] = xv;
-        wx[i] = wv;
-        r[i] = rv;
+        if (x) x[i] = xv;
+        if (r) r[i] = rv;
         t[i] = tv;
         acc[0] += rv * rv;
         acc[1] += tv * tv;
@@ -274,7 +274,6 @@ struct DavWork {
     double* V = nullptr;
     double* W = nullptr;
     double* x = nullptr;
-    double* wx = nullptr;
     double* q = nullptr;    // restart coefficients, kMaxSub x kMaxKeep
     double* h_q = nullptr;  // pinned staging for q
     double* r = nullptr;
@@ -285,16 +284,16 @@ struct DavWork {
     // carves every buffer out of the context's work block (kept between calls)
     int carve(int nsub) {
         const size_t vb = ((size_t)n * 8 + 255) / 256 * 256;
-        const size_t sizes[] = {vb * (size_t)nsub, vb * (size_t)nsub, vb, vb, vb, vb,
+        const size_t sizes[] = {vb * (size_t)nsub, vb * (size_t)nsub, vb, vb, vb,
                                 (size_t)kMaxSub * kMaxKeep * 8, (size_t)kRedBlocks * (kMaxSub + 1) * 8,
                                 (size_t)3 * (kMaxSub + 1) * 8};
         size_t total = 0;
         for (size_t b : sizes) total += (b + 255) / 256 * 256;
         const size_t host = (size_t)kMaxSub * kMaxKeep * 8 + 3 * (kMaxSub + 1) * 8 + 256;
         if (int rc = work_reserve(c, total, host)) return rc;
-        double** dst[] = {&V, &W, &x, &wx, &r, &t, &q, &partial, &red};
+        double** dst[] = {&V, &W, &x, &r, &t, &q, &partial, &red};
         char* p = (char*)c->work_dev;
-        for (int k = 0; k < 9; ++k) {
+        for (int k = 0; k < 8; ++k) {
             *dst[k] = (double*)p;
             p += (sizes[k] + 255) / 256 * 256;
         }
@@ -563,13 +562,26 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         lam = vals[0];
         Coef y{};
         for (int k = 0; k < dim_sub; ++k) y.c[k] = vecs[(size_t)k * dim_sub];
+        // the Ritz vector x and residual r are written when the step reads
+        // them (network features, recorded rows) or leaves the search space
+        // (ensure_x: convergence, the next precision, a restart to x); with
+        // the diagonal correction t goes straight to the next basis column
+        const bool want_xr = net || rec;
+        double* t_out = (!net && dim_sub < max_sub) ? w.col(w.V, dim_sub) : w.t;
         HCI_LAUNCH_KM(k_ritz, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam, h->d_diag, N,
-                      w.x, w.wx, w.r, w.t, w.partial);
+                      want_xr ? w.x : nullptr, want_xr ? w.r : nullptr, t_out, w.partial);
         DT(w.reduce(2));
+        auto ensure_x = [&]() {
+            if (!want_xr)
+                HCI_LAUNCH_KM(k_ritz, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam,
+                              h->d_diag, N, w.x, w.r, w.t, w.partial);
+            return hipGetLastError() == hipSuccess ? HCI_OK : hci_fail(c, HCI_ERR_HIP, "k_ritz launch failed");
+        };
         err = std::sqrt(w.h_red[0]);
         double tnorm = std::sqrt(w.h_red[1]);
         const bool conv = N <= 2 || (err < opts->tol && std::fabs(lam - last_eig) < opts->etol);
         if (conv && sigma_mode == HCI_SIGMA_FULL) {
+            DT(ensure_x());
             converged = true;
             break;
         }
@@ -597,6 +609,7 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         // stalls at the FP16 rounding floor
         if ((sigma_mode == HCI_SIGMA_F16 && (conv || err < f16_tol || since_best >= 8)) ||
             (sigma_mode == HCI_SIGMA_F32 && (conv || err < opts->refine_tol || since_best >= 25))) {
+            DT(ensure_x());
             DT(next_precision());
             continue;
         }
@@ -609,14 +622,15 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
                 ++st.mlp_applied;
                 mlp_prev = true;
             } else {
-                HIP_TRY(c, hipMemcpyAsync(t, w.t, vb, hipMemcpyDeviceToDevice, s));
+                if (t_out != t) HIP_TRY(c, hipMemcpyAsync(t, w.t, vb, hipMemcpyDeviceToDevice, s));
                 mlp_prev = false;
             }
-            // classical Gram-Schmidt, twice, with the coefficients kept on the
-            // device: t = V[:, dim_sub], so one pass over V[:, 0..dim_sub]
-            // gives V^T t and t.t (slot A); each orthogonalisation pass reads
-            // the previous slot and reduces into the next (B, then C); the
-            // host reads all three once, after the second pass
+            // classical Gram-Schmidt with the coefficients kept on the device:
+            // t = V[:, dim_sub], so one pass over V[:, 0..dim_sub] gives V^T t
+            // and t.t (slot A); an orthogonalisation pass reads the previous
+            // slot and reduces into the next (B, then C).  The second pass runs
+            // only when the first left less than 1/sqrt(2) of t's norm (the
+            // DGKS test: otherwise one pass is orthogonal to working precision)
             constexpr int kSlot = kMaxSub + 1;
             double* rA = w.red;
             double* rB = w.red + kSlot;
@@ -628,15 +642,22 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rA, t, N,
                           w.partial);
             w.reduce_dev(dim_sub + 1, rB);
-            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rB, t, N,
-                          w.partial);
-            w.reduce_dev(dim_sub + 1, rC);
-            HIP_TRY(c, hipMemcpyAsync(w.h_red, w.red, 3 * kSlot * 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipMemcpyAsync(w.h_red, w.red, 2 * kSlot * 8, hipMemcpyDeviceToHost, s));
             HIP_TRY(c, hipStreamSynchronize(s));
             if (mlp_now) tnorm = std::sqrt(w.h_red[dim_sub]);
-            const double nrm = std::sqrt(w.h_red[2 * kSlot + dim_sub]);
+            double nrm = std::sqrt(w.h_red[kSlot + dim_sub]);
+            if (!(nrm * nrm >= 0.5 * w.h_red[dim_sub])) {
+                HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rB, t,
+                              N, w.partial);
+                w.reduce_dev(dim_sub + 1, rC);
+                HIP_TRY(c, hipMemcpyAsync(w.h_red + 2 * kSlot, rC, kSlot * 8, hipMemcpyDeviceToHost, s));
+                HIP_TRY(c, hipStreamSynchronize(s));
+                nrm = std::sqrt(w.h_red[2 * kSlot + dim_sub]);
+                ++st.orth_second;
+            }
             // correction floors (davidson.rs:210-218, modified_gram_schmidt.rs:29)
             if (tnorm < 1e-9 || nrm < 1e-9) {
+                DT(ensure_x());
                 if (sigma_mode != HCI_SIGMA_FULL) {
                     DT(next_precision());
                     continue;
@@ -711,6 +732,7 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             mlp_prev = false;
         } else {
             // restart from the Ritz vector (davidson.rs:258-269)
+            DT(ensure_x());
             dim_sub = 1;
             HIP_TRY(c, hipMemcpyAsync(w.V, w.x, vb, hipMemcpyDeviceToDevice, s));
             DT(rebuild_projection());

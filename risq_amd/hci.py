@@ -67,7 +67,8 @@ class DavStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("n_sigma_f32", C.c_int), ("n_sigma_full", C.c_int),
                 ("converged", C.c_int), ("mlp_applied", C.c_int), ("mlp_fallbacks", C.c_int),
                 ("refine_iter", C.c_int), ("residual", C.c_double), ("seconds", C.c_double),
-                ("sigma_ms", C.c_double), ("n_sigma_f16", C.c_int), ("f16_iter", C.c_int)]
+                ("sigma_ms", C.c_double), ("n_sigma_f16", C.c_int), ("f16_iter", C.c_int),
+                ("orth_second", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
