@@ -819,6 +819,12 @@ __global__ void k_task_remap(TileTask* t, int64_t n, const int32_t* nidx) {
 }
 
 // entries of each tile (for the order in which tiles are launched)
+// inv[perm[q]] = q
+__global__ void k_invert_perm(const int32_t* __restrict__ perm, int64_t n, int32_t* inv) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) inv[perm[q]] = (int32_t)q;
+}
+
 __global__ void k_task_work(const TileTask* t, int64_t n, const Window* __restrict__ wins, int64_t* work) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -1694,7 +1700,11 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     }
     HIP_TRY(c, hipMalloc(&H->d_bpos, (size_t)N * 4 + 16));
     HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(c, hipMalloc(&H->d_binv, (size_t)N * 4 + 16));
+    hipLaunchKernelGGL(k_invert_perm, grid_for(N, 256), 256, 0, s, H->d_bpos, N, H->d_binv);
+    HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMalloc(&H->d_xb, (size_t)N * 8 + 16));
+    HIP_TRY(c, hipMalloc(&H->d_yb, (size_t)N * 8 + 16));
     if (H->d_h16) {
         HIP_TRY(c, hipMalloc(&H->d_x32, (size_t)N * 4 + 16));
         HIP_TRY(c, hipMalloc(&H->d_xb32, (size_t)N * 4 + 16));

@@ -541,7 +541,9 @@ void hci_csr_free(hci_csr* h) {
     hipFree(h->d_task_a);
     hipFree(h->d_task_b);
     hipFree(h->d_bpos);
+    hipFree(h->d_binv);
     hipFree(h->d_xb);
+    hipFree(h->d_yb);
     hipFree(h->d_x32);
     hipFree(h->d_xb32);
     delete h;

@@ -184,7 +184,9 @@ struct hci_csr {
     TileTask* d_task_b = nullptr;  // beta-major tiles (kind 2)
     int64_t n_task_b = 0;
     int32_t* d_bpos = nullptr;     // beta-major position -> internal position
+    int32_t* d_binv = nullptr;     // internal position -> beta-major position
     double* d_xb = nullptr;        // x in beta-major order
+    double* d_yb = nullptr;        // the beta-major part's row sums (beta-major order)
     float* d_x32 = nullptr;        // FP32 copies of x for FP16-value products (HCI_PREC_F16)
     float* d_xb32 = nullptr;
     // mirror pairs (TILED): partial sums (n_pair x kWinCols doubles), the

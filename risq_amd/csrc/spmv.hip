@@ -306,8 +306,8 @@ __global__ void __launch_bounds__(kTileThreads)
     k_spmv_win(const TileTask* __restrict__ tasks, const Window* __restrict__ wins,
                const uint16_t* __restrict__ idx, const double* __restrict__ val, const float* __restrict__ hi,
                const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
-               double* __restrict__ y, const double* __restrict__ xdiag, const int32_t* __restrict__ ypos,
-               double* __restrict__ part) {
+               double* __restrict__ y, const double* __restrict__ xdiag, double* __restrict__ yb_out,
+               const double* __restrict__ yb_in, const int32_t* __restrict__ binv, double* __restrict__ part) {
     __shared__ double acc[kTileRows];
     __shared__ XT<MODE> xs[kTileWaves][kWinCols];
     __shared__ XT<MODE> xr[PAIRS ? kTileRows : 1];           // x of the tile's own rows (mirror pairs)
@@ -464,26 +464,29 @@ __global__ void __launch_bounds__(kTileThreads)
     __syncthreads();
     for (int r = tid; r < t.nr; r += kTileThreads) {
         const int64_t p = (int64_t)t.r0 + r;
-        if (ypos)  // beta-major rows: added to their alpha-major rows, which one launch owns
-            y[ypos[p]] += acc[r];
-        else
-            y[p] = (diag && !ABS) ? diag[p] * xdiag[p] + acc[r] : acc[r];  // xdiag: x in FP64
+        if (yb_out) {  // beta-major rows, in their own order
+            yb_out[p] = acc[r];
+        } else {  // alpha-major rows, with the beta-major part's sum of the same row
+            double v = (diag && !ABS) ? diag[p] * xdiag[p] + acc[r] : acc[r];  // xdiag: x in FP64
+            if (yb_in) v += yb_in[binv[p]];
+            y[p] = v;
+        }
     }
 }
 
 // The receiving side of the mirror pairs: one workgroup per receiving tile
 // adds, row by row, the transposed sums of the slots that target it (in slot
-// order) into y (beta-major tiles through ypos, to their alpha-major rows).
-// The tiles of one launch have disjoint rows; one launch per part.
+// order) into its part's rows: y for alpha-major tiles, the beta-major row
+// sums (before the alpha-major launch reads them) for beta-major tiles.  The
+// tiles of one launch have disjoint rows.
 __global__ void __launch_bounds__(256)
     k_pair_reduce(const PairSeg* __restrict__ segs, const int32_t* __restrict__ slots,
-                  const double* __restrict__ part, const int32_t* __restrict__ ypos, double* __restrict__ y) {
+                  const double* __restrict__ part, double* __restrict__ y) {
     const PairSeg g = segs[blockIdx.x];
     for (int r = threadIdx.x; r < g.nr; r += blockDim.x) {
         double sum = 0.0;
         for (int32_t k = g.s_begin; k < g.s_end; ++k) sum += part[(size_t)slots[k] * kWinCols + r];
-        const int64_t row = (int64_t)g.r0 + r;
-        y[g.part ? ypos[row] : row] += sum;
+        y[(int64_t)g.r0 + r] += sum;
     }
 }
 
@@ -561,28 +564,40 @@ int spmv_prepare(hci_csr* h) {
 }
 
 namespace {
-// the two part launches of a TILED product: alpha-major tiles write y, then
-// beta-major tiles add their rows into their alpha-major rows (stream order)
+// the two part launches of a TILED product, with the mirror pairs' sums:
+// beta-major tiles write their row sums in beta-major order, the beta-major
+// receivers add theirs, then alpha-major tiles write y (each row with its
+// beta-major sum, read through binv) and the alpha-major receivers add theirs
 template <int MODE, bool PAIRS>
-void launch_win_parts(const hci_csr* h, hipStream_t s, const double* vals, const double* xa, double* xb,
+void launch_win_parts(const hci_csr* h, hipStream_t s, const double* vals, const double* xa, const double* xb,
                       const double* x, double* y, bool absval) {
     double* part = PAIRS ? h->d_pair_part : nullptr;
-    if (absval)
-        hipLaunchKernelGGL((k_spmv_win<MODE, true, PAIRS>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                           h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr, nullptr, part);
-    else
-        hipLaunchKernelGGL((k_spmv_win<MODE, false, PAIRS>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
-                           h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xa, h->d_diag, y, x, nullptr, part);
-    if (h->n_task_b > 0) {
+    const bool beta = h->n_task_b > 0;
+    if (beta) {
         if (absval)
             hipLaunchKernelGGL((k_spmv_win<MODE, true, PAIRS>), (unsigned)h->n_task_b, kTileThreads, 0, s,
                                h->d_task_b, h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr,
-                               h->d_bpos, part);
+                               h->d_yb, nullptr, nullptr, part);
         else
             hipLaunchKernelGGL((k_spmv_win<MODE, false, PAIRS>), (unsigned)h->n_task_b, kTileThreads, 0, s,
                                h->d_task_b, h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xb, nullptr, y, nullptr,
-                               h->d_bpos, part);
+                               h->d_yb, nullptr, nullptr, part);
+        if (PAIRS && h->n_pair_seg > h->n_pair_seg0)
+            hipLaunchKernelGGL(k_pair_reduce, (unsigned)(h->n_pair_seg - h->n_pair_seg0), 256, 0, s,
+                               h->d_pair_seg + h->n_pair_seg0, h->d_pair_slot, h->d_pair_part, h->d_yb);
     }
+    const double* yb = beta ? h->d_yb : nullptr;
+    if (absval)
+        hipLaunchKernelGGL((k_spmv_win<MODE, true, PAIRS>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
+                           h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, nullptr, nullptr, y, nullptr, nullptr, yb,
+                           h->d_binv, part);
+    else
+        hipLaunchKernelGGL((k_spmv_win<MODE, false, PAIRS>), (unsigned)h->n_task_a, kTileThreads, 0, s, h->d_task_a,
+                           h->d_win, h->d_idx16, vals, h->d_hi, h->d_lo, xa, h->d_diag, y, x, nullptr, yb, h->d_binv,
+                           part);
+    if (PAIRS && h->n_pair_seg0 > 0)
+        hipLaunchKernelGGL(k_pair_reduce, (unsigned)h->n_pair_seg0, 256, 0, s, h->d_pair_seg, h->d_pair_slot,
+                           h->d_pair_part, y);
 }
 
 template <int MODE>
@@ -609,15 +624,6 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
             launch_win_parts<MODE, MODE != 3>(h, s, vals, xa, xb, x, y, absval);
         else
             launch_win_parts<MODE, false>(h, s, vals, xa, xb, x, y, absval);
-        // the mirror pairs' transposed sums, after both parts have written y:
-        // the alpha-major receivers, then the beta-major ones (whose rows
-        // reach y through ypos and may meet an alpha-major receiver's row)
-        if (MODE != 3 && h->n_pair_seg0 > 0)
-            hipLaunchKernelGGL(k_pair_reduce, (unsigned)h->n_pair_seg0, 256, 0, s, h->d_pair_seg, h->d_pair_slot,
-                               h->d_pair_part, h->d_bpos, y);
-        if (MODE != 3 && h->n_pair_seg > h->n_pair_seg0)
-            hipLaunchKernelGGL(k_pair_reduce, (unsigned)(h->n_pair_seg - h->n_pair_seg0), 256, 0, s,
-                               h->d_pair_seg + h->n_pair_seg0, h->d_pair_slot, h->d_pair_part, h->d_bpos, y);
         return;
     }
     if constexpr (MODE == 3) return;  // FP16 copies exist in tiled storage only
