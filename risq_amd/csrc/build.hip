@@ -307,6 +307,11 @@ __global__ void __launch_bounds__(kBlock)
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const SubBlock sb = sbs[w];
     const int n_slices = (sb.nc + kWinCols - 1) / kWinCols;
+    if (FILL && pairf) {  // every window held by its mirror: nothing to search
+        bool all = true;
+        for (int k = 0; k < n_slices; ++k) all = all && pairf[sb.win_begin + k] == 2;
+        if (all) return;
+    }
     int32_t* wcur = s_cur[threadIdx.x >> 6];
     int64_t* woff = s_off[threadIdx.x >> 6];
     if (lane < kWinPerSb) wcur[lane] = 0;
