@@ -298,7 +298,7 @@ template <class S, bool FILL>
 __global__ void __launch_bounds__(kBlock)
     k_tile_connect(const SubBlock* __restrict__ sbs, int64_t n_sb, BuildView<S> B, const S* ub, DevInts T,
                    SingleRules rules, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out,
-                   const int8_t* __restrict__ pairf, const int32_t* __restrict__ mirror) {
+                   const int8_t* __restrict__ pairf) {
     __shared__ int32_t s_cur[kWaves][kWinPerSb];
     __shared__ int64_t s_off[kWaves][kWinPerSb];
     const int lane = threadIdx.x & 63;
@@ -341,14 +341,8 @@ __global__ void __launch_bounds__(kBlock)
         } else if (keep) {
             pos = atomicAdd(&wcur[slice], 1);
         }
-        if (FILL && keep) {
+        if (FILL && keep)
             put16(out, woff[slice] + pos, ((uint32_t)(rl + sb.rb) << 8) | (uint32_t)(cl % kWinCols), v);
-            // both halves stored: the mirror window takes the transposed entry
-            // at the same position
-            if (mirror && f == 1)
-                put16(out, out.woff[mirror[sb.win_begin + slice]] + pos,
-                      ((uint32_t)(cl % kWinCols) << 8) | (uint32_t)(rl + sb.rb), v);
-        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -497,16 +491,13 @@ __global__ void __launch_bounds__(kBlock)
         // visited in order: the running count of the current window is a
         // wave-uniform register (no LDS cursor), stored when its last column
         // is done
-        auto emit_col = [&](bool keep, int32_t rl, int32_t cl, double v, int32_t& run, int64_t base, int wgt,
-                            int64_t mbase) {
+        auto emit_col = [&](bool keep, int32_t rl, int32_t cl, double v, int32_t& run, int64_t base, int wgt) {
             const uint64_t mk = __ballot(keep);
             if (FILL) {
                 nz += (unsigned long long)wgt * __popcll(__ballot(keep && v != 0.0));
-                if (keep) {
-                    const int64_t e = run + __popcll(mk & lt_mask);
-                    put16(out, base + e, ((uint32_t)(rl + sb.rb) << 8) | (uint32_t)(cl % kWinCols), v);
-                    if (mbase >= 0) put16(out, mbase + e, ((uint32_t)(cl % kWinCols) << 8) | (uint32_t)(rl + sb.rb), v);
-                }
+                if (keep)
+                    put16(out, base + run + __popcll(mk & lt_mask), ((uint32_t)(rl + sb.rb) << 8) | (uint32_t)(cl % kWinCols),
+                          v);
             }
             run += __popcll(mk);
         };
@@ -529,7 +520,6 @@ __global__ void __launch_bounds__(kBlock)
             if (f == 2) continue;  // the mirror window holds these entries
             const int wgt = f == 1 ? 2 : 1;
             const int64_t base = FILL ? out.woff[sb.win_begin + slice] : 0;
-            const int64_t mbase = (FILL && mirror && f == 1) ? out.woff[mirror[sb.win_begin + slice]] : -1;
             int32_t run = 0;
             const int32_t w1 = min(sb.nc, w0 + kWinCols);
             for (int32_t cl = w0; cl < w1; ++cl) {
@@ -542,7 +532,7 @@ __global__ void __launch_bounds__(kBlock)
                     bool keep;
                     double v;
                     pair_val(lane + 64 * k < sb.nr, rs[k], ri[k], b, j, keep, v);
-                    emit_col(keep, lane + 64 * k, cl, v, run, base, wgt, mbase);
+                    emit_col(keep, lane + 64 * k, cl, v, run, base, wgt);
                 }
             }
             if (!FILL && lane == 0) win_cnt[sb.win_begin + slice] = run;
@@ -643,11 +633,10 @@ __global__ void k_mirror_keys(const int64_t* wc, const int32_t* wc0, const int32
 // A dense window (rows: one tile's range R, columns C) is paired with the
 // window whose rows are C and whose columns are R, when both are the only
 // windows with their keys and their sizes and counts agree; the half above
-// the diagonal (R before C) is kept (flag 1, mirror[k] = its mirror's index),
-// the other dropped (flag 2).
+// the diagonal (R before C) is kept (flag 1), the other dropped (flag 2).
 __global__ void k_mirror_match(const uint64_t* __restrict__ skey, const int32_t* __restrict__ sidx, int64_t n,
                                const int64_t* wc, const int32_t* wc0, const int32_t* wnc, const int32_t* wr0,
-                               const int32_t* wnr, int8_t* pairf, int32_t* mirror) {
+                               const int32_t* wnr, int8_t* pairf) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || skey[i] == ~0ull) return;
     const uint64_t key = skey[i];
@@ -669,7 +658,6 @@ __global__ void k_mirror_match(const uint64_t* __restrict__ skey, const int32_t*
     const int32_t k2 = sidx[lo];
     if (wnc[k2] != wnr[k] || wnc[k] != wnr[k2] || wc[k2] != wc[k]) return;
     pairf[k] = R0 < C0 ? 1 : 2;
-    if (R0 < C0) mirror[k] = k2;
 }
 
 // dropped windows hold no entries; kept ones are numbered (flag0[k] = 1)
@@ -1533,27 +1521,22 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     if (n_sb > 0) {
         hipLaunchKernelGGL(k_win_init, grid_for(n_sb, 256), 256, 0, s, d_sb, n_sb, d_wc0, d_wnc);
         hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           rules, dbl, d_wc, none, nullptr, nullptr, nullptr);
+                           rules, dbl, d_wc, none, nullptr, nullptr);
     }
-    // mirror pairs: dense windows whose mirror is another tile's window.
-    // Without FP16 copies the kept halves are stored alone: they are numbered
-    // (their partial-sum slots) and grouped by the tile that receives their
-    // transposed sums.  With FP16 copies both halves are stored (the FP16
-    // product is bound by its LDS row atomics rather than its bytes: stored
-    // halves made it slower, 3.20-3.58 ms against 2.96-3.17 ms per product on
-    // the 2e6-determinant workload), and the kept half's elements, evaluated
-    // once, are written to both (d_mirror)
-    const bool store_once = o.value_prec != HCI_PREC_F16;
+    // mirror pairs: dense windows whose mirror is another tile's window; the
+    // kept halves are numbered (their partial-sum slots) and grouped by the
+    // tile that receives their transposed sums.  Not with FP16 copies: the
+    // FP16 product is bound by its LDS row atomics rather than its bytes, and
+    // the pairs' extra LDS work made it slower (3.43-3.58 ms against 3.17 ms
+    // per product on the 2e6-determinant workload)
     int8_t* d_pairf = nullptr;
     int32_t* d_pslot = nullptr;
-    int32_t* d_mirror = nullptr;
-    if (n_wall > 0 && n_sb > 0) {
+    if (n_wall > 0 && n_sb > 0 && o.value_prec != HCI_PREC_F16) {
         int32_t *d_wr0 = nullptr, *d_wnr = nullptr, *d_mi = nullptr, *d_mi2 = nullptr, *d_kept = nullptr;
         int8_t* d_wpart = nullptr;
         uint64_t *d_mk = nullptr, *d_mk2 = nullptr;
         HIP_TRY(c, wb.alloc(&d_pairf, (size_t)n_wall + 1));
         HIP_TRY(c, wb.alloc(&d_pslot, (size_t)n_wall + 1));
-        HIP_TRY(c, wb.alloc(&d_mirror, (size_t)n_wall + 1));
         HIP_TRY(c, wb.alloc(&d_kept, (size_t)n_wall + 1));
         HIP_TRY(c, wb.alloc(&d_wr0, (size_t)n_wall));
         HIP_TRY(c, wb.alloc(&d_wnr, (size_t)n_wall));
@@ -1579,55 +1562,53 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         HIP_TRY(c, tmp.ensure(need));
         HIP_TRY(c, rocprim::radix_sort_pairs(tmp.p, need, d_mk, d_mk2, d_mi, d_mi2, (size_t)n_wall, 0, 64, s));
         hipLaunchKernelGGL(k_mirror_match, grid_for(n_wall, 256), 256, 0, s, d_mk2, d_mi2, n_wall, d_wc, d_wc0, d_wnc,
-                           d_wr0, d_wnr, d_pairf, d_mirror);
-        if (store_once) {
-            hipLaunchKernelGGL(k_mirror_drop, grid_for(n_wall + 1, 256), 256, 0, s, d_pairf, n_wall, d_wc, d_kept);
-            HIP_TRY(c, exclusive_scan(tmp, d_kept, d_pslot, (size_t)n_wall + 1, s));
-            int32_t n_pair = 0;
-            HIP_TRY(c, hipMemcpyAsync(&n_pair, d_pslot + n_wall, 4, hipMemcpyDeviceToHost, s));
+                           d_wr0, d_wnr, d_pairf);
+        hipLaunchKernelGGL(k_mirror_drop, grid_for(n_wall + 1, 256), 256, 0, s, d_pairf, n_wall, d_wc, d_kept);
+        HIP_TRY(c, exclusive_scan(tmp, d_kept, d_pslot, (size_t)n_wall + 1, s));
+        int32_t n_pair = 0;
+        HIP_TRY(c, hipMemcpyAsync(&n_pair, d_pslot + n_wall, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        if (n_pair >= (1 << 24) - 1) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many mirror window pairs");
+        H->n_pair = n_pair;
+        if (n_pair > 0) {
+            uint64_t *d_rk = nullptr, *d_rk2 = nullptr;
+            int32_t *d_rs = nullptr, *d_head = nullptr, *d_hpos = nullptr;
+            HIP_TRY(c, wb.alloc(&d_rk, (size_t)n_pair));
+            HIP_TRY(c, wb.alloc(&d_rk2, (size_t)n_pair));
+            HIP_TRY(c, wb.alloc(&d_rs, (size_t)n_pair));
+            HIP_TRY(c, wb.alloc(&d_head, (size_t)n_pair + 1));
+            HIP_TRY(c, wb.alloc(&d_hpos, (size_t)n_pair + 1));
+            HIP_TRY(c, hipMalloc(&H->d_pair_slot, (size_t)n_pair * 4 + 16));
+            hipLaunchKernelGGL(k_mirror_recv, grid_for(n_wall, 256), 256, 0, s, d_pairf, d_pslot, d_wc0, d_wnc,
+                               d_wpart, n_wall, d_rk, d_rs);
+            need = 0;
+            HIP_TRY(c, rocprim::radix_sort_pairs(nullptr, need, d_rk, d_rk2, d_rs, H->d_pair_slot, (size_t)n_pair, 0,
+                                                 64, s));
+            HIP_TRY(c, tmp.ensure(need));
+            HIP_TRY(c, rocprim::radix_sort_pairs(tmp.p, need, d_rk, d_rk2, d_rs, H->d_pair_slot, (size_t)n_pair, 0,
+                                                 64, s));
+            hipLaunchKernelGGL(k_mirror_heads, grid_for((int64_t)n_pair + 1, 256), 256, 0, s, d_rk2, (int64_t)n_pair,
+                               d_head);
+            HIP_TRY(c, exclusive_scan(tmp, d_head, d_hpos, (size_t)n_pair + 1, s));
+            int32_t n_seg = 0;
+            HIP_TRY(c, hipMemcpyAsync(&n_seg, d_hpos + n_pair, 4, hipMemcpyDeviceToHost, s));
             HIP_TRY(c, hipStreamSynchronize(s));
-            if (n_pair >= (1 << 24) - 1) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many mirror window pairs");
-            H->n_pair = n_pair;
-            if (n_pair > 0) {
-                uint64_t *d_rk = nullptr, *d_rk2 = nullptr;
-                int32_t *d_rs = nullptr, *d_head = nullptr, *d_hpos = nullptr;
-                HIP_TRY(c, wb.alloc(&d_rk, (size_t)n_pair));
-                HIP_TRY(c, wb.alloc(&d_rk2, (size_t)n_pair));
-                HIP_TRY(c, wb.alloc(&d_rs, (size_t)n_pair));
-                HIP_TRY(c, wb.alloc(&d_head, (size_t)n_pair + 1));
-                HIP_TRY(c, wb.alloc(&d_hpos, (size_t)n_pair + 1));
-                HIP_TRY(c, hipMalloc(&H->d_pair_slot, (size_t)n_pair * 4 + 16));
-                hipLaunchKernelGGL(k_mirror_recv, grid_for(n_wall, 256), 256, 0, s, d_pairf, d_pslot, d_wc0, d_wnc,
-                                   d_wpart, n_wall, d_rk, d_rs);
-                need = 0;
-                HIP_TRY(c, rocprim::radix_sort_pairs(nullptr, need, d_rk, d_rk2, d_rs, H->d_pair_slot, (size_t)n_pair, 0,
-                                                     64, s));
-                HIP_TRY(c, tmp.ensure(need));
-                HIP_TRY(c, rocprim::radix_sort_pairs(tmp.p, need, d_rk, d_rk2, d_rs, H->d_pair_slot, (size_t)n_pair, 0,
-                                                     64, s));
-                hipLaunchKernelGGL(k_mirror_heads, grid_for((int64_t)n_pair + 1, 256), 256, 0, s, d_rk2, (int64_t)n_pair,
-                                   d_head);
-                HIP_TRY(c, exclusive_scan(tmp, d_head, d_hpos, (size_t)n_pair + 1, s));
-                int32_t n_seg = 0;
-                HIP_TRY(c, hipMemcpyAsync(&n_seg, d_hpos + n_pair, 4, hipMemcpyDeviceToHost, s));
-                HIP_TRY(c, hipStreamSynchronize(s));
-                H->n_pair_seg = n_seg;
-                HIP_TRY(c, hipMalloc(&H->d_pair_seg, (size_t)n_seg * sizeof(PairSeg) + 16));
-                HIP_TRY(c, hipMalloc(&H->d_pair_part, (size_t)n_pair * kWinCols * 8 + 16));
-                hipLaunchKernelGGL(k_mirror_segs, grid_for(n_pair, 256), 256, 0, s, d_rk2, d_head, d_hpos,
-                                   (int64_t)n_pair, H->d_pair_seg);
-                hipLaunchKernelGGL(k_mirror_seg_end, grid_for(n_seg, 256), 256, 0, s, H->d_pair_seg, n_seg);
-                // the segments are in key order, part first: count the alpha-major ones
-                std::vector<PairSeg> hs((size_t)n_seg);
-                HIP_TRY(c, hipMemcpyAsync(hs.data(), H->d_pair_seg, (size_t)n_seg * sizeof(PairSeg),
-                                          hipMemcpyDeviceToHost, s));
-                HIP_TRY(c, hipStreamSynchronize(s));
-                int64_t n0 = 0;
-                while (n0 < n_seg && hs[(size_t)n0].part == 0) ++n0;
-                for (int64_t g = n0; g < n_seg; ++g)
-                    if (hs[(size_t)g].part == 0) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: mirror segments out of order");
-                H->n_pair_seg0 = n0;
-            }
+            H->n_pair_seg = n_seg;
+            HIP_TRY(c, hipMalloc(&H->d_pair_seg, (size_t)n_seg * sizeof(PairSeg) + 16));
+            HIP_TRY(c, hipMalloc(&H->d_pair_part, (size_t)n_pair * kWinCols * 8 + 16));
+            hipLaunchKernelGGL(k_mirror_segs, grid_for(n_pair, 256), 256, 0, s, d_rk2, d_head, d_hpos,
+                               (int64_t)n_pair, H->d_pair_seg);
+            hipLaunchKernelGGL(k_mirror_seg_end, grid_for(n_seg, 256), 256, 0, s, H->d_pair_seg, n_seg);
+            // the segments are in key order, part first: count the alpha-major ones
+            std::vector<PairSeg> hs((size_t)n_seg);
+            HIP_TRY(c, hipMemcpyAsync(hs.data(), H->d_pair_seg, (size_t)n_seg * sizeof(PairSeg),
+                                      hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+            int64_t n0 = 0;
+            while (n0 < n_seg && hs[(size_t)n0].part == 0) ++n0;
+            for (int64_t g = n0; g < n_seg; ++g)
+                if (hs[(size_t)g].part == 0) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: mirror segments out of order");
+            H->n_pair_seg0 = n0;
         }
         HIP_TRY(c, hipGetLastError());
     }
@@ -1664,10 +1645,9 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMalloc(&H->d_win, (size_t)n_win * sizeof(Window) + 16));
     if (n_sb > 0)
         hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           rules, dbl, nullptr, fo, (unsigned long long*)d_nnz, d_pairf,
-                           store_once ? nullptr : d_mirror);
+                           rules, dbl, nullptr, fo, (unsigned long long*)d_nnz, d_pairf);
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
-                       store_once ? d_pairf : nullptr, d_pslot, H->d_win);
+                       d_pairf, d_pslot, H->d_win);
     if (n_win > 0) {
         const unsigned ilg = std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u);
         if (H->d_val)
