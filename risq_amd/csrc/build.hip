@@ -706,6 +706,121 @@ __global__ void k_mirror_seg_end(PairSeg* seg, int32_t n_seg) {
     if (g + 1 < n_seg) seg[g].s_end = seg[g + 1].s_begin;
 }
 
+// the stored halves by slot: their (compacted) window indices, and the
+// largest entry count among them (for the sort's scratch)
+__global__ void k_pair_list(const int8_t* pairf, const int32_t* pslot, const int32_t* nidx, const int64_t* wc,
+                            int64_t n, int32_t* list, int* max_cnt) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || pairf[k] != 1) return;
+    list[pslot[k]] = nidx[k];
+    atomicMax(max_cnt, (int)wc[k]);
+}
+
+// Each stored half's entries sorted by diagonal ((column - row) mod 256),
+// stably, one wavefront per window: a histogram of the diagonals, its scan,
+// then the entries in fill order, 64 at a time, each lane ranked among the
+// lanes of its diagonal (eight ballots), scattered into the wave's scratch
+// and copied back in place.  Values travel as 8-byte words (an FP64 value,
+// or FP32 hi | lo << 32).
+constexpr int kDiagWaves = 4;
+template <bool F64>
+__global__ void __launch_bounds__(64 * kDiagWaves)
+    k_pair_diag(const Window* __restrict__ wins, const int32_t* __restrict__ list, int64_t n_pair, uint16_t* idx,
+                double* val, float* hi, float* lo, uint16_t* sidx, uint64_t* sval, int64_t per_wave) {
+    __shared__ int32_t s_cur[kDiagWaves][256];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int32_t* cur = s_cur[wv];
+    const int64_t gw = (int64_t)blockIdx.x * kDiagWaves + wv, nw = (int64_t)gridDim.x * kDiagWaves;
+    uint16_t* wi = sidx + gw * per_wave;
+    uint64_t* wv8 = sval + gw * per_wave;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    auto diag_of = [](uint32_t u) { return ((u & 0xFFu) - (u >> 8)) & 0xFFu; };
+    for (int64_t p = gw; p < n_pair; p += nw) {
+        const Window W = wins[list[p]];
+        const int64_t o = W.o();
+        const int cnt = W.cnt();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cur[lane + 64 * j] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int j = lane; j < cnt; j += 64) atomicAdd(&cur[diag_of(idx[o + j])], 1);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // exclusive scan: lane l holds diagonals 4 l .. 4 l + 3
+        int c4[4], sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c4[j] = cur[4 * lane + j];
+            sum += c4[j];
+        }
+        int inc = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(inc, off);
+            if (lane >= off) inc += t;
+        }
+        int run = inc - sum;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cur[4 * lane + j] = run;
+            run += c4[j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int base = 0; base < cnt; base += 64) {
+            const int j = base + lane;
+            const bool ok = j < cnt;
+            uint32_t u = 0, d = 0;
+            uint64_t v = 0;
+            if (ok) {
+                u = idx[o + j];
+                d = diag_of(u);
+                v = F64 ? __builtin_bit_cast(uint64_t, val[o + j])
+                        : (uint64_t)__builtin_bit_cast(uint32_t, hi[o + j]) |
+                              ((uint64_t)__builtin_bit_cast(uint32_t, lo[o + j]) << 32);
+            }
+            uint64_t same = __ballot(ok);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t m = __ballot(ok && ((d >> b) & 1u));
+                same &= ((d >> b) & 1u) ? m : ~m;
+            }
+            const int leader = ok ? (int)__builtin_ctzll(same) : lane;
+            int first = 0;
+            if (ok && leader == lane) first = atomicAdd(&cur[d], (int)__popcll(same));
+            first = __shfl(first, leader);
+            if (ok) {
+                const int q = first + (int)__popcll(same & below);
+                wi[q] = (uint16_t)u;
+                wv8[q] = v;
+            }
+        }
+        // the wave's scratch writes complete before its lanes read them back
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int j = lane; j < cnt; j += 64) {
+            idx[o + j] = wi[j];
+            const uint64_t v = wv8[j];
+            if (F64) {
+                val[o + j] = __builtin_bit_cast(double, v);
+            } else {
+                hi[o + j] = __builtin_bit_cast(float, (uint32_t)v);
+                lo[o + j] = __builtin_bit_cast(float, (uint32_t)(v >> 32));
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
 // In place, one wavefront per window: every 512-entry piece (the last one
 // shorter) is read in enumeration order into the wave's LDS and written back
 // slot by slot (the inverse of win_slot, whose slots stay inside the piece's
@@ -768,13 +883,8 @@ __global__ void __launch_bounds__(64 * kIlWaves)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // slot order out: lane l writes slots 8 l .. 8 l + 7, which hold
         // entries 64 q + l of a full piece, or l + q R of the last one (slots
-        // past its entries are written as zeros).  The stored half of a
-        // mirror pair keeps lane l's group as fill-order entries 8 l .. 8 l + 7,
-        // rotated by l (slot 8 l + q holds entry 8 l + (q + l) % 8): runs of
-        // one column stay mostly together for the product's transposed sums,
-        // and neighbouring lanes, often in neighbouring columns with similar
-        // rows, do not reach the same row in the same step.
-        const bool full = n == kWinChunk, fill_order = W.sp() >= 0;
+        // past its entries are written as zeros)
+        const bool full = n == kWinChunk;
         const int R = (n + kWinAlign - 1) / kWinAlign;
         if (full || lane < R) {
             il_u16x8 ui, uh;
@@ -782,7 +892,7 @@ __global__ void __launch_bounds__(64 * kIlWaves)
             il_f64x2 d2[4];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int j = fill_order ? 8 * lane + ((q + lane) & 7) : full ? q * 64 + lane : lane + q * R;
+                const int j = full ? q * 64 + lane : lane + q * R;
                 const bool ok = j < n;
                 const int pj = j + ((j >> 6) << 3);
                 ui[q] = ok ? si[pj] : (unsigned short)0;
@@ -1524,11 +1634,14 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
                            rules, dbl, d_wc, none, nullptr, nullptr);
     }
     // mirror pairs: dense windows whose mirror is another tile's window; the
-    // kept halves are numbered (their partial-sum slots) and grouped by the
-    // tile that receives their transposed sums.  Not with FP16 copies: the
-    // FP16 product is bound by its LDS row atomics rather than its bytes, and
-    // the pairs' extra LDS work made it slower (3.43-3.58 ms against 3.17 ms
-    // per product on the 2e6-determinant workload)
+    // kept halves are stored alone, numbered (their partial-sum slots) and
+    // grouped by the tile that receives their transposed sums.  On the
+    // 2e6-determinant workload the FP64 product takes 9.1 ms instead of
+    // 10.6 ms, the FP32 hi + lo product 7.25 ms instead of 7.6 ms, the FP32
+    // product the same 4.7 ms.  Not with FP16 copies: the FP16-value product
+    // is bound by its LDS work (the LDS is busy ~87% of its cycles), which the
+    // transposed sums raise, so it takes 3.39 ms with pairs against 3.09 ms
+    // without, though it reads 19% fewer bytes
     int8_t* d_pairf = nullptr;
     int32_t* d_pslot = nullptr;
     if (n_wall > 0 && n_sb > 0 && o.value_prec != HCI_PREC_F16) {
@@ -1648,6 +1761,35 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
                            rules, dbl, nullptr, fo, (unsigned long long*)d_nnz, d_pairf);
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
                        d_pairf, d_pslot, H->d_win);
+    if (H->n_pair > 0) {
+        // the stored halves sorted by diagonal before the interleave, each
+        // wave through its own scratch (at most ~256 MB in all)
+        int32_t* d_plist = nullptr;
+        int* d_pmax = nullptr;
+        HIP_TRY(c, wb.alloc(&d_plist, (size_t)H->n_pair));
+        HIP_TRY(c, wb.alloc(&d_pmax, 1));
+        HIP_TRY(c, hipMemsetAsync(d_pmax, 0, 4, s));
+        hipLaunchKernelGGL(k_pair_list, grid_for(n_wall, 256), 256, 0, s, d_pairf, d_pslot, d_nidx, d_wc, n_wall, d_plist,
+                           d_pmax);
+        int pmax = 0;
+        HIP_TRY(c, hipMemcpyAsync(&pmax, d_pmax, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        const int64_t per_wave = ((int64_t)pmax + 63) / 64 * 64;
+        int64_t waves = std::min<int64_t>((256ll << 20) / (per_wave * 10), 2048);
+        waves = std::max<int64_t>(std::min<int64_t>(waves, H->n_pair), 1);
+        const int64_t blocks = (waves + kDiagWaves - 1) / kDiagWaves;
+        uint16_t* d_sidx = nullptr;
+        uint64_t* d_sval = nullptr;
+        HIP_TRY(c, wb.alloc(&d_sidx, (size_t)(blocks * kDiagWaves * per_wave)));
+        HIP_TRY(c, wb.alloc(&d_sval, (size_t)(blocks * kDiagWaves * per_wave)));
+        if (H->d_val)
+            hipLaunchKernelGGL(k_pair_diag<true>, (unsigned)blocks, 64 * kDiagWaves, 0, s, H->d_win, d_plist,
+                               H->n_pair, H->d_idx16, H->d_val, H->d_hi, H->d_lo, d_sidx, d_sval, per_wave);
+        else
+            hipLaunchKernelGGL(k_pair_diag<false>, (unsigned)blocks, 64 * kDiagWaves, 0, s, H->d_win, d_plist,
+                               H->n_pair, H->d_idx16, H->d_val, H->d_hi, H->d_lo, d_sidx, d_sval, per_wave);
+        HIP_TRY(c, hipGetLastError());
+    }
     if (n_win > 0) {
         const unsigned ilg = std::min<unsigned>(grid_for(n_win, kIlWaves), 4096u);
         if (H->d_val)

@@ -391,13 +391,10 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
                 const bool beta_major = (int64_t)t >= h->n_task_a;
                 for (int32_t k = tasks[t].w_begin; k < tasks[t].w_end; ++k) {
                     const Window& b = wins[(size_t)k];
-                    // the stored half of a mirror pair holds both triangles'
-                    // pairs, in (rotated) fill order
+                    // the stored half of a mirror pair holds both triangles' pairs
                     const bool pair = b.sp() >= 0;
                     for (int64_t j = 0; j < b.cnt(); ++j) {
-                        // pair groups: slot 8 l + q of a chunk holds entry 8 l + (q + l) % 8
-                        const int64_t e =
-                            b.o() + (pair ? (j & ~7ll) + (((j & 7) - ((j >> 3) & 63)) & 7) : win_slot(j, b.cnt()));
+                        const int64_t e = b.o() + win_slot(j, b.cnt());
                         const uint32_t u = (uint32_t)ic[(size_t)e];
                         int32_t p = tasks[t].r0 + (int32_t)(u >> 8), q = b.c0 + (int32_t)(u & 0xFFu);
                         if (beta_major) {

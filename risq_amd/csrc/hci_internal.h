@@ -59,9 +59,11 @@ struct Window {
 // column), which go to a partial-sum buffer, slot sp x kWinCols, and are added
 // into the receiving tile's rows by a reduction after the product.  Windows
 // with at least kPairMinEntries entries are paired (builds without FP16
-// copies).  The stored half keeps its entries in fill order, each lane's
-// group rotated by the lane (slot 8 l + q of a chunk holds entry
-// 8 l + (q + l) % 8, not win_slot's): a group is 8 consecutive entries.
+// copies).  The stored half's entries are sorted by diagonal ((column - row)
+// mod 256, stable) before they move to win_slot's slots: the 64 consecutive
+// entries of one step of the product then lie on one or a few diagonals, on
+// distinct rows and distinct columns, so neither the row nor the column
+// accumulator sees many lanes on one address.
 constexpr int kPairMinEntries = 2048;
 // receiving tile of a run of slots: rows [r0, r0 + nr) of part `part`
 // receive the slots' partial sums, slots s_begin .. s_end of the sorted list

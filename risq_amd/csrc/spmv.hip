@@ -244,8 +244,9 @@ struct Oct {
             }
         }
     }
-    __device__ __forceinline__ double value(int k) const {
-        if (MODE == 3) return (double)(float)__builtin_bit_cast(_Float16, (uint16_t)q[k]);
+    // MODE 3 in FP32 (its accumulator type), the others in FP64
+    __device__ __forceinline__ typename std::conditional<MODE == 3, float, double>::type value(int k) const {
+        if (MODE == 3) return (float)__builtin_bit_cast(_Float16, (uint16_t)q[k]);
         if (MODE == 0) return d[k >> 1][k & 1];
         if (MODE == 1) return (double)h[k >> 2][k & 3];
         return (double)h[k >> 2][k & 3] + (double)l[k >> 2][k & 3];
@@ -253,49 +254,60 @@ struct Oct {
 };
 
 // x as the product reads it: FP64, or FP32 with FP16 values (MODE 3), whose
-// own rounding is 2^-11.  The LDS accumulators stay FP64: ds_add_f64 runs
-// the FP16-value product at 3.6 ms on the 2e6-determinant workload, ds_add_f32
-// at 18.8 ms.
+// own rounding is 2^-11
 template <int MODE>
 using XT = typename std::conditional<MODE == 3, float, double>::type;
+// the tile's row accumulators: FP64 for every value precision (ds_add_f64
+// runs the FP16-value product at 3.3 ms on the 2e6-determinant workload,
+// ds_add_f32 at 21 ms)
+template <int MODE>
+using AT = double;
 
-// H_pq x_q into row p's accumulator
-template <bool ABS, class X>
-__device__ __forceinline__ void add_entry(double* acc, const X* xw, uint32_t u, double v) {
-    if (ABS)
-        atomicAdd(&acc[u >> 8], fabs(v));
+// H_pq x_q for value k of a group: FP16 values times FP32 x in FP32 (exact
+// to 2^-24, far below the values' own 2^-11), the others in FP64
+template <int MODE, class X>
+__device__ __forceinline__ double term(const Oct<MODE>& o, int k, X xv) {
+    if constexpr (MODE == 3)
+        return (double)(o.value(k) * xv);
     else
-        atomicAdd(&acc[u >> 8], v * (double)xw[u & 0xFFu]);
+        return o.value(k) * (double)xv;
 }
 
-// A group of the stored half of a mirror pair (8 entries in fill order,
-// rotated, so columns come mostly in runs; slots past the window's entries
-// hold zeros): every entry goes into its row's accumulator, and H_pq x_p into
-// column q's transposed sum, added once per run of a column.  The x values of
-// four entries are read together before their adds.
-template <bool ABS, class O, class X>
-__device__ __forceinline__ void add_pair_group(double* acc, const X* xw, const X* xr, double* tw, const O& o) {
-    double run = 0.0;
+// One lane's group of 8 entries (those with bit q of `valid`): H_pq x_q into
+// row p's accumulator and, for the stored half of a mirror pair (PR), H_pq x_p
+// into column q's transposed sum.  The x reads of a batch of entries are
+// issued before its first product, so a wave keeps several LDS reads in
+// flight instead of waiting on each one.
+template <bool ABS, bool PR, int MODE, class X>
+__device__ __forceinline__ void add_group(double* acc, const X* xw, const X* xr, double* tw, const Oct<MODE>& o,
+                                          uint32_t valid) {
+    if (ABS) {
 #pragma unroll
-    for (int h = 0; h < 8; h += 4) {
-        X xc[4], xp[4];
-        if (!ABS) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                xc[q] = xw[o.u[h + q] & 0xFFu];
-                xp[q] = xr[o.u[h + q] >> 8];
+        for (int q = 0; q < 8; ++q)
+            if (valid >> q & 1u) {
+                const double a = fabs((double)o.value(q));
+                atomicAdd(&acc[o.u[q] >> 8], a);
+                if (PR) atomicAdd(&tw[o.u[q] & 0xFFu], a);
             }
+        return;
+    }
+    // batches of B reads: 8 with FP32 x and no transposed sums, 4 otherwise
+    // (the registers they hold keep the occupancy up)
+    constexpr int B = (MODE == 3 && !PR) ? 8 : 4;
+#pragma unroll
+    for (int h = 0; h < 8; h += B) {
+        X xc[B], xp[PR ? B : 1];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            xc[q] = xw[o.u[h + q] & 0xFFu];
+            if (PR) xp[q] = xr[o.u[h + q] >> 8];
         }
 #pragma unroll
-        for (int q = h; q < h + 4; ++q) {
-            const uint32_t u = o.u[q];
-            const double v = o.value(q);
-            atomicAdd(&acc[u >> 8], ABS ? fabs(v) : v * (double)xc[q - h]);
-            run += ABS ? fabs(v) : v * (double)xp[q - h];
-            const bool end = q == 7 || ((u ^ o.u[q + 1 < 8 ? q + 1 : q]) & 0xFFu) != 0;
-            if (end) atomicAdd(&tw[u & 0xFFu], run);
-            run = end ? 0.0 : run;
-        }
+        for (int q = 0; q < B; ++q)
+            if (valid >> (h + q) & 1u) {
+                atomicAdd(&acc[o.u[h + q] >> 8], term(o, h + q, xc[q]));
+                if (PR) atomicAdd(&tw[o.u[h + q] & 0xFFu], term(o, h + q, xp[PR ? q : 0]));
+            }
     }
 }
 
@@ -308,16 +320,16 @@ __global__ void __launch_bounds__(kTileThreads)
                const float* __restrict__ lo, const double* __restrict__ x, const double* __restrict__ diag,
                double* __restrict__ y, const double* __restrict__ xdiag, double* __restrict__ yb_out,
                const double* __restrict__ yb_in, const int32_t* __restrict__ binv, double* __restrict__ part) {
-    __shared__ double acc[kTileRows];
+    __shared__ AT<MODE> acc[kTileRows];
     __shared__ XT<MODE> xs[kTileWaves][kWinCols];
     __shared__ XT<MODE> xr[PAIRS ? kTileRows : 1];           // x of the tile's own rows (mirror pairs)
-    __shared__ double ts[kTileWaves][PAIRS ? kWinCols : 1];  // a wave's transposed sums (mirror pairs)
+    __shared__ AT<MODE> ts[kTileWaves][PAIRS ? kWinCols : 1];  // a wave's transposed sums (mirror pairs)
     __shared__ int next, next_small;
     const TileTask t = tasks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63;
     XT<MODE>* xw = xs[tid >> 6];
     const XT<MODE>* xx = (const XT<MODE>*)x;
-    for (int r = tid; r < kTileRows; r += kTileThreads) acc[r] = 0.0;
+    for (int r = tid; r < kTileRows; r += kTileThreads) acc[r] = 0;
     if (PAIRS && !ABS)
         for (int r = tid; r < t.nr; r += kTileThreads) xr[r] = xx[t.r0 + r];
     if (tid == 0) {
@@ -340,10 +352,7 @@ __global__ void __launch_bounds__(kTileThreads)
         Window Wn{};
         if (kn < n_w) Wn = wins[t.w_begin + kn];
         // full chunks: lane l's group of chunk c holds entries 512 c + 64 q + l;
-        // the tail of m entries: group g holds entries g, g + R, ..., g + 7R.
-        // The stored half of a mirror pair (sp >= 0) keeps fill order: lane
-        // l's group of chunk c holds entries 512 c + 8 l .. 512 c + 8 l + 7
-        // (the tail's groups too), at the same addresses.
+        // the tail of m entries: group g holds entries g, g + R, ..., g + 7R
         const int cnt = W.cnt(), nc = W.nc();
         const int n_full = cnt / kWinChunk;
         const int m = cnt - n_full * kWinChunk;
@@ -357,10 +366,7 @@ __global__ void __launch_bounds__(kTileThreads)
             o0.load(idx, val, hi, lo, mine);
         else if (lane < R)
             o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
-        if (n_full > 1)
-            o1.load(idx, val, hi, lo, mine + kWinChunk);
-        else if (sp >= 0 && n_full == 1 && lane < R)
-            o1.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+        if (n_full > 1) o1.load(idx, val, hi, lo, mine + kWinChunk);
         if (!ABS) {
             XT<MODE> xv[kWinCols / 64];
 #pragma unroll
@@ -372,51 +378,43 @@ __global__ void __launch_bounds__(kTileThreads)
             for (int j = 0; j < kWinCols / 64; ++j) xw[lane + 64 * j] = xv[j];
         }
         // the stored half of a mirror pair also sums its transposed products
-        double* tw = ts[PAIRS ? tid >> 6 : 0];
-        if (sp >= 0) {
+        AT<MODE>* tw = ts[PAIRS ? tid >> 6 : 0];
+        if (PAIRS && sp >= 0) {
 #pragma unroll
-            for (int j = 0; j < kWinCols / 64; ++j) tw[lane + 64 * j] = 0.0;
+            for (int j = 0; j < kWinCols / 64; ++j) tw[lane + 64 * j] = 0;
         }
         // the slice is written and read by this wave only: order its LDS
         // writes before the reads below
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (sp < 0) {
+        // the stream of one window; PR: a stored mirror half (its entries are
+        // sorted by diagonal, so the 64 entries of a step fall on distinct
+        // rows and distinct columns, mostly, for both accumulators)
+        auto stream = [&](auto pr_tag) {
+            constexpr bool PR = decltype(pr_tag)::value;
             int c = 0;
             for (; c + 1 < n_full; c += 2) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
-#pragma unroll
-                for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o1.u[q], o1.value(q));
+                add_group<ABS, PR>(acc, xw, xr, tw, o0, 0xFFu);
+                add_group<ABS, PR>(acc, xw, xr, tw, o1, 0xFFu);
                 if (c + 2 < n_full) o0.load(idx, val, hi, lo, mine + (int64_t)(c + 2) * kWinChunk);
                 if (c + 3 < n_full) o1.load(idx, val, hi, lo, mine + (int64_t)(c + 3) * kWinChunk);
             }
             if (c < n_full) {  // one full chunk left, in o0 (odd count) ...
-#pragma unroll
-                for (int q = 0; q < 8; ++q) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+                add_group<ABS, PR>(acc, xw, xr, tw, o0, 0xFFu);
                 if (lane < R) o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
             } else if (n_full > 0 && lane < R) {  // ... or none: the tail is still to load
                 o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
             }
             if (lane < R) {
+                uint32_t valid = 0;
 #pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (lane + q * R < m) add_entry<ABS>(acc, xw, o0.u[q], o0.value(q));
+                for (int q = 0; q < 8; ++q) valid |= (lane + q * R < m ? 1u : 0u) << q;
+                add_group<ABS, PR>(acc, xw, xr, tw, o0, valid);
             }
-        } else {
-            // chunks c and c + 1 in o0 and o1; the tail is chunk n_full, whose
-            // groups from lane R on hold no entries
-            const int n_grp = n_full + (m > 0 ? 1 : 0);
-            for (int c = 0; c < n_grp; c += 2) {
-                if (c < n_full || lane < R) add_pair_group<ABS>(acc, xw, xr, tw, o0);
-                if (c + 1 < n_full || (c + 1 == n_full && lane < R))
-                    add_pair_group<ABS>(acc, xw, xr, tw, o1);
-                if (c + 2 < n_full || (c + 2 == n_full && lane < R))
-                    o0.load(idx, val, hi, lo, mine + (int64_t)(c + 2) * kWinChunk);
-                if (c + 3 < n_full || (c + 3 == n_full && lane < R))
-                    o1.load(idx, val, hi, lo, mine + (int64_t)(c + 3) * kWinChunk);
-            }
+        };
+        if (PAIRS && sp >= 0) {
+            stream(std::true_type{});
             // the transposed sums to the pair's slot, for the receiving tile
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -424,8 +422,10 @@ __global__ void __launch_bounds__(kTileThreads)
 #pragma unroll
             for (int j = 0; j < kWinCols / 64; ++j) {
                 const int cc = lane + 64 * j;
-                if (cc < nc) part[(size_t)sp * kWinCols + cc] = tw[cc];
+                if (cc < nc) part[(size_t)sp * kWinCols + cc] = (double)tw[cc];
             }
+        } else {
+            stream(std::false_type{});
         }
         // reads of this slice precede the next window's staging
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -452,11 +452,11 @@ __global__ void __launch_bounds__(kTileThreads)
                 for (int q = 0; q < 8; ++q)
                     if (g + q * R < m) {
                         const uint32_t u = o.u[q];
-                        const double v = o.value(q);
+                        const AT<MODE> v = o.value(q);
                         if (ABS)
                             atomicAdd(&acc[u >> 8], fabs(v));
                         else
-                            atomicAdd(&acc[u >> 8], v * (double)xx[Ws.c0 + (int)(u & 0xFFu)]);
+                            atomicAdd(&acc[u >> 8], v * (AT<MODE>)xx[Ws.c0 + (int)(u & 0xFFu)]);
                     }
             }
         }
@@ -465,9 +465,10 @@ __global__ void __launch_bounds__(kTileThreads)
     for (int r = tid; r < t.nr; r += kTileThreads) {
         const int64_t p = (int64_t)t.r0 + r;
         if (yb_out) {  // beta-major rows, in their own order
-            yb_out[p] = acc[r];
+            yb_out[p] = (double)acc[r];
         } else {  // alpha-major rows, with the beta-major part's sum of the same row
-            double v = (diag && !ABS) ? diag[p] * xdiag[p] + acc[r] : acc[r];  // xdiag: x in FP64
+            const double a = (double)acc[r];
+            double v = (diag && !ABS) ? diag[p] * xdiag[p] + a : a;  // xdiag: x in FP64
             if (yb_in) v += yb_in[binv[p]];
             y[p] = v;
         }
@@ -618,7 +619,7 @@ void launch_spmv(const hci_csr* h, const double* x, double* y, bool absval) {
         if (h->n_task_b > 0 && !absval)
             hipLaunchKernelGGL(k_gather_x<XT<MODE>>, grid_for(h->n, 256), 256, 0, s, h->d_bpos,
                                (const XT<MODE>*)xa, h->n, (XT<MODE>*)xb);
-        // the stored halves of mirror pairs (never with FP16 copies) take
+        // a matrix with stored mirror halves (never with FP16 copies) takes
         // the instantiation with their code
         if (MODE != 3 && h->n_pair > 0)
             launch_win_parts<MODE, MODE != 3>(h, s, vals, xa, xb, x, y, absval);

@@ -495,8 +495,10 @@ def test_tiled_mirror_pairs(gpu_ctx, oracle_lib, prec):
     every sigma mode (FP64 and FP32 hi + lo within 1e-12; FP32 hi within its
     rounding), the row norms of the learned correction's features come out
     the same (the learned-correction Davidson reaches the FULL-storage E0).
-    Builds with FP16 copies store both halves."""
-    from risq_amd.hci import PREC_F16, PRECOND_MLP, SIGMA_F32
+    Builds with FP16 copies store both halves: download and hi + lo product
+    equal FULL storage's, the FP32 and FP16 products within their
+    roundings."""
+    from risq_amd.hci import PREC_F16, PRECOND_MLP, SIGMA_F16, SIGMA_F32
     from risq_amd import precond_train
     ints, dets = mol_space(40, 80, 1, 6, 6, 100000)
     gpu_ctx.set_integrals(ints)
@@ -522,6 +524,12 @@ def test_tiled_mirror_pairs(gpu_ctx, oracle_lib, prec):
     t.free()
     f16 = gpu_ctx.build_h(dets, diag, value_prec=PREC_F16, storage=STORE_TILED)
     assert f16.build_stats["mirror_pairs"] == 0 and f16.nnz == full.nnz
+    if prec == 32:
+        for u, v in zip(full.download(), f16.download()):
+            assert np.array_equal(u, v)
+        assert np.max(np.abs(f16.matrix_vector_prod(x) - a)) < 1e-12 * np.max(np.abs(a))
+        assert np.linalg.norm(f16.matrix_vector_prod(x, mode=SIGMA_F32) - a) / off < 1e-6
+        assert np.linalg.norm(f16.matrix_vector_prod(x, mode=SIGMA_F16) - a) / off < 2e-3
     f16.free()
     full.free()
 
