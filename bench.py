@@ -154,15 +154,17 @@ def host_cpu() -> dict:
     return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
 
-def cpu_baseline(ints, dets, n_full: int, nnz_full: int, fp64_iters: int | None) -> dict:
+def cpu_baseline(ints, dets, n_full: int, nnz_full: int, fp64_iters: int | None, full_matrix=None) -> dict:
     """The oracle (C++ restatement of the reference path, one core) on bounded
     prefixes of the same determinant list.  The build is timed at four prefix
     sizes and fitted as t = a N^b (its singles search is all-pairs, so b is
     near 2), then extrapolated to the full list; the product is timed on the
     largest prefix and charged per stored pair; Davidson is timed on that
-    prefix.  The full-size time to E0 is extrapolated as build(N) +
-    iterations x product(N), with the iteration count of the plain FP64 GPU
-    path (the same control flow)."""
+    prefix.  With `full_matrix` (a callable returning the full list's
+    caller-order upper CSR, built on the GPU) the oracle's product is also
+    timed at full size, one core.  The full-size time to E0 is extrapolated
+    as build(N) + iterations x product(N), with the iteration count of the
+    plain FP64 GPU path (the same control flow)."""
     from oracle import pyoracle
     H = pyoracle.OracleHam(ints)
     sizes = [n for n in (5000, 10000, 20000, 30000) if n <= dets.shape[0]]
@@ -197,17 +199,39 @@ def cpu_baseline(ints, dets, n_full: int, nnz_full: int, fp64_iters: int | None)
         "davidson_converged": rc == 0,
         "build_sweep": [{"n": a, "s": round(b, 4), "nnz_u": c} for a, b, c in builds],
     }
+    t_prod_measured = None
+    if full_matrix is not None:
+        rp, col, val, dg = full_matrix()
+        xf = np.random.default_rng(1).standard_normal(n_full)
+        yf = np.zeros(n_full)
+        L = pyoracle.lib()
+        reps = []
+        for _ in range(2):
+            t = time.perf_counter()
+            L.or_spmv_csr(n_full, rp.ctypes.data, col.ctypes.data, val.ctypes.data, dg.ctypes.data, xf.ctypes.data,
+                          yf.ctypes.data)
+            reps.append(time.perf_counter() - t)
+        t_prod_measured = min(reps)
+        del rp, col, val, dg
+        out["full_size_spmv"] = {
+            "n": n_full, "nnz_u": nnz_full, "s_per_product": round(t_prod_measured, 3),
+            "pairs_per_s": nnz_full / t_prod_measured,
+            "gbps": spmv_bytes(nnz_full, n_full, 8) / t_prod_measured / 1e9,
+            "what": "the oracle's upper-triangle product (src/var/sparse.rs:64-78), one core, over the full "
+                    "list's matrix (built on the GPU, FULL storage, downloaded in caller order)"}
+        log(f"cpu full-size product: {t_prod_measured:.2f}s")
     if len(builds) >= 2:
         lx = np.log([b[0] for b in builds])
         ly = np.log([b[1] for b in builds])
         slope, icpt = np.polyfit(lx, ly, 1)
         t_build_full = float(np.exp(icpt) * n_full ** slope)
-        t_prod_full = nnz_full / out["spmv_nnz_per_s"]
+        t_prod_full = t_prod_measured if t_prod_measured is not None else nnz_full / out["spmv_nnz_per_s"]
         out["extrapolated"] = {
             "label": "extrapolated, not measured",
             "n": n_full, "build_fit_exponent": round(float(slope), 3),
             "build_s": round(t_build_full, 1),
             "product_s": round(t_prod_full, 3),
+            "product_measured_at_full_size": t_prod_measured is not None,
             "davidson_iters_assumed": fp64_iters,
             "s_to_E0": None if fp64_iters is None else round(t_build_full + fp64_iters * t_prod_full, 1),
         }
@@ -224,6 +248,8 @@ def main() -> int:
     ap.add_argument("--n-dets", type=int, default=None,
                     help="determinants of the energy-truncated space (workloads 1 and 4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-cpu-full", action="store_true",
+                    help="skip the CPU baseline's full-size product (workload 1: FULL build and download)")
     ap.add_argument("--no-hci-space", action="store_true",
                     help="workload 1: skip the run on the HCI-selected space of the same integrals")
     ap.add_argument("--no-compare", action="store_true",
@@ -344,6 +370,7 @@ def main() -> int:
                    "davidson": f"mixed precision, subspace {args.max_sub}, restart keeps {args.keep} vectors, "
                                f"correction={args.precond}",
                    "parallelism": "single GPU"},
+        "matrix_hbm_gb": round(h.build_stats["matrix_bytes"] / 1e9, 2),
         "E0": results[-1].energy,
         "davidson": {"iters": last["iters"], "n_sigma_f16": n16, "f16_iter": last["f16_iter"],
                      "n_sigma_f32": n32, "n_sigma_full": nfull,
@@ -422,7 +449,17 @@ def main() -> int:
         out["hci_selected_space"] = hci_space_run(ctx, ints, failures)
     if rank == 0 and not args.no_cpu:
         log("cpu baseline (oracle, 1 core, bounded sample)...")
-        out["cpu_baseline"] = cpu_baseline(ints, dets, n, nnz, fp64_iters)
+        full_matrix = None
+        if args.config == 1 and not args.no_cpu_full:
+            def full_matrix():
+                from risq_amd.hci import STORE_FULL
+                hf = ctx.build_h(d_dets, d_diag, n=n, storage=STORE_FULL)
+                t = time.perf_counter()
+                m = hf.download()
+                log(f"full matrix downloaded in caller order ({time.perf_counter() - t:.1f}s)")
+                hf.free()
+                return m
+        out["cpu_baseline"] = cpu_baseline(ints, dets, n, nnz, fp64_iters, full_matrix)
     if failures:
         out["error"] = "; ".join(failures)
     print(json.dumps(out), flush=True)

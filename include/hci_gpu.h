@@ -115,6 +115,11 @@ typedef struct hci_build_stats {
     int64_t alpha_pairs;    /* single-connected alpha-string pairs (ordered) */
     double seconds;         /* wall time of the call (device work included) */
     int64_t mirror_pairs;   /* HCI_STORE_TILED: dense window pairs stored as one half */
+    int64_t stored_entries; /* entry slots held on the device (both triangles for FULL
+                               and TILED, TILED's alignment slots included) */
+    int64_t matrix_bytes;   /* device bytes of the entry arrays: indices and values
+                               (FP64, FP32 hi + lo, FP16 copies), plus FULL/UPPER's
+                               row pointers */
 } hci_build_stats;
 
 #define HCI_DAV_MAX_SUB 32

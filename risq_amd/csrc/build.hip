@@ -2056,6 +2056,9 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
             stats->n_beta = Ub;
             stats->alpha_pairs = n_pairs;
             stats->mirror_pairs = H->n_pair;
+            stats->stored_entries = H->nnz_alloc;
+            const int64_t vb = o.value_prec == HCI_PREC_F16 ? 10 : 8;
+            stats->matrix_bytes = (int64_t)H->entry_cap * (2 + vb);
         }
         *out = H;
         return HCI_OK;
@@ -2179,6 +2182,8 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
         stats->n_beta = Ub;
         stats->alpha_pairs = n_pairs;
         stats->mirror_pairs = 0;
+        stats->stored_entries = H->nnz_s;
+        stats->matrix_bytes = H->nnz_s * (4 + 8) + (int64_t)(N + 1) * 8;
     }
     *out = H;
     return HCI_OK;

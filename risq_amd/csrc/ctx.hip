@@ -1,6 +1,7 @@
 // Context, integral tables, per-determinant and per-pair element entry points.
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 #include "hci_internal.h"
 #include "mlp.h"
@@ -353,6 +354,60 @@ int fetch_internal(const hci_csr* h, std::vector<int64_t>& rp, std::vector<int32
     }
     return HCI_OK;
 }
+// FULL storage: internal row p holds both triangles of caller row perm[p],
+// so the caller's upper row perm[p] is exactly row p's entries with
+// perm[col] > perm[p]: rows are counted, placed and sorted on host threads,
+// each thread a contiguous range of internal rows (no two write one row)
+int download_full(const hci_csr* h, const std::vector<int64_t>& rp, const std::vector<int32_t>& ic,
+                  const std::vector<double>& iv, const std::vector<int32_t>& perm, int64_t* row_ptr, int32_t* col,
+                  double* val, double* diag) {
+    hci_ctx* c = h->ctx;
+    const int64_t n = h->n;
+    const int nt = (int)std::max<unsigned>(1, std::min<unsigned>(32, std::thread::hardware_concurrency()));
+    auto parallel = [&](auto f) {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back([&, t]() { f(n * t / nt, n * (t + 1) / nt); });
+        for (auto& x : th) x.join();
+    };
+    std::vector<int64_t> cnt((size_t)n + 1, 0);
+    parallel([&](int64_t p0, int64_t p1) {
+        for (int64_t p = p0; p < p1; ++p) {
+            const int32_t i = perm[p];
+            int64_t k = 0;
+            for (int64_t e = rp[p]; e < rp[p + 1]; ++e) k += perm[ic[e]] > i;
+            cnt[(size_t)i + 1] = k;
+        }
+    });
+    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    if (row_ptr) std::copy(cnt.begin(), cnt.end(), row_ptr);
+    parallel([&](int64_t p0, int64_t p1) {
+        std::vector<std::pair<int32_t, double>> row;
+        for (int64_t p = p0; p < p1; ++p) {
+            const int32_t i = perm[p];
+            row.clear();
+            for (int64_t e = rp[p]; e < rp[p + 1]; ++e) {
+                const int32_t j = perm[ic[e]];
+                if (j > i) row.emplace_back(j, iv[e]);
+            }
+            std::sort(row.begin(), row.end(),
+                      [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+                          return a.first < b.first;
+                      });
+            int64_t o = cnt[i];
+            for (const auto& q : row) {
+                if (col) col[o] = q.first;
+                if (val) val[o] = q.second;
+                ++o;
+            }
+        }
+    });
+    if (diag && n) {
+        std::vector<double> d((size_t)n);
+        HIP_TRY(c, hipMemcpy(d.data(), h->d_diag, (size_t)n * 8, hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < n; ++p) diag[perm[p]] = d[p];
+    }
+    return HCI_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -384,6 +439,7 @@ int hci_csr_download(const hci_csr* h, int64_t* row_ptr, int32_t* col, double* v
                                  hipMemcpyDeviceToHost));
         HIP_TRY(c, hipMemcpy(bpos.data(), h->d_bpos, (size_t)n * 4, hipMemcpyDeviceToHost));
     }
+    if (h->storage == HCI_STORE_FULL) return download_full(h, rp, ic, iv, perm, row_ptr, col, val, diag);
     std::vector<int64_t> cnt((size_t)n + 1, 0);
     auto each = [&](auto f) {
         if (h->storage == HCI_STORE_TILED) {

@@ -585,10 +585,9 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             converged = true;
             break;
         }
-        // an MLP correction is judged by what it did to the residual: after two
-        // steps that did not reduce it, the diagonal correction takes over
-        // the step after a network correction that did not lower the residual
-        // norm takes the diagonal correction; the network is tried again after it
+        // an MLP correction is judged by what it did to the residual: the step
+        // after a network correction that did not lower the residual norm
+        // takes the diagonal correction; the network is tried again after it
         const bool failed = net && mlp_prev && !(err < err_prev);
         if (failed) ++st.mlp_fallbacks;
         const bool mlp_now = net && !failed;

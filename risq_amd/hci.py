@@ -53,7 +53,8 @@ class BuildOpts(C.Structure):
 
 class BuildStats(C.Structure):
     _fields_ = [("nnz_u", C.c_int64), ("n_alpha", C.c_int64), ("n_beta", C.c_int64),
-                ("alpha_pairs", C.c_int64), ("seconds", C.c_double), ("mirror_pairs", C.c_int64)]
+                ("alpha_pairs", C.c_int64), ("seconds", C.c_double), ("mirror_pairs", C.c_int64),
+                ("stored_entries", C.c_int64), ("matrix_bytes", C.c_int64)]
 
 
 class DavOpts(C.Structure):
