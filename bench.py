@@ -16,8 +16,9 @@ root, residual 1e-4 and energy change 1e-8).  Workloads (--config):
                (1.08e7 determinants), built before the timed region.
   4            n_orb=80, 10 alpha + 10 beta electrons, the same molecular
                recipe with K = 160 (seed 4), energy-truncated space of
-               --n-dets determinants (default 8e6, the largest whose FP32
-               hi+lo H fits one GPU's HBM next to the build's scratch).
+               --n-dets determinants (default 1.2e7: its FP32 hi+lo H, dense
+               mirror window pairs stored once, takes 272 GB of the 288 GB;
+               1.6e7 does not fit).
 
 Timed path (value): tiled storage with FP32 hi+lo values (plus an FP16 copy of
 each value for workload 1: --values), Davidson with FP16 sigmas, then FP32
@@ -87,7 +88,7 @@ def make_inputs(config: int, n_dets: int | None, ctx):
         return ints, dets, ("Hubbard 6x6 periodic, t=1, U=4, disorder 0.1 (seed 2), 18a18b, "
                             f"HCI space after 10 iterations at eps_var=1e-3, {dets.shape[0]} dets")
     if config == 4:
-        n = n_dets or 8_000_000
+        n = n_dets or 12_000_000
         ints, dets = molecular_inputs(n, 80, 10, 160, 4)
         return ints, dets, ("molecular n_orb=80, 10a10b, K=160, seed 4, energy-truncated space, "
                             f"{dets.shape[0]} dets")
