@@ -38,6 +38,8 @@ constexpr int kMaxSub = HCI_DAV_MAX_SUB;
 constexpr int kMaxKeep = HCI_DAV_MAX_KEEP;
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
+constexpr int kRitzSlot = 2 + 2 * kMaxSub;               // k_ritz_dots' values at KM = kMaxSub
+constexpr int kRedValues = 3 * (kMaxSub + 1) + kRitzSlot;  // the device reduction slots
 
 struct Coef {
     double c[kMaxSub];
@@ -133,6 +135,47 @@ __global__ void __launch_bounds__(kRedThreads)
         acc[1] += tv * tv;
     }
     block_store<2>(acc, 2, partial);
+}
+
+// k_ritz that also sums V^T t and W^T t (W = H V) while V and W pass through
+// its registers: partial slots r.r, t.t, V[c].t (2 + c), W[c].t (2 + KM + c).
+// With the diagonal correction these give the Gram-Schmidt coefficients of t
+// and, through W^T t' = W^T t - (V^T W) c, the new column of the projected
+// matrix, so neither needs a pass over V of its own
+template <int KM>
+__global__ void __launch_bounds__(kRedThreads)
+    k_ritz_dots(const double* __restrict__ V, const double* __restrict__ W, int64_t ld, int k, Coef y,
+                double lam, const double* __restrict__ d, int64_t n, double* x, double* r, double* t,
+                double* partial) {
+    double acc[2 + 2 * KM];
+#pragma unroll
+    for (int c = 0; c < 2 + 2 * KM; ++c) acc[c] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double vc[KM], wc[KM];
+        double xv = 0.0, wv = 0.0;
+#pragma unroll
+        for (int c = 0; c < KM; ++c)
+            if (c < k) {
+                vc[c] = V[(size_t)c * ld + i];
+                wc[c] = W[(size_t)c * ld + i];
+                xv += vc[c] * y.c[c];
+                wv += wc[c] * y.c[c];
+            }
+        const double rv = wv - xv * lam;
+        const double tv = rv / (lam - d[i]);
+        if (x) x[i] = xv;
+        if (r) r[i] = rv;
+        t[i] = tv;
+        acc[0] += rv * rv;
+        acc[1] += tv * tv;
+#pragma unroll
+        for (int c = 0; c < KM; ++c)
+            if (c < k) {
+                acc[2 + c] += vc[c] * tv;
+                acc[2 + KM + c] += wc[c] * tv;
+            }
+    }
+    block_store<2 + 2 * KM>(acc, 2 + 2 * KM, partial);
 }
 
 // t -= sum_c V[c] h[c]; then partial dots of V[c].t (c < k) and t.t (slot k); k <= KM.
@@ -279,17 +322,19 @@ struct DavWork {
     double* r = nullptr;
     double* t = nullptr;
     double* partial = nullptr;
-    double* red = nullptr;    // device reductions: 3 slots of kMaxSub + 1 values
-    double* h_red = nullptr;  // pinned host copy of reductions (same layout)
+    // device reductions: 3 slots of kMaxSub + 1 values, then k_ritz_dots'
+    // kRitzSlot values; h_red is their pinned host copy (same layout)
+    double* red = nullptr;
+    double* h_red = nullptr;
     // carves every buffer out of the context's work block (kept between calls)
     int carve(int nsub) {
         const size_t vb = ((size_t)n * 8 + 255) / 256 * 256;
         const size_t sizes[] = {vb * (size_t)nsub, vb * (size_t)nsub, vb, vb, vb,
-                                (size_t)kMaxSub * kMaxKeep * 8, (size_t)kRedBlocks * (kMaxSub + 1) * 8,
-                                (size_t)3 * (kMaxSub + 1) * 8};
+                                (size_t)kMaxSub * kMaxKeep * 8, (size_t)kRedBlocks * kRitzSlot * 8,
+                                (size_t)kRedValues * 8};
         size_t total = 0;
         for (size_t b : sizes) total += (b + 255) / 256 * 256;
-        const size_t host = (size_t)kMaxSub * kMaxKeep * 8 + 3 * (kMaxSub + 1) * 8 + 256;
+        const size_t host = (size_t)kMaxSub * kMaxKeep * 8 + (size_t)kRedValues * 8 + 256;
         if (int rc = work_reserve(c, total, host)) return rc;
         double** dst[] = {&V, &W, &x, &r, &t, &q, &partial, &red};
         char* p = (char*)c->work_dev;
@@ -314,6 +359,14 @@ struct DavWork {
     int reduce(int k) {
         hipLaunchKernelGGL(k_reduce, k, kRedThreads, 0, c->stream, partial, (int)red_grid(), k, red);
         HIP_TRY(c, hipMemcpyAsync(h_red, red, (size_t)k * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return HCI_OK;
+    }
+    // k_ritz_dots' values into their own slot, copied to the host
+    int reduce_ritz(int k) {
+        double* dst = red + 3 * (kMaxSub + 1);
+        hipLaunchKernelGGL(k_reduce, k, kRedThreads, 0, c->stream, partial, (int)red_grid(), k, dst);
+        HIP_TRY(c, hipMemcpyAsync(h_red + 3 * (kMaxSub + 1), dst, (size_t)k * 8, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         return HCI_OK;
     }
@@ -568,17 +621,29 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         // the diagonal correction t goes straight to the next basis column
         const bool want_xr = net || rec;
         double* t_out = (!net && dim_sub < max_sub) ? w.col(w.V, dim_sub) : w.t;
-        HCI_LAUNCH_KM(k_ritz, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam, h->d_diag, N,
-                      want_xr ? w.x : nullptr, want_xr ? w.r : nullptr, t_out, w.partial);
-        DT(w.reduce(2));
+        // diagonal-correction steps (no network, no recorder) take V^T t and
+        // W^T t from the Ritz pass itself: rd[2 + c] and rd[2 + km + c]
+        const bool fused = !net && !rec && dim_sub < max_sub;
+        const int km = w.max_sub <= 8 ? 8 : w.max_sub <= 16 ? 16 : 32;
+        const double* rd = w.h_red + 3 * (kMaxSub + 1);
+        if (fused) {
+            HCI_LAUNCH_KM(k_ritz_dots, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam,
+                          h->d_diag, N, nullptr, nullptr, t_out, w.partial);
+            DT(w.reduce_ritz(2 + 2 * km));
+        } else {
+            HCI_LAUNCH_KM(k_ritz, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam, h->d_diag,
+                          N, want_xr ? w.x : nullptr, want_xr ? w.r : nullptr, t_out, w.partial);
+            DT(w.reduce(2));
+        }
+        const double rr = fused ? rd[0] : w.h_red[0], tt = fused ? rd[1] : w.h_red[1];
         auto ensure_x = [&]() {
             if (!want_xr)
                 HCI_LAUNCH_KM(k_ritz, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam,
                               h->d_diag, N, w.x, w.r, w.t, w.partial);
             return hipGetLastError() == hipSuccess ? HCI_OK : hci_fail(c, HCI_ERR_HIP, "k_ritz launch failed");
         };
-        err = std::sqrt(w.h_red[0]);
-        double tnorm = std::sqrt(w.h_red[1]);
+        err = std::sqrt(rr);
+        double tnorm = std::sqrt(tt);
         const bool conv = N <= 2 || (err < opts->tol && std::fabs(lam - last_eig) < opts->etol);
         if (conv && sigma_mode == HCI_SIGMA_FULL) {
             DT(ensure_x());
@@ -634,18 +699,29 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             double* rA = w.red;
             double* rB = w.red + kSlot;
             double* rC = w.red + 2 * kSlot;
-            HCI_LAUNCH_KM(k_multidot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub + 1, t, N,
-                          w.partial);
-            w.reduce_dev(dim_sub + 1, rA);
             Coef hc{};
-            HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rA, t, N,
-                          w.partial);
-            w.reduce_dev(dim_sub + 1, rB);
-            HIP_TRY(c, hipMemcpyAsync(w.h_red, w.red, 2 * kSlot * 8, hipMemcpyDeviceToHost, s));
+            if (fused) {
+                // V^T t from the Ritz pass: the first pass needs no dot pass
+                for (int q = 0; q < dim_sub; ++q) hc.c[q] = rd[2 + q];
+                HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, nullptr,
+                              t, N, w.partial);
+                w.reduce_dev(dim_sub + 1, rB);
+                w.h_red[dim_sub] = tt;  // slot A's t.t
+                HIP_TRY(c, hipMemcpyAsync(w.h_red + kSlot, rB, kSlot * 8, hipMemcpyDeviceToHost, s));
+            } else {
+                HCI_LAUNCH_KM(k_multidot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub + 1, t, N,
+                              w.partial);
+                w.reduce_dev(dim_sub + 1, rA);
+                HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rA, t,
+                              N, w.partial);
+                w.reduce_dev(dim_sub + 1, rB);
+                HIP_TRY(c, hipMemcpyAsync(w.h_red, w.red, 2 * kSlot * 8, hipMemcpyDeviceToHost, s));
+            }
             HIP_TRY(c, hipStreamSynchronize(s));
             if (mlp_now) tnorm = std::sqrt(w.h_red[dim_sub]);
             double nrm = std::sqrt(w.h_red[kSlot + dim_sub]);
-            if (!(nrm * nrm >= 0.5 * w.h_red[dim_sub])) {
+            const bool second = !(nrm * nrm >= 0.5 * w.h_red[dim_sub]);
+            if (second) {
                 HCI_LAUNCH_KM(k_orth_sub_dot, w.max_sub, w.red_grid(), kRedThreads, s, w.V, N, dim_sub, hc, rB, t,
                               N, w.partial);
                 w.reduce_dev(dim_sub + 1, rC);
@@ -666,14 +742,29 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
             }
             hipLaunchKernelGGL(k_scale, grid_for(N, 256), 256, 0, s, t, 1.0 / nrm, t, N);
             DT(sigma(t, w.col(w.W, dim_sub)));
-            DT(w.multidot(w.V, dim_sub + 1, w.col(w.W, dim_sub)));
             const int nd = dim_sub + 1;
+            std::vector<double> pcol((size_t)nd);
+            if (fused && !second) {
+                // V^T w_new = W^T v_new = (W^T t - (V^T W) c) / |t'| for the old
+                // basis (W = H V, H symmetric; c = V^T t, the one pass's
+                // coefficients); only v_new . w_new needs a pass (two vectors)
+                for (int r = 0; r < dim_sub; ++r) {
+                    double a = rd[2 + km + r];
+                    for (int q = 0; q < dim_sub; ++q) a -= P[(size_t)r * dim_sub + q] * hc.c[q];
+                    pcol[r] = a / nrm;
+                }
+                DT(w.multidot(t, 1, w.col(w.W, dim_sub)));
+                pcol[dim_sub] = w.h_red[0];
+            } else {
+                DT(w.multidot(w.V, dim_sub + 1, w.col(w.W, dim_sub)));
+                for (int r = 0; r < nd; ++r) pcol[r] = w.h_red[r];
+            }
             std::vector<double> np((size_t)nd * nd, 0.0);
             for (int r = 0; r < dim_sub; ++r)
                 for (int q = 0; q < dim_sub; ++q) np[(size_t)r * nd + q] = P[(size_t)r * dim_sub + q];
             for (int r = 0; r < nd; ++r) {
-                np[(size_t)r * nd + dim_sub] = w.h_red[r];
-                np[(size_t)dim_sub * nd + r] = w.h_red[r];
+                np[(size_t)r * nd + dim_sub] = pcol[r];
+                np[(size_t)dim_sub * nd + r] = pcol[r];
             }
             P.swap(np);
             dim_sub = nd;
