@@ -384,6 +384,48 @@ int hci_semistoch_dtm(hci_ctx* ctx, const uint64_t* dets, const double* coeffs, 
                       double e0, double eps, double* e2, int64_t* n_ext, double* rel_probs,
                       hci_det_orb_sample* samples, int64_t cap, int64_t* n_samples, int mem);
 
+/* The external products that also set up a screened sampler: replaces the
+ * other Wf::approx_matmul_external_* variants (src/wf/mod.rs), selected by
+ * `variant` (their callers: src/wf/operations.rs:85-111,
+ * src/semistoch/mod.rs:69-73,555).  For each determinant i (in order) with
+ * local = eps / |c_i|, the lists of hci_excite_init are walked in the order
+ * of valence_elecs_and_epairs (opposite-spin pairs, same-spin pairs up then
+ * down, electrons up then down) and every candidate target outside `dets`
+ * receives H_ai c_i; a list that ends below local yields a DetOrbSample record
+ * (hci_det_orb_sample, src/stoch/mod.rs:104-124).
+ *   HCI_EXT_SEPARATE (separate_doubles_and_singles, :341-533): a kind's lists
+ *     are walked only when its maximum is >= local; each is cut at its first
+ *     entry below local (record), valid or not; valid entries before the cut
+ *     are candidates, singles included without a test of their element.
+ *   HCI_EXT_SKIP_SINGLES (skip_singles, :535-711): doubles only; occupied
+ *     targets skipped first; entries >= local are candidates; the first entry
+ *     below local whose target is outside the space ends the list (record);
+ *     entries below local inside the space are passed over.
+ *   HCI_EXT_SEMISTOCH_SINGLES (semistoch_singles, :714-1033): doubles as
+ *     SKIP_SINGLES; singles lists walked when max_sing >= local: a valid
+ *     external entry >= local is a candidate when |H_ai| >= local, otherwise
+ *     the list's first record; below local the list ends if it has a record,
+ *     else at its first valid external entry with a record whose four sums
+ *     are scaled by 1e-6 (single_sample_factor).  The oversampling statistics
+ *     the reference prints are not computed.
+ *   HCI_EXT_NO_SINGLES (no_singles, :1035-1193): doubles as SEPARATE but
+ *     without the maxima test; every valence electron gives a record from the
+ *     first entry of its singles list (fails with HCI_ERR_ARG when those
+ *     lists are empty, where the reference's index panics).
+ * Outputs as hci_matmul_external (first-discovery order, first discoverer's
+ * new_diag in out_diag); records in the reference's push order, at most scap
+ * (samples may be NULL: *n_samples still receives the count).  A capacity
+ * too small for either fails the call with *count / *n_samples set and
+ * nothing else written. */
+#define HCI_EXT_SEPARATE 1
+#define HCI_EXT_SKIP_SINGLES 2
+#define HCI_EXT_SEMISTOCH_SINGLES 3
+#define HCI_EXT_NO_SINGLES 4
+int hci_matmul_external_sampled(hci_ctx* ctx, int variant, const uint64_t* dets, const double* coeffs,
+                                const double* diag, int64_t n, double eps, uint64_t* out, double* out_coeffs,
+                                double* out_diag, int64_t cap, int64_t* count, hci_det_orb_sample* samples,
+                                int64_t scap, int64_t* n_samples, int mem);
+
 /* Device-memory and timing plumbing (no reference counterpart: for callers
  * that keep inputs resident in HBM, and for timing on the context's stream).
  * Events are 16 slots per context recorded on the context stream. */

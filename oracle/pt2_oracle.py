@@ -391,3 +391,143 @@ def semistoch_dtm(ints, dets: np.ndarray, coeffs: np.ndarray, e0: float, eps: fl
         for num, d in zip(nums, H.ham_diag(words)):
             e2 += num * num / (e0 - d)
     return words, nums, [(psi[t][1], psi[t][2]) for t in keys], rel, records, e2
+
+
+def matmul_external_sampled(ints, dets: np.ndarray, coeffs: np.ndarray, eps: float, n_core: int, n_up: int,
+                            n_dn: int, variant: str):
+    """The external products that set up a screened sampler, restated from
+    src/wf/mod.rs: `variant` is "separate_doubles_and_singles" (:341-533),
+    "skip_singles" (:535-711), "semistoch_singles" (:714-1033) or
+    "no_singles" (:1035-1193).  For each determinant (in order) with
+    local = eps / |c|, its lists in the order of valence_elecs_and_epairs
+    (opposite-spin pairs, same-spin pairs up then down, electrons up then
+    down) are walked as the named function walks them; candidates are the
+    targets outside the space, summed per target in discovery order
+    (add_det_with_coeff, :144-165), and every DetOrbSample pushed is kept as
+    (i, kind, is_alpha, p, q, sum_abs_h, sum_h_squared, sum_abs_hc,
+    sum_hc_squared) in push order.  Returns (external words (k, 4),
+    coefficients, first discoverers (i, excitation), records).  Elements from
+    the C++ oracle."""
+    valence, opp, same, sing, max_opp, max_same, max_sing = _excite_lists(ints, n_core, n_up, n_dn)
+    H = pyoracle.OracleHam(ints)
+    space = {(int(w[0]) | int(w[1]) << 64, int(w[2]) | int(w[3]) << 64) for w in dets}
+    cut_first = variant in ("separate_doubles_and_singles", "no_singles")
+    gate = variant == "separate_doubles_and_singles"  # lists entered only when a kind's maximum >= local
+    factor = 1e-6  # single_sample_factor (:895)
+    records = []
+    cand = []  # (target, source index, excitation), in discovery order
+
+    def elem(src_w, t):
+        v, _ = H.off_diag(np.asarray(src_w, dtype=np.uint64)[None, :], _words([t[0]], [t[1]]))
+        return float(v[0])
+
+    def add(t, k, exc):
+        if t not in space:
+            cand.append((t, k, exc))
+
+    for k, w in enumerate(dets):
+        up, dn, c = int(w[0]) | int(w[1]) << 64, int(w[2]) | int(w[3]) << 64, float(coeffs[k])
+        local = eps / abs(c) if c != 0.0 else np.inf
+
+        def record(kind, alpha, i, j, rab, rsq, scale=1.0):
+            if scale == 1.0:
+                records.append((k, kind, alpha, i, j, rab, rsq, abs(c) * rab, c * c * rsq))
+            else:
+                records.append((k, kind, alpha, i, j, rab * scale, rsq * scale, abs(c) * rab * scale,
+                                c * c * rsq * scale))
+
+        doubles = []  # (kind, alpha, i, j, list, occupied test, target)
+        for i in _bits_asc(valence & up):
+            for j in _bits_asc(valence & dn):
+                doubles.append((0, -1, i, j, opp[(i, j)],
+                                lambda t: (up >> t[0]) & 1 or (dn >> t[1]) & 1,
+                                lambda t, i=i, j=j: (up ^ (1 << i) | (1 << t[0]), dn ^ (1 << j) | (1 << t[1]))))
+        for cfg, alpha in ((up, 1), (dn, 0)):
+            b = list(_bits_asc(valence & cfg))
+            for x in range(len(b)):
+                for y in range(x + 1, len(b)):
+                    i, j = b[x], b[y]
+
+                    def mk(t, i=i, j=j, cfg=cfg, alpha=alpha):
+                        nw = cfg ^ (1 << i) ^ (1 << j) | (1 << t[0]) | (1 << t[1])
+                        return (nw, dn) if alpha else (up, nw)
+                    doubles.append((1, alpha, i, j, same[(i, j)],
+                                    lambda t, cfg=cfg: (cfg >> t[0]) & 1 or (cfg >> t[1]) & 1, mk))
+        for kind, alpha, i, j, lst, occupied, make in doubles:
+            if gate and (max_opp if kind == 0 else max_same) < local:
+                continue
+            for a, tgt, rsq, rab in lst:
+                if cut_first and a < local:
+                    record(kind, alpha, i, j, rab, rsq)
+                    break
+                if occupied(tgt):
+                    continue
+                t = make(tgt)
+                if not cut_first and a < local:
+                    if t in space:
+                        continue
+                    record(kind, alpha, i, j, rab, rsq)
+                    break
+                add(t, k, (kind, alpha, i, j, tgt))
+        if variant == "skip_singles":
+            continue
+        for cfg, alpha in ((up, 1), (dn, 0)):
+            for i in _bits_asc(valence & cfg):
+                lst = sing[i]
+
+                def make1(r, i=i, cfg=cfg, alpha=alpha):
+                    nw = cfg ^ (1 << i) | (1 << r)
+                    return (nw, dn) if alpha else (up, nw)
+                if variant == "no_singles":
+                    _, _, rsq, rab = lst[0]
+                    record(2, alpha, i, -1, rab, rsq)
+                    continue
+                if max_sing < local:
+                    continue
+                stored = False
+                for a, r, rsq, rab in lst:
+                    if variant == "separate_doubles_and_singles":
+                        if a < local:
+                            record(2, alpha, i, -1, rab, rsq)
+                            break
+                        if (cfg >> r) & 1:
+                            continue
+                        t = make1(r)
+                        add(t, k, (2, alpha, i, -1, r))
+                        continue
+                    # semistoch_singles
+                    if a >= local:
+                        if (cfg >> r) & 1:
+                            continue
+                        t = make1(r)
+                        if t in space:
+                            continue
+                        hs = elem(w, t)
+                        if abs(hs) >= local:
+                            add(t, k, (2, alpha, i, -1, r))
+                        elif not stored:
+                            record(2, alpha, i, -1, rab, rsq)
+                            stored = True
+                    elif stored:
+                        break
+                    else:
+                        if (cfg >> r) & 1:
+                            continue
+                        t = make1(r)
+                        if t in space:
+                            continue
+                        record(2, alpha, i, -1, rab, rsq, factor)
+                        break
+    out: dict[tuple[int, int], list] = {}
+    if cand:
+        src = np.array([dets[k] for _, k, _ in cand], dtype=np.uint64)
+        h, _ = H.off_diag(src, _words([t[0] for t, _, _ in cand], [t[1] for t, _, _ in cand]))
+        for (t, k, exc), hv in zip(cand, h):
+            x = float(hv) * float(coeffs[k])
+            if t in out:
+                out[t][0] += x
+            else:
+                out[t] = [x, k, exc]
+    keys = list(out)
+    words = _words([t[0] for t in keys], [t[1] for t in keys]) if keys else np.zeros((0, 4), dtype=np.uint64)
+    return words, np.array([out[t][0] for t in keys]), [(out[t][1], out[t][2]) for t in keys], records

@@ -39,6 +39,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <string>
+#include <type_traits>
 #include <rocprim/rocprim.hpp>
 
 #include "hci_internal.h"
@@ -842,6 +844,199 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+// The screened-sampler variants of the external product, one determinant
+// with coefficient c and local = eps / |c|; lists in the order of
+// valence_elecs_and_epairs.  emit(up', dn', code) receives every deterministic
+// candidate (targets inside the space too: the sorted pass drops them);
+// sample(kind, alpha, p, q, sum_remaining_abs_h, sum_remaining_h_squared,
+// scaled) receives the DetOrbSample of a list (scaled: the singles' 1e-6
+// factor).  Per variant:
+//  * SEPARATE (approx_matmul_external_separate_doubles_and_singles,
+//    src/wf/mod.rs:341-533): a kind's lists are walked when its maximum is
+//    >= local; the first entry below local ends a list with a record, valid or
+//    not; every valid entry before it is a candidate (singles with their
+//    exact element, unscreened);
+//  * SKIP_SINGLES (:535-711) and the doubles of SEMISTOCH_SINGLES (:714-1033):
+//    every doubles list, invalid entries skipped first; entries >= local are
+//    candidates, and the first valid entry below local whose target is outside
+//    the space ends the list with a record (entries inside it are passed over);
+//  * SEMISTOCH_SINGLES singles (when max_sing >= local): entries >= local
+//    with a valid external target are candidates when |H_ai| >= local and
+//    otherwise give the list's first record; below local, a list with a
+//    record ends, and one without ends at its first valid external entry with a
+//    record scaled by 1e-6;
+//  * NO_SINGLES (:1035-1193): doubles as SEPARATE without the maxima; every
+//    valence electron gives a record from its singles list's first entry.
+template <class S, int V, class Emit, class Sample>
+__device__ void walk_var(const DevSel& L, const DevInts& T, S vmask, S up, S dn, double local, const uint64_t* space,
+                         const int64_t* didx, int64_t n, Emit emit, Sample sample) {
+    // SEPARATE and NO_SINGLES cut a list at its first entry below local,
+    // valid or not; SKIP_SINGLES and SEMISTOCH_SINGLES at the first valid
+    // external one
+    constexpr bool kCutFirst = V == HCI_EXT_SEPARATE || V == HCI_EXT_NO_SINGLES;
+    if (V != HCI_EXT_SEPARATE || L.max_opp >= local) {
+        for (S a = vmask & up; a; a &= a - (S)1) {
+            const int i = ctz(a);
+            for (S b = vmask & dn; b; b &= b - (S)1) {
+                const int j = ctz(b);
+                const int64_t base = ((int64_t)L.vpos[i] * L.nv + L.vpos[j]) * L.Lo;
+                for (int64_t e = 0; e < L.Lo; ++e) {
+                    const double h = L.oh[base + e];
+                    if (kCutFirst && h < local) {
+                        sample(0, -1, i, j, L.oabs[base + e], L.orem[base + e], false);
+                        break;
+                    }
+                    const uint16_t t = L.ot[base + e];
+                    const int r = t & 0xFF, s = t >> 8;
+                    if (((up >> r) & (S)1) || ((dn >> s) & (S)1)) continue;
+                    const S nu = (up ^ bit<S>(i)) | bit<S>(r), nd = (dn ^ bit<S>(j)) | bit<S>(s);
+                    if (!kCutFirst && h < local) {
+                        if (in_space<S>(nu, nd, space, didx, n)) continue;
+                        sample(0, -1, i, j, L.oabs[base + e], L.orem[base + e], false);
+                        break;
+                    }
+                    emit(nu, nd, exc_code(0, 0, i, j, r, s));
+                }
+            }
+        }
+    }
+    if ((V != HCI_EXT_SEPARATE || L.max_same >= local) && L.Ls > 0) {
+        for (int spin = 0; spin < 2; ++spin) {
+            const S cfg = spin == 0 ? up : dn;
+            for (S a = vmask & cfg; a; a &= a - (S)1) {
+                const int i = ctz(a);
+                for (S b = a & (a - (S)1); b; b &= b - (S)1) {
+                    const int j = ctz(b);
+                    const int64_t base = (int64_t)L.spos[L.vpos[i] * L.nv + L.vpos[j]] * L.Ls;
+                    for (int64_t e = 0; e < L.Ls; ++e) {
+                        const double h = L.sh[base + e];
+                        if (kCutFirst && h < local) {
+                            sample(1, spin == 0, i, j, L.sabs[base + e], L.srem[base + e], false);
+                            break;
+                        }
+                        const uint16_t t = L.st[base + e];
+                        const int r = t & 0xFF, s = t >> 8;
+                        if (((cfg >> r) & (S)1) || ((cfg >> s) & (S)1)) continue;
+                        const S nw = (cfg ^ bit<S>(i) ^ bit<S>(j)) | bit<S>(r) | bit<S>(s);
+                        const S nu = spin == 0 ? nw : up, nd = spin == 0 ? dn : nw;
+                        if (!kCutFirst && h < local) {
+                            if (in_space<S>(nu, nd, space, didx, n)) continue;
+                            sample(1, spin == 0, i, j, L.sabs[base + e], L.srem[base + e], false);
+                            break;
+                        }
+                        emit(nu, nd, exc_code(1, spin == 0, i, j, r, s));
+                    }
+                }
+            }
+        }
+    }
+    if (V == HCI_EXT_SKIP_SINGLES) return;
+    if (V != HCI_EXT_NO_SINGLES && L.max_sing < local) return;
+    for (int spin = 0; spin < 2; ++spin) {
+        const S cfg = spin == 0 ? up : dn, other = spin == 0 ? dn : up;
+        for (S a = vmask & cfg; a; a &= a - (S)1) {
+            const int i = ctz(a);
+            const int64_t base = (int64_t)L.vpos[i] * L.Lg;
+            if (V == HCI_EXT_NO_SINGLES) {
+                sample(2, spin == 0, i, -1, L.gabs[base], L.grem[base], false);
+                continue;
+            }
+            bool stored = false;
+            for (int64_t e = 0; e < L.Lg; ++e) {
+                const double h = L.gh[base + e];
+                if (V == HCI_EXT_SEPARATE && h < local) {
+                    sample(2, spin == 0, i, -1, L.gabs[base + e], L.grem[base + e], false);
+                    break;
+                }
+                if (V == HCI_EXT_SEMISTOCH_SINGLES && h < local && stored) break;
+                const int r = L.gt[base + e] & 0xFF;
+                if ((cfg >> r) & (S)1) continue;
+                const S nw = (cfg ^ bit<S>(i)) | bit<S>(r);
+                const S nu = spin == 0 ? nw : up, nd = spin == 0 ? dn : nw;
+                const uint32_t code = exc_code(2, spin == 0, i, 0, r, 0);
+                if (V == HCI_EXT_SEPARATE) {
+                    emit(nu, nd, code);
+                    continue;
+                }
+                if (h >= local) {
+                    // ham_sing(D, new) (src/ham/mod.rs:135-168)
+                    if (fabs(elem_single<S>(T, cfg, nw, other)) >= local) {
+                        emit(nu, nd, code);
+                    } else if (!stored && !in_space<S>(nu, nd, space, didx, n)) {
+                        sample(2, spin == 0, i, -1, L.gabs[base + e], L.grem[base + e], false);
+                        stored = true;
+                    }
+                } else if (!in_space<S>(nu, nd, space, didx, n)) {
+                    sample(2, spin == 0, i, -1, L.gabs[base + e], L.grem[base + e], true);
+                    break;
+                }
+            }
+        }
+    }
+}
+
+template <class S, int V, bool FILL>
+__global__ void k_extv_walk(DevSel L, DevInts T, uint64_t vm0, uint64_t vm1, const uint64_t* __restrict__ dets,
+                            const double* __restrict__ coeffs, int64_t n, double eps, const int64_t* __restrict__ didx,
+                            int64_t* counts, int64_t* scounts, const int64_t* __restrict__ offs,
+                            const int64_t* __restrict__ soffs, uint64_t* words, double* vals, int64_t* src,
+                            uint32_t* codes, hci_det_orb_sample* samples) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const S up = load_string<S>(dets + 4 * k), dn = load_string<S>(dets + 4 * k + 2);
+    S vmask;
+    if constexpr (sizeof(S) > 8)
+        vmask = ((S)vm1 << 64) | (S)vm0;
+    else
+        vmask = (S)vm0;
+    const double c = coeffs[k];
+    const double local = eps / fabs(c);  // +inf for c = 0: every magnitude is below it
+    int64_t cnt = 0, scnt = 0;
+    const int64_t o = FILL ? offs[k] : 0, so = FILL ? soffs[k] : 0;
+    walk_var<S, V>(
+        L, T, vmask, up, dn, local, dets, didx, n,
+        [&](S nu, S nd, uint32_t code) {
+            if (FILL) {
+                const int kind = code & 3, alpha = (code >> 2) & 1;
+                double h;
+                if (kind == 0)
+                    h = elem_opp_double<S>(T, up, nu, dn, nd);
+                else if (kind == 1)
+                    h = alpha ? elem_same_double<S>(T, up, nu) : elem_same_double<S>(T, dn, nd);
+                else
+                    h = alpha ? elem_single<S>(T, up, nu, dn) : elem_single<S>(T, dn, nd, up);
+                store_string<S>(words + 4 * (o + cnt), nu);
+                store_string<S>(words + 4 * (o + cnt) + 2, nd);
+                vals[o + cnt] = h * c;
+                src[o + cnt] = k;
+                codes[o + cnt] = code;
+            }
+            ++cnt;
+        },
+        [&](int kind, int alpha, int p, int q, double rabs, double rsq, bool scaled) {
+            if (FILL) {
+                // single_sample_factor (src/wf/mod.rs:895), applied last as the reference does
+                constexpr double f = 1e-6;
+                hci_det_orb_sample r;
+                r.det = k;
+                r.kind = kind;
+                r.is_alpha = alpha;
+                r.p = p;
+                r.q = q;
+                r.sum_abs_h = scaled ? rabs * f : rabs;
+                r.sum_h_squared = scaled ? rsq * f : rsq;
+                r.sum_abs_hc = scaled ? fabs(c) * rabs * f : fabs(c) * rabs;
+                r.sum_hc_squared = scaled ? c * c * rsq * f : c * c * rsq;
+                samples[so + scnt] = r;
+            }
+            ++scnt;
+        });
+    if (!FILL) {
+        counts[k] = cnt;
+        scounts[k] = scnt;
+    }
+}
+
 struct Tmp {
     void* p = nullptr;
     size_t bytes = 0;
@@ -1369,14 +1564,28 @@ int hci_pt2_dtm(hci_ctx* c, const uint64_t* dets, const double* coeffs, int64_t 
     return HCI_OK;
 }
 
-int hci_matmul_external(hci_ctx* c, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
-                        double eps, uint64_t* out, double* out_coeffs, double* out_diag, int64_t cap, int64_t* count,
-                        double* sum_remaining, int mem) {
+}  // extern "C"
+
+namespace {
+
+// The external product of approx_matmul_external_dtm_only (variant 0) or of a
+// screened-sampler variant (HCI_EXT_*): walk (count, then fill), stable sort
+// of the candidates by their words, run sums in discovery order, members of the
+// space dropped, outputs in first-discovery order.
+int ext_product(hci_ctx* c, const char* fn, int variant, const uint64_t* dets, const double* coeffs,
+                const double* diag, int64_t n, double eps, uint64_t* out, double* out_coeffs, double* out_diag,
+                int64_t cap, int64_t* count, double* sum_remaining, hci_det_orb_sample* samples, int64_t scap,
+                int64_t* n_samples, int mem) {
+    const std::string name(fn);
     if (!c) return HCI_ERR_ARG;
-    if (!c->sel) return hci_fail(c, HCI_ERR_STATE, "hci_matmul_external: call hci_excite_init first");
+    if (!c->sel) return hci_fail(c, HCI_ERR_STATE, name + ": call hci_excite_init first");
     if (n < 1 || !dets || !coeffs || !count || cap < 0 || (cap > 0 && (!out || !out_coeffs)) || !(eps > 0.0) ||
-        (out_diag && !diag))
-        return hci_fail(c, HCI_ERR_ARG, "hci_matmul_external: bad arguments");
+        (out_diag && !diag) || scap < 0 || (scap > 0 && !samples))
+        return hci_fail(c, HCI_ERR_ARG, name + ": bad arguments");
+    if (variant == HCI_EXT_NO_SINGLES && c->sel->Lg == 0)
+        return hci_fail(c, HCI_ERR_ARG,
+                        name + ": no_singles reads the first entry of every valence electron's singles list, "
+                               "and these lists are empty");
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const SelectLists& L = *c->sel;
@@ -1413,45 +1622,88 @@ int hci_matmul_external(hci_ctx* c, const uint64_t* dets, const double* coeffs, 
         if ((e = tmp.ensure(need)) != hipSuccess) return e;
         return rocprim::exclusive_scan(tmp.p, need, in, o, (int64_t)0, (size_t)len, rocprim::plus<int64_t>(), s);
     };
-    int64_t *cnt = nullptr, *off = nullptr, *didx = nullptr;
+    int64_t *cnt = nullptr, *off = nullptr, *scnt = nullptr, *soff = nullptr, *didx = nullptr;
     double* srem = nullptr;
     HIP_TRY(c, b.alloc(&cnt, (size_t)n + 1));
     HIP_TRY(c, b.alloc(&off, (size_t)n + 1));
     HIP_TRY(c, b.alloc(&didx, (size_t)n));
+    if (variant != 0) {
+        HIP_TRY(c, b.alloc(&scnt, (size_t)n + 1));
+        HIP_TRY(c, b.alloc(&soff, (size_t)n + 1));
+    }
     if (sum_remaining) HIP_TRY(c, b.alloc(&srem, (size_t)n));
     HIP_TRY(c, sort_dets(tmp, b, d_dets, n, nw, didx, s));
-    if (wide)
-        hipLaunchKernelGGL((k_ext_walk<u128, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
-                           d_dets, d_coef, n, eps, cnt, srem, nullptr, nullptr, nullptr, nullptr, nullptr);
-    else
-        hipLaunchKernelGGL((k_ext_walk<uint64_t, false>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
-                           d_dets, d_coef, n, eps, cnt, srem, nullptr, nullptr, nullptr, nullptr, nullptr);
+    // one launch of the walk: variant 0 through k_ext_walk, the others through
+    // k_extv_walk<S, variant, FILL>
+    auto launch = [&](bool fill, uint64_t* cw, double* cv, int64_t* csrc, uint32_t* ccode,
+                      hci_det_orb_sample* smp) -> hipError_t {
+        const dim3 g = grid_for(n, 256);
+        auto run = [&](auto tag_s, auto tag_v, auto tag_f) {
+            using S = decltype(tag_s);
+            constexpr int V = decltype(tag_v)::value;
+            constexpr bool F = decltype(tag_f)::value;
+            if constexpr (V == 0)
+                hipLaunchKernelGGL((k_ext_walk<S, F>), g, 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets, d_coef, n,
+                                   eps, F ? nullptr : cnt, F ? nullptr : srem, F ? off : nullptr, cw, cv, csrc, ccode);
+            else
+                hipLaunchKernelGGL((k_extv_walk<S, V, F>), g, 256, 0, s, D, T, L.vmask[0], L.vmask[1], d_dets, d_coef,
+                                   n, eps, didx, F ? nullptr : cnt, F ? nullptr : scnt, F ? off : nullptr,
+                                   F ? soff : nullptr, cw, cv, csrc, ccode, smp);
+        };
+        auto by_v = [&](auto tag_s, auto tag_f) {
+            switch (variant) {
+                case HCI_EXT_SEPARATE: run(tag_s, std::integral_constant<int, HCI_EXT_SEPARATE>(), tag_f); break;
+                case HCI_EXT_SKIP_SINGLES: run(tag_s, std::integral_constant<int, HCI_EXT_SKIP_SINGLES>(), tag_f); break;
+                case HCI_EXT_SEMISTOCH_SINGLES:
+                    run(tag_s, std::integral_constant<int, HCI_EXT_SEMISTOCH_SINGLES>(), tag_f);
+                    break;
+                case HCI_EXT_NO_SINGLES: run(tag_s, std::integral_constant<int, HCI_EXT_NO_SINGLES>(), tag_f); break;
+                default: run(tag_s, std::integral_constant<int, 0>(), tag_f); break;
+            }
+        };
+        if (wide)
+            fill ? by_v(u128(), std::true_type()) : by_v(u128(), std::false_type());
+        else
+            fill ? by_v(uint64_t(), std::true_type()) : by_v(uint64_t(), std::false_type());
+        return hipGetLastError();
+    };
+    HIP_TRY(c, launch(false, nullptr, nullptr, nullptr, nullptr, nullptr));
     HIP_TRY(c, hipMemsetAsync(cnt + n, 0, 8, s));
     HIP_TRY(c, scan(cnt, off, n + 1));
     if (sum_remaining) HIP_TRY(c, hipMemcpyAsync(sum_remaining, srem, (size_t)n * 8, out_kind, s));
-    int64_t m = 0;
+    int64_t m = 0, ns = 0;
     HIP_TRY(c, hipMemcpyAsync(&m, off + n, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
-    if (m == 0) {
-        *count = 0;
-        return HCI_OK;
+    if (variant != 0) {
+        HIP_TRY(c, hipMemsetAsync(scnt + n, 0, 8, s));
+        HIP_TRY(c, scan(scnt, soff, n + 1));
+        HIP_TRY(c, hipMemcpyAsync(&ns, soff + n, 8, hipMemcpyDeviceToHost, s));
     }
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (n_samples) *n_samples = ns;
+    if (samples && ns > scap) return hci_fail(c, HCI_ERR_ARG, name + ": sample capacity too small (n_samples holds the need)");
     uint64_t *cw = nullptr, *rw = nullptr;
     double *cv = nullptr, *rv = nullptr;
     int64_t *csrc = nullptr, *cidx = nullptr, *head = nullptr, *hpos = nullptr, *start = nullptr, *keep = nullptr,
             *kpos = nullptr;
     uint32_t* ccode = nullptr;
+    hci_det_orb_sample* d_smp = nullptr;
     HIP_TRY(c, b.alloc(&cw, (size_t)m * 4));
     HIP_TRY(c, b.alloc(&cv, (size_t)m));
     HIP_TRY(c, b.alloc(&csrc, (size_t)m));
     HIP_TRY(c, b.alloc(&ccode, (size_t)m));
-    if (wide)
-        hipLaunchKernelGGL((k_ext_walk<u128, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
-                           d_dets, d_coef, n, eps, nullptr, nullptr, off, cw, cv, csrc, ccode);
-    else
-        hipLaunchKernelGGL((k_ext_walk<uint64_t, true>), grid_for(n, 256), 256, 0, s, D, T, L.vmask[0], L.vmask[1],
-                           d_dets, d_coef, n, eps, nullptr, nullptr, off, cw, cv, csrc, ccode);
-    HIP_TRY(c, hipGetLastError());
+    if (ns > 0) HIP_TRY(c, b.alloc(&d_smp, (size_t)ns));
+    if (m > 0 || ns > 0) HIP_TRY(c, launch(true, cw, cv, csrc, ccode, d_smp));
+    // records are copied once the determinant capacity has been checked
+    auto copy_samples = [&]() -> hipError_t {
+        if (!samples || ns == 0) return hipSuccess;
+        return hipMemcpyAsync(samples, d_smp, (size_t)ns * sizeof(hci_det_orb_sample), out_kind, s);
+    };
+    if (m == 0) {
+        *count = 0;
+        HIP_TRY(c, copy_samples());
+        HIP_TRY(c, hipStreamSynchronize(s));
+        return HCI_OK;
+    }
     // stable sort by words: each target's contributions stay in discovery
     // order (add_det_with_coeff, src/wf/mod.rs:144-165), summed per run; runs
     // of determinants in the space are dropped
@@ -1479,8 +1731,12 @@ int hci_matmul_external(hci_ctx* c, const uint64_t* dets, const double* coeffs, 
     HIP_TRY(c, hipMemcpyAsync(&k, kpos + runs, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     *count = k;
-    if (k > cap) return hci_fail(c, HCI_ERR_ARG, "hci_matmul_external: output capacity too small (count holds the need)");
-    if (k == 0) return HCI_OK;
+    if (k > cap) return hci_fail(c, HCI_ERR_ARG, name + ": output capacity too small (count holds the need)");
+    HIP_TRY(c, copy_samples());
+    if (k == 0) {
+        HIP_TRY(c, hipStreamSynchronize(s));
+        return HCI_OK;
+    }
     // first-discovery order: kept runs sorted by their head's candidate index
     int64_t *first = nullptr, *rid = nullptr, *first2 = nullptr, *rid2 = nullptr;
     HIP_TRY(c, b.alloc(&first, (size_t)k));
@@ -1511,6 +1767,27 @@ int hci_matmul_external(hci_ctx* c, const uint64_t* dets, const double* coeffs, 
     return HCI_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int hci_matmul_external(hci_ctx* c, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
+                        double eps, uint64_t* out, double* out_coeffs, double* out_diag, int64_t cap, int64_t* count,
+                        double* sum_remaining, int mem) {
+    return ext_product(c, "hci_matmul_external", 0, dets, coeffs, diag, n, eps, out, out_coeffs, out_diag, cap, count,
+                       sum_remaining, nullptr, 0, nullptr, mem);
+}
+
+int hci_matmul_external_sampled(hci_ctx* c, int variant, const uint64_t* dets, const double* coeffs,
+                                const double* diag, int64_t n, double eps, uint64_t* out, double* out_coeffs,
+                                double* out_diag, int64_t cap, int64_t* count, hci_det_orb_sample* samples,
+                                int64_t scap, int64_t* n_samples, int mem) {
+    if (c && variant != HCI_EXT_SEPARATE && variant != HCI_EXT_SKIP_SINGLES && variant != HCI_EXT_SEMISTOCH_SINGLES &&
+        variant != HCI_EXT_NO_SINGLES)
+        return hci_fail(c, HCI_ERR_ARG, "hci_matmul_external_sampled: unknown variant");
+    return ext_product(c, "hci_matmul_external_sampled", variant, dets, coeffs, diag, n, eps, out, out_coeffs,
+                       out_diag, cap, count, nullptr, samples, scap, n_samples, mem);
+}
 
 int hci_semistoch_dtm(hci_ctx* c, const uint64_t* dets, const double* coeffs, const double* diag, int64_t n,
                       double e0, double eps, double* e2, int64_t* n_ext, double* rel_probs,

@@ -39,6 +39,14 @@ PRECOND_MLP = 1
 STORE_FULL = 0
 STORE_UPPER = 1
 STORE_TILED = 2
+# external-product variants of hci_matmul_external_sampled (the reference's
+# Wf::approx_matmul_external_* with a screened sampler)
+EXT_SEPARATE = 1
+EXT_SKIP_SINGLES = 2
+EXT_SEMISTOCH_SINGLES = 3
+EXT_NO_SINGLES = 4
+EXT_VARIANTS = {"separate_doubles_and_singles": EXT_SEPARATE, "skip_singles": EXT_SKIP_SINGLES,
+                "semistoch_singles": EXT_SEMISTOCH_SINGLES, "no_singles": EXT_NO_SINGLES}
 
 
 class HciError(RuntimeError):
@@ -136,6 +144,9 @@ SIGNATURES = {
                                       C.POINTER(C.c_int64), _P, C.c_int]),
     "hci_semistoch_dtm": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int64), _P, _P, C.c_int64, C.POINTER(C.c_int64), C.c_int]),
+    "hci_matmul_external_sampled": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_int64, C.c_double, _P, _P, _P,
+                                              C.c_int64, C.POINTER(C.c_int64), _P, C.c_int64,
+                                              C.POINTER(C.c_int64), C.c_int]),
 }
 
 # hci_det_orb_sample (include/hci_gpu.h), the reference's DetOrbSample
@@ -332,6 +343,44 @@ class HciContext:
             if rc == HCI_OK:
                 k = cnt.value
                 return out[:k], oc[:k], od[:k], rem
+            if cnt.value > cap:
+                cap = cnt.value
+                continue
+            self._check(rc)
+
+    def matmul_external_sampled(self, variant, dets: np.ndarray, coeffs: np.ndarray, diag: np.ndarray,
+                                eps: float):
+        """The external products that set up a screened sampler
+        (Wf::approx_matmul_external_separate_doubles_and_singles,
+        _skip_singles, _semistoch_singles, _no_singles; src/wf/mod.rs:341-1193),
+        `variant` an EXT_* code or its name: (external determinants (k, 4)
+        uint64 in first-discovery order, their coefficients sum_i H_ai c_i,
+        their diagonals from the first discoverer, the DetOrbSample records
+        as a DET_ORB_SAMPLE array in push order).  Needs excite_init on the
+        same integrals."""
+        v = EXT_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        dets = np.ascontiguousarray(dets, dtype=np.uint64)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        n = dets.shape[0]
+        cnt = C.c_int64()
+        ns = C.c_int64()
+        cap = max(1024, 8 * n)
+        scap = max(1024, 64 * n)
+        while True:
+            out = np.empty((cap, 4), dtype=np.uint64)
+            oc = np.empty(cap)
+            od = np.empty(cap)
+            smp = np.empty(scap, dtype=DET_ORB_SAMPLE)
+            rc = self.lib.hci_matmul_external_sampled(self.handle, v, _ptr(dets), _ptr(coeffs), _ptr(diag), n, eps,
+                                                      _ptr(out), _ptr(oc), _ptr(od), cap, C.byref(cnt), _ptr(smp),
+                                                      scap, C.byref(ns), MEM_HOST)
+            if rc == HCI_OK:
+                k = cnt.value
+                return out[:k], oc[:k], od[:k], smp[:ns.value]
+            if ns.value > scap:
+                scap = ns.value
+                continue
             if cnt.value > cap:
                 cap = cnt.value
                 continue

@@ -252,3 +252,102 @@ def test_semistoch_dtm_h_psi_equals_dtm_pt(gpu_ctx):
     p2, pn = gpu_ctx.pt2_dtm(dets, c, e0, 1e-6)
     assert n_ext == pn
     assert abs(e2 - p2) < 1e-11 * abs(p2)
+
+
+def _check_sampled(gpu_ctx, ints, dets, c, eps, n_core, n_up, n_dn, variant):
+    """hci_matmul_external_sampled against pt2_oracle.matmul_external_sampled:
+    the same external determinants in the same first-discovery order,
+    coefficients to the last bits (the same products summed in the same
+    order), the first discoverer's new_diag bit-identical to the host
+    restatement, and every DetOrbSample record equal (integers exact, sums
+    within 1e-14 relative)."""
+    from risq_amd.hci_select import new_diag
+    diag = gpu_ctx.ham_diag(dets)
+    gpu_ctx.excite_init(n_core, n_up, n_dn, 1e-3)
+    w, v, d, smp = gpu_ctx.matmul_external_sampled(variant, dets, c, diag, eps)
+    ow, ov, firsts, orec = pt2_oracle.matmul_external_sampled(ints, dets, c, eps, n_core, n_up, n_dn, variant)
+    print(f"{variant} eps={eps:g} n_core={n_core}: {w.shape[0]} external determinants (oracle {ow.shape[0]}), "
+          f"records {smp.size} (oracle {len(orec)})", flush=True)
+    assert w.shape[0] == ow.shape[0] > 0
+    assert np.array_equal(w, ow)
+    assert np.max(np.abs(v - ov) / np.maximum(np.abs(ov), 1e-300)) < 1e-14
+    ref = []
+    for k, (kind, alpha, i, j, tgt) in firsts:
+        parent = (int(dets[k][0]) | int(dets[k][1]) << 64, int(dets[k][2]) | int(dets[k][3]) << 64)
+        exc = (("opp", (i, j), tgt) if kind == 0 else ("same", (i, j), tgt, alpha == 1) if kind == 1
+               else ("sing", i, tgt, alpha == 1))
+        ref.append(new_diag(ints, parent, float(diag[k]), exc))
+    assert np.array_equal(d, np.array(ref))
+    assert smp.size == len(orec) > 0
+    orec = np.array(orec, dtype=object)
+    for f, col in (("det", 0), ("kind", 1), ("is_alpha", 2), ("p", 3), ("q", 4)):
+        assert np.array_equal(smp[f], orec[:, col].astype(np.int64)), f
+    for f, col in (("sum_abs_h", 5), ("sum_h_squared", 6), ("sum_abs_hc", 7), ("sum_hc_squared", 8)):
+        want = orec[:, col].astype(np.float64)
+        assert np.all(np.abs(smp[f] - want) <= 1e-14 * np.abs(want)), f
+
+
+SAMPLED = ["separate_doubles_and_singles", "skip_singles", "semistoch_singles", "no_singles"]
+
+
+@pytest.mark.parametrize("variant", SAMPLED)
+@pytest.mark.parametrize("eps,n_core", [(1e-5, 0), (1e-4, 1)])
+def test_external_sampled_matches_oracle(gpu_ctx, variant, eps, n_core):
+    """The four screened-sampler external products (src/wf/mod.rs:341-1193)
+    on a 16-orbital space of 300 determinants."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    if n_core:
+        dets = dets[(dets[:, 0] & dets[:, 2] & np.uint64(1)) == 1][:300]
+    else:
+        dets = dets[:300]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    _check_sampled(gpu_ctx, ints, dets, c, eps, n_core, 4, 4, variant)
+
+
+@pytest.mark.parametrize("variant", SAMPLED)
+def test_external_sampled_two_word_strings(gpu_ctx, variant):
+    ints, dets = mol_space(70, 8, 5, 3, 3, 1500)
+    dets = dets[:8]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    _check_sampled(gpu_ctx, ints, dets, c, 1e-4, 0, 3, 3, variant)
+
+
+def test_external_sampled_capacity_and_device_memory(gpu_ctx):
+    """Too small a record capacity fails with the needed count and writes no
+    determinants; device-memory inputs and outputs give the host call's
+    arrays; an unknown variant is refused."""
+    import ctypes as C
+    from risq_amd.hci import DET_ORB_SAMPLE, DeviceBuffer, EXT_SEMISTOCH_SINGLES, HCI_OK, MEM_DEVICE, MEM_HOST
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    dets = np.ascontiguousarray(dets[:300])
+    e0, c = _solve(gpu_ctx, ints, dets)
+    c = np.ascontiguousarray(c)
+    diag = gpu_ctx.ham_diag(dets)
+    gpu_ctx.excite_init(0, 4, 4, 1e-3)
+    w, v, d, smp = gpu_ctx.matmul_external_sampled(EXT_SEMISTOCH_SINGLES, dets, c, diag, 1e-5)
+    k, ns = w.shape[0], smp.size
+    cnt, nsv = C.c_int64(-1), C.c_int64()
+    out = np.empty((k, 4), dtype=np.uint64)
+    oc, od = np.empty(k), np.empty(k)
+    small = np.empty(4, dtype=DET_ORB_SAMPLE)
+    rc = gpu_ctx.lib.hci_matmul_external_sampled(gpu_ctx.handle, EXT_SEMISTOCH_SINGLES, dets.ctypes.data,
+                                                 c.ctypes.data, diag.ctypes.data, dets.shape[0], 1e-5,
+                                                 out.ctypes.data, oc.ctypes.data, od.ctypes.data, k, C.byref(cnt),
+                                                 small.ctypes.data, 4, C.byref(nsv), MEM_HOST)
+    assert rc != HCI_OK and nsv.value == ns > 4 and cnt.value == -1
+    rc = gpu_ctx.lib.hci_matmul_external_sampled(gpu_ctx.handle, 7, dets.ctypes.data, c.ctypes.data,
+                                                 diag.ctypes.data, dets.shape[0], 1e-5, out.ctypes.data,
+                                                 oc.ctypes.data, od.ctypes.data, k, C.byref(cnt), small.ctypes.data,
+                                                 4, C.byref(nsv), MEM_HOST)
+    assert rc != HCI_OK
+    dd, dc, dg = (DeviceBuffer.from_array(gpu_ctx, a) for a in (dets, c, diag))
+    ow, ov, od2, osm = (DeviceBuffer(gpu_ctx, nb) for nb in (k * 32, k * 8, k * 8, ns * DET_ORB_SAMPLE.itemsize))
+    rc = gpu_ctx.lib.hci_matmul_external_sampled(gpu_ctx.handle, EXT_SEMISTOCH_SINGLES, dd.ptr, dc.ptr, dg.ptr,
+                                                 dets.shape[0], 1e-5, ow.ptr, ov.ptr, od2.ptr, k, C.byref(cnt),
+                                                 osm.ptr, ns, C.byref(nsv), MEM_DEVICE)
+    assert rc == HCI_OK and cnt.value == k and nsv.value == ns
+    assert np.array_equal(ow.to_array(np.uint64, 4 * k).reshape(k, 4), w)
+    assert np.array_equal(ov.to_array(np.float64, k), v)
+    assert np.array_equal(od2.to_array(np.float64, k), d)
+    raw = osm.to_array(np.uint8, ns * DET_ORB_SAMPLE.itemsize)
+    assert np.array_equal(raw.view(DET_ORB_SAMPLE), smp)
