@@ -72,10 +72,13 @@ __device__ __forceinline__ void block_store(double (&acc)[NV], int nv, double* p
     }
 }
 
-// The vector kernels below are instantiated for KM = 8, 16 and 32 columns at
-// most (the solve's subspace size rounded up): their per-thread arrays take
+// The vector kernels below are instantiated for KM = 8, 12, 16, 24 and 32
+// columns at most (the solve's subspace size rounded up): their per-thread arrays take
 // registers in proportion to KM, and the occupancy that the registers leave
-// decides how many rows' loads are in flight.
+// decides how many rows' loads are in flight.  Rows are 32-bit offsets from
+// per-column bases (n < 2^31), so a column's address stays in scalar
+// registers (k_ritz_dots<16>: 149 VGPRs against 222 with a 64-bit address per
+// column and thread).
 
 // partial[b*k + c] = sum over this block's rows of A[c][i] * b[i], c < k <= KM + 1
 template <int KM>
@@ -85,11 +88,11 @@ __global__ void __launch_bounds__(kRedThreads)
     double acc[KM + 1];
 #pragma unroll
     for (int c = 0; c <= KM; ++c) acc[c] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)n; i += gridDim.x * blockDim.x) {
         const double bi = b[i];
 #pragma unroll
         for (int c = 0; c <= KM; ++c)
-            if (c < k) acc[c] += A[(size_t)c * ld + i] * bi;
+            if (c < k) acc[c] += (A + (size_t)c * ld)[i] * bi;
     }
     block_store<KM + 1>(acc, k, partial);
 }
@@ -118,13 +121,13 @@ __global__ void __launch_bounds__(kRedThreads)
            double lam, const double* __restrict__ d, int64_t n, double* x, double* r, double* t,
            double* partial) {
     double acc[2] = {0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)n; i += gridDim.x * blockDim.x) {
         double xv = 0.0, wv = 0.0;
 #pragma unroll
         for (int c = 0; c < KM; ++c)
             if (c < k) {
-                xv += V[(size_t)c * ld + i] * y.c[c];
-                wv += W[(size_t)c * ld + i] * y.c[c];
+                xv += (V + (size_t)c * ld)[i] * y.c[c];
+                wv += (W + (size_t)c * ld)[i] * y.c[c];
             }
         const double rv = wv - xv * lam;
         const double tv = rv / (lam - d[i]);
@@ -150,14 +153,16 @@ __global__ void __launch_bounds__(kRedThreads)
     double acc[2 + 2 * KM];
 #pragma unroll
     for (int c = 0; c < 2 + 2 * KM; ++c) acc[c] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    // rows as 32-bit offsets from per-column bases (n < 2^31): the column
+    // bases stay scalar instead of one 64-bit address per column per thread
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)n; i += gridDim.x * blockDim.x) {
         double vc[KM], wc[KM];
         double xv = 0.0, wv = 0.0;
 #pragma unroll
         for (int c = 0; c < KM; ++c)
             if (c < k) {
-                vc[c] = V[(size_t)c * ld + i];
-                wc[c] = W[(size_t)c * ld + i];
+                vc[c] = (V + (size_t)c * ld)[i];
+                wc[c] = (W + (size_t)c * ld)[i];
                 xv += vc[c] * y.c[c];
                 wv += wc[c] * y.c[c];
             }
@@ -193,13 +198,13 @@ __global__ void __launch_bounds__(kRedThreads)
     double acc[KM + 1];
 #pragma unroll
     for (int c = 0; c <= KM; ++c) acc[c] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)n; i += gridDim.x * blockDim.x) {
         double tv = t[i];
         double vc[KM];
 #pragma unroll
         for (int c = 0; c < KM; ++c)
             if (c < k) {
-                vc[c] = V[(size_t)c * ld + i];
+                vc[c] = (V + (size_t)c * ld)[i];
                 tv -= vc[c] * h.c[c];
             }
         t[i] = tv;
@@ -215,15 +220,18 @@ __global__ void __launch_bounds__(kRedThreads)
     block_store<KM + 1>(acc, k + 1, partial);
 }
 
-// launch kern<KM> with the smallest KM in {8, 16, 32} that is >= km
+// the column bound KM of the vector kernels for a subspace of km columns: the
+// smallest of 8, 12, 16, 24 and 32 that is >= km
+inline int km_bound(int km) { return km <= 8 ? 8 : km <= 12 ? 12 : km <= 16 ? 16 : km <= 24 ? 24 : 32; }
 #define HCI_LAUNCH_KM(kern, km, grid, block, stream, ...)                                   \
     do {                                                                                    \
-        if ((km) <= 8)                                                                      \
-            hipLaunchKernelGGL(kern<8>, grid, block, 0, stream, __VA_ARGS__);               \
-        else if ((km) <= 16)                                                                \
-            hipLaunchKernelGGL(kern<16>, grid, block, 0, stream, __VA_ARGS__);              \
-        else                                                                                \
-            hipLaunchKernelGGL(kern<32>, grid, block, 0, stream, __VA_ARGS__);              \
+        switch (km_bound(km)) {                                                             \
+            case 8: hipLaunchKernelGGL(kern<8>, grid, block, 0, stream, __VA_ARGS__); break;   \
+            case 12: hipLaunchKernelGGL(kern<12>, grid, block, 0, stream, __VA_ARGS__); break; \
+            case 16: hipLaunchKernelGGL(kern<16>, grid, block, 0, stream, __VA_ARGS__); break; \
+            case 24: hipLaunchKernelGGL(kern<24>, grid, block, 0, stream, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL(kern<32>, grid, block, 0, stream, __VA_ARGS__); break; \
+        }                                                                                   \
     } while (0)
 
 // In place: V[:, j] = sum_c V[:, c] q[c * kMaxKeep + j] for j < m (same for W).
@@ -624,7 +632,7 @@ static int davidson_run(hci_ctx* c, const hci_csr* h, const double* init, const 
         // diagonal-correction steps (no network, no recorder) take V^T t and
         // W^T t from the Ritz pass itself: rd[2 + c] and rd[2 + km + c]
         const bool fused = !net && !rec && dim_sub < max_sub;
-        const int km = w.max_sub <= 8 ? 8 : w.max_sub <= 16 ? 16 : 32;
+        const int km = km_bound(w.max_sub);
         const double* rd = w.h_red + 3 * (kMaxSub + 1);
         if (fused) {
             HCI_LAUNCH_KM(k_ritz_dots, w.max_sub, w.red_grid(), kRedThreads, s, w.V, w.W, N, dim_sub, y, lam,
