@@ -257,7 +257,7 @@ def main() -> int:
                     help="skip the untimed comparison runs (FP64 reference path, other preconditioner)")
     ap.add_argument("--values", choices=["f16", "f32"], default=None,
                     help="lowest value precision of the timed path: f16 (FP16 copy, FP16 -> FP32 -> FP64 "
-                         "sigmas) or f32 (FP32 -> FP64); default f16 for workload 1, f32 otherwise")
+                         "sigmas) or f32 (FP32 -> FP64); default f16 for workloads 1 and 2, f32 for 4")
     ap.add_argument("--max-sub", type=int, default=None,
                     help="Davidson subspace size (default 16 for workload 1, 11 otherwise)")
     ap.add_argument("--keep", type=int, default=None,
@@ -312,9 +312,9 @@ def main() -> int:
         return h, res, tb, time.perf_counter() - ts
 
     if args.values is None:
-        # workload 4's matrix leaves no room for the FP16 copy; the lattice's
-        # products are bound by per-row work, which FP16 values do not shrink
-        args.values = "f16" if args.config == 1 else "f32"
+        # workload 4's matrix leaves no room for the FP16 copy (the lattice's
+        # hopping elements, +-t with t = 1, are exact in FP16)
+        args.values = "f32" if args.config == 4 else "f16"
     fast_prec, fast_mixed = (PREC_F16, 2) if args.values == "f16" else (PREC_F32, 1)
 
     def step():

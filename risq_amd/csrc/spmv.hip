@@ -227,8 +227,16 @@ struct Oct {
     f32x4 h[MODE == 0 || MODE == 3 ? 1 : 2];
     f32x4 l[MODE == 2 ? 2 : 1];
     u16x8 q;  // MODE 3: FP16 values
+    // entries base + off .. base + off + 7: base is wave-uniform (scalar
+    // address arithmetic), off a 32-bit per-lane offset
     __device__ __forceinline__ void load(const uint16_t* idx, const double* val, const float* hi, const float* lo,
-                                         int64_t e) {
+                                         int64_t base, uint32_t off) {
+        // MODE 3 passes the FP16 copies through val: their base in 2-byte units
+        const double* v = MODE == 3 ? (const double*)((const uint16_t*)val + base) : val + base;
+        load(idx + base, v, hi + base, lo + base, off);
+    }
+    __device__ __forceinline__ void load(const uint16_t* idx, const double* val, const float* hi, const float* lo,
+                                         uint32_t e) {
         u = __builtin_nontemporal_load((const u16x8*)(idx + e));
         if (MODE == 3) {
             // MODE 3 passes the FP16 copies in place of the FP64 values
@@ -341,7 +349,7 @@ __global__ void __launch_bounds__(kTileThreads)
     auto grab = [&]() {
         int k = 0;
         if (lane == 0) k = atomicAdd(&next, 1);
-        return __shfl(k, 0);
+        return __builtin_amdgcn_readfirstlane(k);
     };
     int k = grab();
     Window W{};
@@ -358,21 +366,21 @@ __global__ void __launch_bounds__(kTileThreads)
         const int m = cnt - n_full * kWinChunk;
         const int R = (m + kWinAlign - 1) / kWinAlign;
         const int64_t tail = W.o() + (int64_t)n_full * kWinChunk;
-        const int64_t mine = W.o() + (int64_t)kWinAlign * lane;
+        const uint32_t mine = kWinAlign * lane;
         const int32_t sp = PAIRS ? W.sp() : -1;
         // the first two groups' entries and the x slice are requested together
         Oct<MODE> o0{}, o1{};
         if (n_full > 0)
-            o0.load(idx, val, hi, lo, mine);
+            o0.load(idx, val, hi, lo, W.o(), mine);
         else if (lane < R)
-            o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
-        if (n_full > 1) o1.load(idx, val, hi, lo, mine + kWinChunk);
+            o0.load(idx, val, hi, lo, tail, mine);
+        if (n_full > 1) o1.load(idx, val, hi, lo, W.o() + kWinChunk, mine);
         if (!ABS) {
             XT<MODE> xv[kWinCols / 64];
 #pragma unroll
             for (int j = 0; j < kWinCols / 64; ++j) {
                 const int c = lane + 64 * j;
-                xv[j] = c < nc ? xx[W.c0 + c] : (XT<MODE>)0;
+                xv[j] = c < nc ? (xx + W.c0)[(uint32_t)c] : (XT<MODE>)0;
             }
 #pragma unroll
             for (int j = 0; j < kWinCols / 64; ++j) xw[lane + 64 * j] = xv[j];
@@ -397,14 +405,14 @@ __global__ void __launch_bounds__(kTileThreads)
             for (; c + 1 < n_full; c += 2) {
                 add_group<ABS, PR>(acc, xw, xr, tw, o0, 0xFFu);
                 add_group<ABS, PR>(acc, xw, xr, tw, o1, 0xFFu);
-                if (c + 2 < n_full) o0.load(idx, val, hi, lo, mine + (int64_t)(c + 2) * kWinChunk);
-                if (c + 3 < n_full) o1.load(idx, val, hi, lo, mine + (int64_t)(c + 3) * kWinChunk);
+                if (c + 2 < n_full) o0.load(idx, val, hi, lo, W.o() + (int64_t)(c + 2) * kWinChunk, mine);
+                if (c + 3 < n_full) o1.load(idx, val, hi, lo, W.o() + (int64_t)(c + 3) * kWinChunk, mine);
             }
             if (c < n_full) {  // one full chunk left, in o0 (odd count) ...
                 add_group<ABS, PR>(acc, xw, xr, tw, o0, 0xFFu);
-                if (lane < R) o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+                if (lane < R) o0.load(idx, val, hi, lo, tail, mine);
             } else if (n_full > 0 && lane < R) {  // ... or none: the tail is still to load
-                o0.load(idx, val, hi, lo, tail + (int64_t)kWinAlign * lane);
+                o0.load(idx, val, hi, lo, tail, mine);
             }
             if (lane < R) {
                 uint32_t valid = 0;
@@ -447,7 +455,7 @@ __global__ void __launch_bounds__(kTileThreads)
             const int R = (m + kWinAlign - 1) / kWinAlign;
             for (int g = 0; g < R; ++g) {
                 Oct<MODE> o;
-                o.load(idx, val, hi, lo, Ws.o() + (int64_t)kWinAlign * g);
+                o.load(idx, val, hi, lo, Ws.o(), (uint32_t)(kWinAlign * g));
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                     if (g + q * R < m) {
