@@ -5,7 +5,10 @@
 //   start vectors e0, e1 (or the warm-start vector alone), W = H V, P = V^T W;
 //   each iteration: lowest Ritz pair of P, residual r = W y - lambda V y,
 //   converged iff |r| < tol and |lambda - lambda_prev| < etol (or n <= 2);
-//   while the subspace has room: t = r / (lambda - H_ii) (davidson.rs:391-408),
+//   while the subspace has room: t = r / (lambda - H_ii) (davidson.rs:391-408;
+//   a component whose denominator is exactly zero is set to 0, where the
+//   reference's 0/0 would put a NaN into the basis -- e.g. when the Ritz
+//   vector is an exact unit vector of an unconnected determinant),
 //   stop if |t| < 1e-9, orthonormalise t against V (stop if the remainder
 //   is < 1e-9), append t to V and H t to W, extend P; when full, restart from
 //   the Ritz vector (davidson.rs:258-269).
@@ -130,7 +133,8 @@ __global__ void __launch_bounds__(kRedThreads)
                 wv += (W + (size_t)c * ld)[i] * y.c[c];
             }
         const double rv = wv - xv * lam;
-        const double tv = rv / (lam - d[i]);
+        const double den = lam - d[i];
+        const double tv = den != 0.0 ? rv / den : 0.0;
         if (x) x[i] = xv;
         if (r) r[i] = rv;
         t[i] = tv;
@@ -167,7 +171,8 @@ __global__ void __launch_bounds__(kRedThreads)
                 wv += wc[c] * y.c[c];
             }
         const double rv = wv - xv * lam;
-        const double tv = rv / (lam - d[i]);
+        const double den = lam - d[i];
+        const double tv = den != 0.0 ? rv / den : 0.0;
         if (x) x[i] = xv;
         if (r) r[i] = rv;
         t[i] = tv;

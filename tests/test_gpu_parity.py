@@ -362,6 +362,31 @@ def test_two_dets(gpu_ctx):
     assert res.converged
 
 
+@pytest.mark.parametrize("storage,prec", [(STORE_FULL, 64), (STORE_UPPER, 64), (STORE_UPPER, 32), (STORE_TILED, 64),
+                                          (STORE_TILED, 32), (STORE_TILED, 16)])
+def test_unconnected_determinants(gpu_ctx, storage, prec):
+    """Spaces with no stored pair: one determinant, and three that are
+    triple excitations of one another.  Nothing is stored, every product mode
+    gives diag * x exactly, and Davidson returns the lowest diagonal."""
+    from risq_amd.fcidump import synthetic_molecular
+    from risq_amd.hci import SIGMA_F16, SIGMA_FULL
+    ints = synthetic_molecular(8, 8, 3)
+    gpu_ctx.set_integrals(ints)
+    modes = [SIGMA_FULL] + ([SIGMA_F32] if prec <= 32 else []) + ([SIGMA_F16] if prec == 16 else [])
+    for orbs in ([([0, 1, 2], [0, 1, 2])],
+                 [([0, 1, 2], [0, 1, 2]), ([3, 4, 5], [0, 1, 2]), ([0, 1, 2], [3, 4, 5])]):
+        dets = from_orbitals(orbs)
+        diag = gpu_ctx.ham_diag(dets)
+        h = gpu_ctx.build_h(dets, diag, value_prec=prec, storage=storage)
+        assert h.nnz == 0
+        x = np.random.default_rng(5).standard_normal(dets.shape[0])
+        for m in modes:
+            assert np.array_equal(h.matrix_vector_prod(x, mode=m), diag * x), m
+        res = gpu_ctx.davidson(h, tol=1e-8, etol=1e-12, max_iters=50)
+        assert res.converged and res.energy == pytest.approx(float(diag.min()), abs=1e-12)
+        h.free()
+
+
 def test_full_storage_is_deterministic(gpu_ctx, oracle_lib):
     """FULL storage: builds and sigma products are bitwise reproducible."""
     ints, dets = mol_space(16, 32, 0, 4, 4, 20000)
