@@ -351,3 +351,29 @@ def test_external_sampled_capacity_and_device_memory(gpu_ctx):
     assert np.array_equal(od2.to_array(np.float64, k), d)
     raw = osm.to_array(np.uint8, ns * DET_ORB_SAMPLE.itemsize)
     assert np.array_equal(raw.view(DET_ORB_SAMPLE), smp)
+
+
+@pytest.mark.parametrize("variant", SAMPLED)
+def test_external_sampled_nothing_deterministic(gpu_ctx, variant):
+    """eps above every magnitude: no external determinant; the records (none
+    for separate_doubles_and_singles, one per list with a valid external
+    target for skip_singles and semistoch_singles' doubles, one per list for
+    no_singles) equal the oracle's."""
+    ints, dets = mol_space(16, 32, 0, 4, 4, 3000)
+    dets = dets[:200]
+    e0, c = _solve(gpu_ctx, ints, dets)
+    diag = gpu_ctx.ham_diag(dets)
+    gpu_ctx.excite_init(0, 4, 4, 1e-3)
+    w, v, d, smp = gpu_ctx.matmul_external_sampled(variant, dets, c, diag, 1e6)
+    ow, ov, firsts, orec = pt2_oracle.matmul_external_sampled(ints, dets, c, 1e6, 0, 4, 4, variant)
+    assert w.shape[0] == ow.shape[0] == 0
+    assert smp.size == len(orec)
+    if orec:
+        orec = np.array(orec, dtype=object)
+        for f, col in (("det", 0), ("kind", 1), ("is_alpha", 2), ("p", 3), ("q", 4)):
+            assert np.array_equal(smp[f], orec[:, col].astype(np.int64)), f
+        for f, col in (("sum_abs_h", 5), ("sum_h_squared", 6), ("sum_abs_hc", 7), ("sum_hc_squared", 8)):
+            want = orec[:, col].astype(np.float64)
+            assert np.all(np.abs(smp[f] - want) <= 1e-14 * np.abs(want)), f
+    w2, v2, d2, rem = gpu_ctx.matmul_external(dets, c, diag, 1e6)
+    assert w2.shape[0] == 0 and np.all(rem >= 0)
