@@ -345,6 +345,16 @@ __global__ void __launch_bounds__(kTileThreads)
         next_small = 0;
     }
     __syncthreads();
+    // the tile's epilogue terms (diagonal term and the beta-major part's sum of
+    // the same row) are requested now and wait in registers, so their loads,
+    // one of them a dependent gather, overlap the windows: one row per thread
+    static_assert(kTileRows == kTileThreads, "one epilogue row per thread");
+    double pre_d = 0.0, pre_b = 0.0;
+    if (!yb_out && tid < t.nr) {
+        const int64_t p = (int64_t)t.r0 + tid;
+        if (diag && !ABS) pre_d = diag[p] * xdiag[p];  // xdiag: x in FP64
+        if (yb_in) pre_b = yb_in[binv[p]];
+    }
     const int n_w = t.w_split - t.w_begin;
     auto grab = [&]() {
         int k = 0;
@@ -470,16 +480,12 @@ __global__ void __launch_bounds__(kTileThreads)
         }
     }
     __syncthreads();
-    for (int r = tid; r < t.nr; r += kTileThreads) {
-        const int64_t p = (int64_t)t.r0 + r;
-        if (yb_out) {  // beta-major rows, in their own order
-            yb_out[p] = (double)acc[r];
-        } else {  // alpha-major rows, with the beta-major part's sum of the same row
-            const double a = (double)acc[r];
-            double v = (diag && !ABS) ? diag[p] * xdiag[p] + a : a;  // xdiag: x in FP64
-            if (yb_in) v += yb_in[binv[p]];
-            y[p] = v;
-        }
+    if (tid < t.nr) {
+        const int64_t p = (int64_t)t.r0 + tid;
+        if (yb_out)  // beta-major rows, in their own order
+            yb_out[p] = (double)acc[tid];
+        else  // alpha-major rows: d_p x_p + acc_p, then the beta-major sum
+            y[p] = ((diag && !ABS) ? pre_d + (double)acc[tid] : (double)acc[tid]) + (yb_in ? pre_b : 0.0);
     }
 }
 
