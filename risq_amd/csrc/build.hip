@@ -77,7 +77,8 @@ __device__ __forceinline__ bool keep_single(double v, int rule) {
 
 // TILED builds place pairs by pattern; a single the storage rule rejects is
 // written as an explicit zero, which the product adds harmlessly and the
-// download skips (no stored element is ever exactly zero)
+// download skips (no stored element is ever exactly zero).  Singles whose
+// element vanishes for every occupation (DevInts::sok) are not placed at all
 __device__ __forceinline__ double single_or_zero(double v, int rule) { return keep_single(v, rule) ? v : 0.0; }
 
 // the rule of a pair whose larger caller index is m
@@ -366,6 +367,8 @@ __global__ void __launch_bounds__(kBlock)
         const int64_t seg_lo = (int64_t)goff[sb.g_col] * nk, seg_hi = (int64_t)goff[sb.g_col + 1] * nk;
         const S other_row = bm ? ub[sb.g_row] : B.ua[sb.g_row];
         const S ua_col = bm ? other_row : B.ua[sb.g_col];
+        // kind 3: the alpha single between the two groups (same beta string)
+        const bool asingle = sb.kind == 3 && single_possible(T, other_row, ua_col);
         for (int32_t gb = 0; gb < G; gb += 64) {
             const int32_t g = gb + lane;
             int32_t rl = 0, kk = 0, i = 0;
@@ -418,7 +421,7 @@ __global__ void __launch_bounds__(kBlock)
                         const bool same = (da == db);
                         if (sb.kind == 3) {
                             // an equal beta string matches every key: count it once
-                            keep = (same ? kk == 0 : dbl != 0) && !dropped(cl);
+                            keep = (same ? kk == 0 && asingle : dbl != 0) && !dropped(cl);
                             if (FILL && keep) {
                                 S u1 = other_row, u2 = ua_col;
                                 swap_if(j < i, u1, u2);
@@ -428,7 +431,7 @@ __global__ void __launch_bounds__(kBlock)
                             }
                         } else {
                             // strings are unique within a group: equal only on the diagonal
-                            keep = !same && !dropped(cl);
+                            keep = !same && single_possible(T, da, db) && !dropped(cl);
                             if (FILL && keep) {
                                 swap_if(j < i, da, db);
                                 v = single_or_zero(elem_single<S>(T, da, db, other_row), rule_of(rules, max(i, j)));
@@ -452,6 +455,8 @@ __global__ void __launch_bounds__(kBlock)
     const bool bpart = sb.kind == 2;
     const S* rstr = bpart ? B.sb_up : B.sa_dn;
     const int32_t* ridx = bpart ? B.sb_idx : B.sa_idx;
+    // kind 1: the alpha single between the two groups (same beta string)
+    const bool asingle = sb.kind == 1 && single_possible(T, B.ua[sb.g_row], B.ua[sb.g_col]);
     auto pair_val = [&](bool ok, S a, int32_t i, S b, int32_t j, bool& keep, double& v) {
         keep = false;
         v = 0.0;
@@ -460,14 +465,14 @@ __global__ void __launch_bounds__(kBlock)
             if (sb.kind == 2) {
                 // same beta string: alpha doubles, or alpha singles when every
                 // double vanishes (kinds 1 and 3 are then not built)
-                keep = c == (dbl ? 4 : 2);
+                keep = dbl ? c == 4 : (c == 2 && single_possible(T, a, b));
                 if (FILL && keep) {
                     swap_if(j < i, a, b);
                     v = dbl ? elem_same_double<S>(T, a, b)
                             : single_or_zero(elem_single<S>(T, a, b, ub[sb.g_row]), rule_of(rules, max(i, j)));
                 }
             } else if (sb.kind == 0) {
-                keep = c == 2 || (c == 4 && dbl);
+                keep = (c == 2 && single_possible(T, a, b)) || (c == 4 && dbl);
                 if (FILL && keep) {
                     const S up = B.ua[sb.g_row];
                     swap_if(j < i, a, b);
@@ -475,7 +480,7 @@ __global__ void __launch_bounds__(kBlock)
                                : elem_same_double<S>(T, a, b);
                 }
             } else {
-                keep = c == 0 || (c == 2 && dbl);
+                keep = (c == 0 && asingle) || (c == 2 && dbl);
                 if (FILL && keep) {
                     S ua = B.ua[sb.g_row], uc = B.ua[sb.g_col];
                     swap_if(j < i, ua, uc);

@@ -109,6 +109,7 @@ void hci_destroy(hci_ctx* c) {
     hipFree(c->d_h1);
     hipFree(c->d_J3);
     hipFree(c->d_D3);
+    hipFree(c->d_sok);
     for (auto& e : c->events)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -190,11 +191,27 @@ int hci_set_integrals(hci_ctx* c, int n_orb, double e_core, const double* one_bo
                 J3[(i * N + j) * N + k] = ijkk;
                 D3[(i * N + j) * N + k] = ijkk - ikkj;
             }
+    // singles whose element is zero for every occupation (on a lattice: every
+    // hop that is not a bond); symmetric, as h, J3 and D3 are
+    std::vector<uint8_t> sok(N * N, 0);
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) {
+            bool nz = h1[i * N + j] != 0.0;
+            for (size_t k = 0; k < N && !nz; ++k)
+                nz = J3[(i * N + j) * N + k] != 0.0 || D3[(i * N + j) * N + k] != 0.0;
+            if (nz) sok[i * N + j] = sok[j * N + i] = 1;
+        }
+    int all = 1;
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j)
+            if (i != j && !sok[i * N + j]) all = 0;
     hipFree(c->d_two);
     hipFree(c->d_h1);
     hipFree(c->d_J3);
     hipFree(c->d_D3);
+    hipFree(c->d_sok);
     c->d_two = c->d_h1 = c->d_J3 = c->d_D3 = nullptr;
+    c->d_sok = nullptr;
     HIP_TRY(c, hipMalloc(&c->d_two, need2 * sizeof(double)));
     HIP_TRY(c, hipMalloc(&c->d_h1, N * N * sizeof(double)));
     HIP_TRY(c, hipMalloc(&c->d_J3, N * N * N * sizeof(double)));
@@ -203,6 +220,9 @@ int hci_set_integrals(hci_ctx* c, int n_orb, double e_core, const double* one_bo
     HIP_TRY(c, hipMemcpy(c->d_h1, h1.data(), N * N * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_J3, J3.data(), N * N * N * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_D3, D3.data(), N * N * N * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMalloc(&c->d_sok, N * N));
+    HIP_TRY(c, hipMemcpy(c->d_sok, sok.data(), N * N, hipMemcpyHostToDevice));
+    c->sok_all = all;
     // double excitations only read (pr|qs) with p != r and q != s
     bool vanish = true;
     for (size_t i = 1; i <= N && vanish; ++i)

@@ -50,6 +50,10 @@ struct DevInts {
     const double* h1;
     const double* J3;
     const double* D3;
+    // sok[i * n + j] = 0: the single excitation i -> j has a zero element for
+    // every occupation (h_ij and every (ij|kk), (ij|kk) - (ik|kj) are zero)
+    const uint8_t* sok;
+    int sok_all;  // every single can be nonzero (sok is then not read)
 };
 
 // combine_2 on positive 1-based indices (src/utils/ints.rs:38-46)
@@ -82,6 +86,14 @@ __device__ __forceinline__ double elem_single(const DevInts& T, S d1s, S d2s, S 
         m &= m - (S)1;
     }
     return out * (double)permute(d1s, d2s);
+}
+
+// false when the single excitation from -> to (strings of the changed spin)
+// has a zero element whatever the other electrons: ham_sing then sums only
+// zero terms, so neither storage rule ever keeps the pair
+template <class S>
+__device__ __forceinline__ bool single_possible(const DevInts& T, S from, S to) {
+    return T.sok_all || T.sok[ctz(from & ~to) * T.n + ctz(to & ~from)] != 0;
 }
 
 // Ham::ham_doub, same-spin branch (src/ham/mod.rs:180-207): a/b are the

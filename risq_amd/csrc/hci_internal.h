@@ -107,6 +107,8 @@ struct hci_ctx {
     double* d_h1 = nullptr;
     double* d_J3 = nullptr;
     double* d_D3 = nullptr;
+    uint8_t* d_sok = nullptr;  // DevInts::sok
+    int sok_all = 0;
     size_t n_two = 0;
     bool doubles_vanish = false;  // every (pq|rs) with p != q and r != s is zero (Hubbard)
     std::vector<double> h_one, h_two;  // host copies of the packed tables (selection lists)
@@ -138,6 +140,8 @@ struct hci_ctx {
         t.h1 = d_h1;
         t.J3 = d_J3;
         t.D3 = d_D3;
+        t.sok = d_sok;
+        t.sok_all = sok_all;
         return t;
     }
 };

@@ -101,8 +101,8 @@ def test_build_u128_strings(gpu_ctx, oracle_lib):
 def test_build_hubbard_doubles_vanish(gpu_ctx, oracle_lib, storage):
     """Hubbard: every double is exactly zero and dropped; singles are +-t hops.
     Hops between non-adjacent sites are exact zeros that the storage rule drops
-    (the tiled layout places them as explicit zeros, which must not show up in
-    the pattern, the count or the product)."""
+    (the tiled layout does not place them at all: their element vanishes for
+    every occupation)."""
     ints, dets = hubbard_space(3, 3, 4, 4, 2000)
     gpu_ctx.set_integrals(ints)
     diag = oracle_lib.OracleHam(ints).ham_diag(dets)
@@ -608,6 +608,39 @@ def test_tiled_hubbard_large_groups(gpu_ctx, oracle_lib, prec):
     for mode in ([0, 1] if prec == 32 else [0]):
         a, b = f.matrix_vector_prod(x, mode=mode), t.matrix_vector_prod(x, mode=mode)
         assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
+
+
+def test_tiled_skips_singles_that_always_vanish(gpu_ctx, oracle_lib):
+    """A single excitation whose element is zero for every occupation (on the
+    lattice: a hop that is not a bond) takes no slot in tiled storage.  The
+    same lattice with every non-bond h_pq set to 1e-14 (never above the
+    1e-12 rule, but no longer identically zero) stores the same pairs, and
+    its tiled matrix holds those hops as explicit zeros: more slots, the same
+    download and products."""
+    from risq_amd.fcidump import Integrals, combine_2
+    ints, dets = hubbard_space(4, 4, 5, 5, 20000)
+    one = ints.one_body.copy()
+    for p in range(ints.n_orb):
+        for q in range(p):
+            if one[combine_2(p + 1, q + 1)] == 0.0:
+                one[combine_2(p + 1, q + 1)] = 1e-14
+    tiny = Integrals(ints.n_orb, ints.e_core, one, ints.two_body)
+    x = np.random.default_rng(11).standard_normal(dets.shape[0])
+    got = []
+    for I in (ints, tiny):
+        gpu_ctx.set_integrals(I)
+        diag = gpu_ctx.ham_diag(dets)
+        t = gpu_ctx.build_h(dets, diag, storage=STORE_TILED)
+        S = oracle_lib.OracleSparse().build(oracle_lib.OracleHam(I), dets, diag)
+        for u, v in zip(t.download(), S.export()):
+            assert np.array_equal(u, v)
+        got.append((t.nnz, t.build_stats["stored_entries"], t.matrix_vector_prod(x)))
+        t.free()
+    (n0, s0, y0), (n1, s1, y1) = got
+    # the non-bond hops among these determinants are many: each electron
+    # has 4 bonds out of 15 possible hops
+    assert n0 == n1 and s1 - s0 > 0.1 * 2 * n0, (n0, n1, s0, s1)
+    assert np.max(np.abs(y0 - y1)) < 1e-12 * np.max(np.abs(y0))
 
 
 @pytest.mark.parametrize("system", ["mol", "u128", "hubbard"])
