@@ -21,9 +21,9 @@ root, residual 1e-4 and energy change 1e-8).  Workloads (--config):
                1.6e7 does not fit).
 
 Timed path (value): tiled storage with FP32 hi+lo values (plus an FP16 copy of
-each value for workload 1: --values), Davidson with FP16 sigmas, then FP32
-sigmas from 3x the tolerance on, then FP64 refinement (FP32 then FP64 for
-workloads 2 and 4), thick restarts (workload 1: subspace 16, 4 vectors kept;
+each value for workloads 1 and 2: --values), Davidson with FP16 sigmas, then
+FP32 sigmas from 3x the tolerance on, then FP64 refinement (FP32 then FP64 for
+workload 4), thick restarts (workload 1: subspace 16, 4 vectors kept;
 others: subspace 11, 2 kept; --max-sub, --keep), and the correction chosen by
 --precond (learned for workload 1, diagonal otherwise).  Reported beside it from the same run: the
 plain FP64 GPU path (FP64 values, the reference's restart-to-one-vector control
