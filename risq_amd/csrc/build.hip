@@ -945,12 +945,22 @@ __global__ void k_invert_perm(const int32_t* __restrict__ perm, int64_t n, int32
     if (q < n) inv[perm[q]] = (int32_t)q;
 }
 
-__global__ void k_task_work(const TileTask* t, int64_t n, const Window* __restrict__ wins, int64_t* work) {
+// sort key of each tile: heavier tiles first (an ascending sort of
+// INT64_MAX - work), ties in position order (the sort is stable)
+__global__ void k_task_work(const TileTask* t, int64_t n, const Window* __restrict__ wins, uint64_t* key,
+                            int32_t* ord) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int64_t w = 0;
     for (int32_t i = t[k].w_begin; i < t[k].w_end; ++i) w += wins[i].cnt();
-    work[k] = w + t[k].nr;  // rows cost their write even without entries
+    w += t[k].nr;  // rows cost their write even without entries
+    key[k] = (uint64_t)(INT64_MAX - w);
+    ord[k] = (int32_t)k;
+}
+
+__global__ void k_gather_tasks(const TileTask* t, const int32_t* ord, int64_t n, TileTask* out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = t[ord[k]];
 }
 
 // Stable partition of each tile's windows into those of more than kSmallWin
@@ -1833,22 +1843,21 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     // workload 3.125 -> 3.054 ms, FP32 product on the 6x6 lattice 0.905 ->
     // 0.878 ms)
     if (ta.size() >= 2) {
-        std::vector<TileTask>& tv = ta;
         TileTask* d_t = H->d_task_a;
-        const int64_t nt = (int64_t)tv.size();
-        int64_t* d_work = nullptr;
-        HIP_TRY(c, wb.alloc(&d_work, (size_t)nt));
-        hipLaunchKernelGGL(k_task_work, grid_for(nt, 256), 256, 0, s, d_t, nt, H->d_win, d_work);
-        std::vector<int64_t> work((size_t)nt);
-        HIP_TRY(c, hipMemcpyAsync(work.data(), d_work, (size_t)nt * 8, hipMemcpyDeviceToHost, s));
-        HIP_TRY(c, hipMemcpyAsync(tv.data(), d_t, (size_t)nt * sizeof(TileTask), hipMemcpyDeviceToHost, s));
-        HIP_TRY(c, hipStreamSynchronize(s));
-        std::vector<int64_t> order((size_t)nt);
-        for (int64_t k = 0; k < nt; ++k) order[(size_t)k] = k;
-        std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return work[(size_t)a] > work[(size_t)b]; });
-        std::vector<TileTask> sorted((size_t)nt);
-        for (int64_t k = 0; k < nt; ++k) sorted[(size_t)k] = tv[(size_t)order[(size_t)k]];
-        HIP_TRY(c, hipMemcpy(d_t, sorted.data(), (size_t)nt * sizeof(TileTask), hipMemcpyHostToDevice));
+        const int64_t nt = (int64_t)ta.size();
+        // keys, indices and the sorted tasks in one allocation
+        const size_t nk = ((size_t)nt + 63) / 64 * 64;
+        uint64_t* d_key = nullptr;
+        HIP_TRY(c, wb.alloc(&d_key, nk * 2 + nk + (nk * sizeof(TileTask) + 7) / 8));
+        uint64_t* d_key2 = d_key + nk;
+        int32_t* d_ord = (int32_t*)(d_key2 + nk);
+        int32_t* d_ord2 = d_ord + nk;
+        TileTask* d_sorted = (TileTask*)(d_ord2 + nk);
+        hipLaunchKernelGGL(k_task_work, grid_for(nt, 256), 256, 0, s, d_t, nt, H->d_win, d_key, d_ord);
+        HIP_TRY(c, sort_pairs(tmp, d_key, d_key2, d_ord, d_ord2, (size_t)nt, 64, s));
+        hipLaunchKernelGGL(k_gather_tasks, grid_for(nt, 256), 256, 0, s, d_t, d_ord2, nt, d_sorted);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(d_t, d_sorted, (size_t)nt * sizeof(TileTask), hipMemcpyDeviceToDevice, s));
     }
     HIP_TRY(c, hipMalloc(&H->d_bpos, (size_t)N * 4 + 16));
     HIP_TRY(c, hipMemcpyAsync(H->d_bpos, B.sb_pos, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
