@@ -295,7 +295,10 @@ __global__ void __launch_bounds__(kBlock)
 // per-window totals; the fill pass stores each pair at its window's offset +
 // position.  Positions within a window are a permutation of its slots; the
 // product does not depend on their order.
-template <class S, bool FILL>
+// SOK: some singles vanish for every occupation and are looked up in
+// DevInts::sok (instantiated apart: the lookup costs registers that the
+// builds of integrals where every single can be nonzero do not pay)
+template <class S, bool FILL, bool SOK>
 __global__ void __launch_bounds__(kBlock)
     k_tile_connect(const SubBlock* __restrict__ sbs, int64_t n_sb, BuildView<S> B, const S* ub, DevInts T,
                    SingleRules rules, int dbl, int64_t* win_cnt, TileOut out, unsigned long long* nnz_out,
@@ -306,6 +309,7 @@ __global__ void __launch_bounds__(kBlock)
     const int64_t w = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
     if (w >= n_sb) return;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    auto single_ok = [&](S from, S to) { return !SOK || single_possible(T, from, to); };
     const SubBlock sb = sbs[w];
     const int n_slices = (sb.nc + kWinCols - 1) / kWinCols;
     if (FILL && pairf) {  // every window held by its mirror: nothing to search
@@ -368,7 +372,7 @@ __global__ void __launch_bounds__(kBlock)
         const S other_row = bm ? ub[sb.g_row] : B.ua[sb.g_row];
         const S ua_col = bm ? other_row : B.ua[sb.g_col];
         // kind 3: the alpha single between the two groups (same beta string)
-        const bool asingle = sb.kind == 3 && single_possible(T, other_row, ua_col);
+        const bool asingle = sb.kind == 3 && single_ok(other_row, ua_col);
         for (int32_t gb = 0; gb < G; gb += 64) {
             const int32_t g = gb + lane;
             int32_t rl = 0, kk = 0, i = 0;
@@ -431,7 +435,7 @@ __global__ void __launch_bounds__(kBlock)
                             }
                         } else {
                             // strings are unique within a group: equal only on the diagonal
-                            keep = !same && single_possible(T, da, db) && !dropped(cl);
+                            keep = !same && single_ok(da, db) && !dropped(cl);
                             if (FILL && keep) {
                                 swap_if(j < i, da, db);
                                 v = single_or_zero(elem_single<S>(T, da, db, other_row), rule_of(rules, max(i, j)));
@@ -456,7 +460,7 @@ __global__ void __launch_bounds__(kBlock)
     const S* rstr = bpart ? B.sb_up : B.sa_dn;
     const int32_t* ridx = bpart ? B.sb_idx : B.sa_idx;
     // kind 1: the alpha single between the two groups (same beta string)
-    const bool asingle = sb.kind == 1 && single_possible(T, B.ua[sb.g_row], B.ua[sb.g_col]);
+    const bool asingle = sb.kind == 1 && single_ok(B.ua[sb.g_row], B.ua[sb.g_col]);
     auto pair_val = [&](bool ok, S a, int32_t i, S b, int32_t j, bool& keep, double& v) {
         keep = false;
         v = 0.0;
@@ -465,14 +469,14 @@ __global__ void __launch_bounds__(kBlock)
             if (sb.kind == 2) {
                 // same beta string: alpha doubles, or alpha singles when every
                 // double vanishes (kinds 1 and 3 are then not built)
-                keep = dbl ? c == 4 : (c == 2 && single_possible(T, a, b));
+                keep = dbl ? c == 4 : (c == 2 && single_ok(a, b));
                 if (FILL && keep) {
                     swap_if(j < i, a, b);
                     v = dbl ? elem_same_double<S>(T, a, b)
                             : single_or_zero(elem_single<S>(T, a, b, ub[sb.g_row]), rule_of(rules, max(i, j)));
                 }
             } else if (sb.kind == 0) {
-                keep = (c == 2 && single_possible(T, a, b)) || (c == 4 && dbl);
+                keep = (c == 2 && single_ok(a, b)) || (c == 4 && dbl);
                 if (FILL && keep) {
                     const S up = B.ua[sb.g_row];
                     swap_if(j < i, a, b);
@@ -1645,8 +1649,12 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     TileOut none{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (n_sb > 0) {
         hipLaunchKernelGGL(k_win_init, grid_for(n_sb, 256), 256, 0, s, d_sb, n_sb, d_wc0, d_wnc);
-        hipLaunchKernelGGL((k_tile_connect<S, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           rules, dbl, d_wc, none, nullptr, nullptr);
+        if (T.sok_all)
+            hipLaunchKernelGGL((k_tile_connect<S, false, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B,
+                               ub, T, rules, dbl, d_wc, none, nullptr, nullptr);
+        else
+            hipLaunchKernelGGL((k_tile_connect<S, false, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B,
+                               ub, T, rules, dbl, d_wc, none, nullptr, nullptr);
     }
     // mirror pairs: dense windows whose mirror is another tile's window; the
     // kept halves are stored alone, numbered (their partial-sum slots) and
@@ -1771,9 +1779,14 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     recycled_flush(c);
     TileOut fo{H->d_idx16, H->d_val, H->d_hi, H->d_lo, d_woff};
     HIP_TRY(c, hipMalloc(&H->d_win, (size_t)n_win * sizeof(Window) + 16));
-    if (n_sb > 0)
-        hipLaunchKernelGGL((k_tile_connect<S, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B, ub, T,
-                           rules, dbl, nullptr, fo, (unsigned long long*)d_nnz, d_pairf);
+    if (n_sb > 0) {
+        if (T.sok_all)
+            hipLaunchKernelGGL((k_tile_connect<S, true, false>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B,
+                               ub, T, rules, dbl, nullptr, fo, (unsigned long long*)d_nnz, d_pairf);
+        else
+            hipLaunchKernelGGL((k_tile_connect<S, true, true>), grid_for(n_sb, kWaves), kBlock, 0, s, d_sb, n_sb, B,
+                               ub, T, rules, dbl, nullptr, fo, (unsigned long long*)d_nnz, d_pairf);
+    }
     hipLaunchKernelGGL(k_win_compact, grid_for(n_wall, 256), 256, 0, s, d_wc, d_woff, d_nidx, d_wc0, d_wnc, n_wall,
                        d_pairf, d_pslot, H->d_win);
     if (H->n_pair > 0) {

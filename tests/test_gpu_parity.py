@@ -610,15 +610,17 @@ def test_tiled_hubbard_large_groups(gpu_ctx, oracle_lib, prec):
         assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
 
 
-def test_tiled_skips_singles_that_always_vanish(gpu_ctx, oracle_lib):
+@pytest.mark.parametrize("lattice", [(4, 4, 5, 5, 20000), (9, 8, 3, 3, 6000)])
+def test_tiled_skips_singles_that_always_vanish(gpu_ctx, oracle_lib, lattice):
     """A single excitation whose element is zero for every occupation (on the
     lattice: a hop that is not a bond) takes no slot in tiled storage.  The
     same lattice with every non-bond h_pq set to 1e-14 (never above the
     1e-12 rule, but no longer identically zero) stores the same pairs, and
     its tiled matrix holds those hops as explicit zeros: more slots, the same
-    download and products."""
+    download and products.  One-word (16 sites) and two-word (72 sites)
+    strings."""
     from risq_amd.fcidump import Integrals, combine_2
-    ints, dets = hubbard_space(4, 4, 5, 5, 20000)
+    ints, dets = hubbard_space(*lattice)
     one = ints.one_body.copy()
     for p in range(ints.n_orb):
         for q in range(p):
