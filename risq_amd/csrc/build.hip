@@ -971,21 +971,22 @@ __global__ void k_gather_tasks(const TileTask* t, const int32_t* ord, int64_t n,
 // entries, then the rest; sets w_split.  One wavefront per tile.
 constexpr int kPartWaves = 4;
 __global__ void __launch_bounds__(64 * kPartWaves)
-    k_win_partition(TileTask* tasks, int64_t n_task, const Window* __restrict__ in, Window* __restrict__ out) {
+    k_win_partition(TileTask* tasks, int64_t n_task, const Window* __restrict__ in, Window* __restrict__ out,
+                    int small_win) {
     const int lane = threadIdx.x & 63;
     const int64_t k = (int64_t)blockIdx.x * kPartWaves + (threadIdx.x >> 6);
     if (k >= n_task) return;
     const int32_t b = tasks[k].w_begin, e = tasks[k].w_end;
     int32_t n_big = 0;
     for (int32_t w = b; w < e; w += 64)
-        n_big += __popcll(__ballot(w + lane < e && in[w + lane].cnt() > kSmallWin));
+        n_big += __popcll(__ballot(w + lane < e && in[w + lane].cnt() > small_win));
     const uint64_t below = (1ull << lane) - 1;
     int32_t pb = b, ps = b + n_big;
     for (int32_t w = b; w < e; w += 64) {
         const bool valid = w + lane < e;
         Window W{};
         if (valid) W = in[w + lane];
-        const bool big = valid && W.cnt() > kSmallWin;
+        const bool big = valid && W.cnt() > small_win;
         const uint64_t mb = __ballot(big), ms = __ballot(valid && !big);
         if (big) out[pb + __popcll(mb & below)] = W;
         if (valid && !big) out[ps + __popcll(ms & below)] = W;
@@ -1835,15 +1836,19 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
                            (int64_t)tb.size(), d_nidx);
     HIP_TRY(c, hipGetLastError());
     {
-        // every window lies in the range of exactly one tile of one part
+        // every window lies in the range of exactly one tile of one part.  The
+        // windows taken one per lane: at most kSmallWin entries, or
+        // kSmallWinDense when rows hold many entries (HCI-selected spaces,
+        // whose many small groups leave many windows of a few dozen entries)
+        const int small_win = (int64_t)cap >= kDenseRowSlots * N ? kSmallWinDense : kSmallWin;
         Window* parted = nullptr;
         HIP_TRY(c, wb.alloc(&parted, (size_t)n_win + 1));
         if (!ta.empty())
             hipLaunchKernelGGL(k_win_partition, grid_for((int64_t)ta.size(), kPartWaves), 64 * kPartWaves, 0, s,
-                               H->d_task_a, (int64_t)ta.size(), H->d_win, parted);
+                               H->d_task_a, (int64_t)ta.size(), H->d_win, parted, small_win);
         if (!tb.empty())
             hipLaunchKernelGGL(k_win_partition, grid_for((int64_t)tb.size(), kPartWaves), 64 * kPartWaves, 0, s,
-                               H->d_task_b, (int64_t)tb.size(), H->d_win, parted);
+                               H->d_task_b, (int64_t)tb.size(), H->d_win, parted, small_win);
         HIP_TRY(c, hipGetLastError());
         if (n_win > 0)
             HIP_TRY(c, hipMemcpyAsync(H->d_win, parted, (size_t)n_win * sizeof(Window), hipMemcpyDeviceToDevice, s));

@@ -81,6 +81,10 @@ __host__ __device__ inline int64_t win_slot(int64_t j, int64_t cnt) {
 // more than kSmallWin entries come first and are taken one per wavefront,
 // those from w_split on are taken one per lane
 constexpr int kSmallWin = 8;
+// ... or kSmallWinDense entries in matrices of at least kDenseRowSlots
+// stored slots per row
+constexpr int kSmallWinDense = 32;
+constexpr int64_t kDenseRowSlots = 64;
 struct TileTask {
     int32_t r0, nr, w_begin, w_end, w_split;
 };

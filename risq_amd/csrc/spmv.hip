@@ -451,8 +451,9 @@ __global__ void __launch_bounds__(kTileThreads)
         W = Wn;
         k = kn;
     }
-    // windows of at most kSmallWin entries, one per lane: x is read where an
-    // entry needs it instead of staging the window's slice
+    // the tile's small windows (at most kSmallWin or kSmallWinDense entries,
+    // chosen at build time), one per lane: x is read where an entry needs it
+    // instead of staging the window's slice
     const int n_s = t.w_end - t.w_split;
     for (;;) {
         int kb = 0;
