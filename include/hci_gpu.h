@@ -116,7 +116,9 @@ typedef struct hci_build_stats {
     double seconds;         /* wall time of the call (device work included) */
     int64_t mirror_pairs;   /* HCI_STORE_TILED: dense window pairs stored as one half */
     int64_t stored_entries; /* entry slots held on the device (both triangles for FULL
-                               and TILED, TILED's alignment slots included) */
+                               and TILED; TILED's alignment slots included, and its
+                               explicit zeros: singles the storage rule rejects whose
+                               element is not zero for every occupation) */
     int64_t matrix_bytes;   /* device bytes of the entry arrays: indices and values
                                (FP64, FP32 hi + lo, FP16 copies), plus FULL/UPPER's
                                row pointers */
