@@ -1225,24 +1225,31 @@ struct Scratch {
     ~Scratch() { hipFree(p); }
 };
 
+// a build's work buffers, from the context's cache of them (scratch_take /
+// scratch_give): released when the object goes out of scope or by release()
 struct DevBufs {
-    std::vector<void*> ptrs;
+    hci_ctx* c;
+    std::vector<std::pair<void*, size_t>> ptrs;
+    explicit DevBufs(hci_ctx* ctx) : c(ctx) {}
+    DevBufs(const DevBufs&) = delete;
+    DevBufs& operator=(const DevBufs&) = delete;
     template <class T>
     hipError_t alloc(T** p, size_t count) {
         *p = nullptr;
-        hipError_t e = hipMalloc((void**)p, count * sizeof(T) + 16);
-        if (e == hipSuccess) ptrs.push_back((void*)*p);
+        const size_t bytes = count * sizeof(T) + 16;
+        hipError_t e = scratch_take(c, (void**)p, bytes);
+        if (e == hipSuccess) ptrs.push_back({(void*)*p, bytes});
         return e;
     }
     void release(void* p) {
         for (auto& q : ptrs)
-            if (q == p) {
-                hipFree(q);
-                q = nullptr;
+            if (q.first == p) {
+                scratch_give(c, q.first, q.second);
+                q.first = nullptr;
             }
     }
     ~DevBufs() {
-        for (void* p : ptrs) hipFree(p);
+        for (auto& q : ptrs) scratch_give(c, q.first, q.second);
     }
 };
 
@@ -1461,7 +1468,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         HIP_TRY(c, hipMemcpy(w, B.sa_dn, sizeof(S) <= 8 ? 8 : 16, hipMemcpyDeviceToHost));
         nbk = __builtin_popcountll(w[0]) + (sizeof(S) > 8 ? __builtin_popcountll(w[1]) : 0);
     }
-    DevBufs kb;
+    DevBufs kb(c);
     const int64_t NK = N * (int64_t)std::max(nbk, 1);
     S *k_in = nullptr, *k_out = nullptr;
     int32_t *q_in = nullptr, *q_out = nullptr, *seg = nullptr;
@@ -1539,7 +1546,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
     HIP_TRY(c, hipMemcpy(gb.data(), B.gb_off, gb.size() * 4, hipMemcpyDeviceToHost));
     std::vector<TileTask> ta, tb;
     int64_t n_wall = 0, n_sb = 0;
-    DevBufs gbufs;
+    DevBufs gbufs(c);
     struct PartGen {
         int part;
         int32_t U;
@@ -1619,7 +1626,7 @@ int build_tiled(hci_ctx* c, Scratch& tmp, BuildView<S> B, const S* ub, int32_t U
         HIP_TRY(c, hipMemcpy(P.d_rb, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
     }
     if (n_wall >= (int64_t)INT32_MAX) return hci_fail(c, HCI_ERR_ARG, "hci_build_h: too many tiles");
-    DevBufs wb;
+    DevBufs wb(c);
     SubBlock* d_sb = nullptr;
     int64_t *d_wc = nullptr, *d_sz = nullptr, *d_woff = nullptr, *d_sum = nullptr, *d_nnz = nullptr;
     int32_t *d_flag = nullptr, *d_nidx = nullptr, *d_wc0 = nullptr, *d_wnc = nullptr;
@@ -1902,7 +1909,7 @@ int build_impl(hci_ctx* c, const uint64_t* dets_dev, const double* diag_dev, int
                hci_csr** out, hci_build_stats* stats) {
     hipStream_t s = c->stream;
     Scratch tmp;
-    DevBufs db;
+    DevBufs db(c);
     const int n_orb = c->n_orb;
     const bool full = (o.storage == HCI_STORE_FULL);
 

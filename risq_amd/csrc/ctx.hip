@@ -61,6 +61,7 @@ hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes) {
         return hipSuccess;
     }
     recycled_flush(c);  // nothing fits: give the memory back before allocating
+    scratch_flush(c);
     hipError_t e = hipMalloc(p, bytes);
     if (e == hipErrorOutOfMemory && c->work_dev) {
         (void)hipGetLastError();
@@ -73,6 +74,43 @@ hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes) {
 void recycled_flush(hci_ctx* c) {
     for (auto& r : c->recycled) hipFree(r.p);
     c->recycled.clear();
+}
+
+hipError_t scratch_take(hci_ctx* c, void** p, size_t bytes) {
+    int best = -1;
+    for (int i = 0; i < (int)c->scratch_cache.size(); ++i) {
+        const size_t b = c->scratch_cache[i].bytes;
+        if (b >= bytes && b - bytes <= bytes / 4 && (best < 0 || b < c->scratch_cache[best].bytes)) best = i;
+    }
+    if (best >= 0) {
+        *p = c->scratch_cache[best].p;
+        c->scratch_cached -= c->scratch_cache[best].bytes;
+        c->scratch_cache.erase(c->scratch_cache.begin() + best);
+        return hipSuccess;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory && !c->scratch_cache.empty()) {
+        (void)hipGetLastError();
+        scratch_flush(c);
+        e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+
+void scratch_give(hci_ctx* c, void* p, size_t bytes) {
+    if (!p) return;
+    if (bytes <= kScratchCacheBuf && c->scratch_cached + bytes <= kScratchCacheMax) {
+        c->scratch_cache.push_back({p, bytes});
+        c->scratch_cached += bytes;
+    } else {
+        hipFree(p);
+    }
+}
+
+void scratch_flush(hci_ctx* c) {
+    for (auto& r : c->scratch_cache) hipFree(r.p);
+    c->scratch_cache.clear();
+    c->scratch_cached = 0;
 }
 
 extern "C" {
@@ -102,6 +140,7 @@ void hci_destroy(hci_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     recycled_flush(c);
+    scratch_flush(c);
     work_release(c);
     mlp_free_state(c);
     select_free(c);

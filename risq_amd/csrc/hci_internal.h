@@ -127,6 +127,11 @@ struct hci_ctx {
         size_t bytes;
     };
     std::vector<Recycled> recycled;
+    // the builds' work buffers of at most kScratchCacheBuf bytes, kept for the
+    // next build (up to kScratchCacheMax bytes in all): a build allocates and
+    // frees ~140 of them, and every hipFree waits for the device
+    std::vector<Recycled> scratch_cache;
+    size_t scratch_cached = 0;
     // Davidson and sigma work vectors, kept between calls (davidson.hip):
     // one device block and one pinned host block, grown on demand
     void* work_dev = nullptr;
@@ -214,6 +219,15 @@ struct hci_csr {
 // least `bytes` or allocates; recycled_flush frees what is left
 hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes);
 void recycled_flush(hci_ctx* c);
+// build work buffers (ctx.hip): scratch_take hands out a kept buffer of
+// bytes .. 5/4 bytes or allocates (on the context's stream, whose order makes
+// a kept buffer free for reuse); scratch_give keeps or frees one;
+// scratch_flush frees them all
+constexpr size_t kScratchCacheBuf = (size_t)128 << 20;
+constexpr size_t kScratchCacheMax = (size_t)1 << 30;
+hipError_t scratch_take(hci_ctx* c, void** p, size_t bytes);
+void scratch_give(hci_ctx* c, void* p, size_t bytes);
+void scratch_flush(hci_ctx* c);
 // the context's Davidson / sigma work block (davidson.hip): work_reserve
 // returns a device block of at least dev_bytes and a pinned host block of at
 // least host_bytes, reallocating only when they grow; work_release frees both
