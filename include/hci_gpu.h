@@ -167,6 +167,8 @@ typedef struct hci_dav_stats {
 
 /* context ---------------------------------------------------------------- */
 int hci_create(int device, hci_ctx** out);
+/* frees the context's integral tables, work vectors, recycled entry arrays
+ * and the build work buffers it keeps between builds (at most 1 GB) */
 void hci_destroy(hci_ctx* ctx);
 const char* hci_last_error(const hci_ctx* ctx);
 int hci_device_sync(hci_ctx* ctx);
