@@ -61,8 +61,12 @@ hipError_t recycled_take(hci_ctx* c, void** p, size_t bytes) {
         return hipSuccess;
     }
     recycled_flush(c);  // nothing fits: give the memory back before allocating
-    scratch_flush(c);
     hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory && !c->scratch_cache.empty()) {
+        (void)hipGetLastError();
+        scratch_flush(c);  // the kept build work buffers
+        e = hipMalloc(p, bytes);
+    }
     if (e == hipErrorOutOfMemory && c->work_dev) {
         (void)hipGetLastError();
         work_release(c);  // the Davidson work vectors are cheaper to allocate again
