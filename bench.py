@@ -124,9 +124,18 @@ def hci_space_run(ctx, ints, failures) -> dict:
                          max_sub=max_sub)
         ctx.sync()
         tt = time.perf_counter() - ts
-        res["nnz_u"] = h.nnz
+        nnz, nd = h.nnz, words.shape[0]
+        res["nnz_u"] = nnz
         h.free(recycle=True)
-        res[name] = {"s_to_E0": round(tt, 4), "build_s": round(tb, 4), "E0": d.energy, "iters": d.stats["iters"],
+        st = d.stats
+        n16, n32, nfull = st["n_sigma_f16"], st["n_sigma_f32"], st["n_sigma_full"]
+        # full-precision products read 8 bytes per value (FP64, or FP32 hi + lo)
+        alg = (n16 * spmv_bytes(nnz, nd, 2) + n32 * spmv_bytes(nnz, nd, 4) + nfull * spmv_bytes(nnz, nd, 8))
+        frac = alg / (st["sigma_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS if st["sigma_ms"] > 0 else None
+        res[name] = {"s_to_E0": round(tt, 4), "build_s": round(tb, 4), "E0": d.energy, "iters": st["iters"],
+                     "products": {"f16": n16, "f32": n32, "full": nfull},
+                     "sigma_ms_per_product": round(st["sigma_ms"] / max(1, n16 + n32 + nfull), 3),
+                     "sigma_roofline_frac": round(frac, 4) if frac is not None else None,
                      "converged": bool(d.converged)}
         log(f"hci space {name}: {tt:.3f}s build {tb:.3f}s E0={d.energy:.10f} {d.stats}")
         if not d.converged:
